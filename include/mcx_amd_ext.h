@@ -1,0 +1,49 @@
+/*
+ * mcx_amd_ext.h -- extensions of the MI355X engine beyond the reference ABI.
+ *
+ * None of these has a counterpart in the reference; they are library-wide
+ * switches and statistics that the reference's Config cannot carry without
+ * changing its layout.  Plain C, no HIP types.
+ */
+#ifndef MCX_AMD_EXT_H
+#define MCX_AMD_EXT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Fluence tally precision used by subsequent mcx_run_simulation calls:
+ *  0  FP32 atomics with the reference's MAX_ACCUM shadow half (default,
+ *     reference semantics, src/mcx_core.cu:2246-2253)
+ *  1  FP32 hi + FP32 lo compensated (TwoSum) tally, merged in FP64
+ *  2  FP64 atomics (accuracy reference)
+ * Returns the previous mode, or -1 for an invalid mode. */
+int  mcx_amd_set_tally_mode(int mode);
+
+/* Collect per-run work counters (steps, deposits, scatters, launches) in
+ * subsequent runs; costs one atomic per wave at lane exit. */
+void mcx_amd_set_counters(int enable);
+
+/* Statistics of the last mcx_run_simulation call on this process:
+ *  out[0] kernel time in ms (HIP events, sum over launches)
+ *  out[1] launch+sync wall time in ms (the reference's runtime definition)
+ *  out[2] steps  out[3] deposits  out[4] scatters  out[5] launches
+ *  out[6] vector registers per lane  out[7] lanes launched
+ *  out[8] workgroup size  out[9] kernel launches
+ *  out[10] lanes that hit the "should never happen" exit
+ * Returns the number of values written (<= n). */
+int  mcx_amd_last_stats(double* out, int n);
+
+/* sizeof/offsetof values of the ABI structs as compiled into the library:
+ * sizeof(Config), sizeof(GPUInfo), sizeof(History), then the offsets of
+ * Config.srcpos, .session, .flog, .his, .exportfield, .srcparam1,
+ * .deviceid, .energytot, .bc, .srcdata.  Returns the count available. */
+int  mcx_amd_abi_layout(size_t* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

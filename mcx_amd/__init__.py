@@ -1,0 +1,26 @@
+"""mcx_amd -- MI355X-native Monte Carlo photon transport (drop-in for MCX's
+mcx_run_simulation path).
+
+The engine is libmcx_amd.so (HIP kernel + C++ host layer, C ABI declared in
+include/mcx_abi.h).  This package exposes the pmcx-style dictionary API.
+"""
+from .pmcx import gpuinfo, run, version  # noqa: F401
+from . import _lib  # noqa: F401
+
+
+def set_tally_mode(mode):
+    """0: FP32 + MAX_ACCUM shadow (reference), 1: FP32 hi/lo compensated, 2: FP64."""
+    names = {"f32": 0, "shadow": 0, "hilo": 1, "f64": 2}
+    m = names.get(mode, mode) if isinstance(mode, str) else int(mode)
+    old = _lib.lib().mcx_amd_set_tally_mode(m)
+    if old < 0:
+        raise ValueError("invalid tally mode %r" % (mode,))
+    return old
+
+
+def set_counters(enable):
+    _lib.lib().mcx_amd_set_counters(1 if enable else 0)
+
+
+def last_stats():
+    return _lib.last_stats()

@@ -1,0 +1,191 @@
+"""ctypes binding of libmcx_amd.so (the C-ABI boundary).
+
+The Structure classes mirror include/mcx_abi.h field for field; the library
+exports mcx_amd_abi_layout() so the mirror is checked against the compiled
+layout at load time.  Loading fails loudly when the shared library is
+missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcx_amd.so")
+
+MAX_SESSION_LENGTH = 256
+MAX_PATH_LENGTH = 1024
+MAX_DEVICE = 256
+MAX_LANG_ID = 32
+mpStandalone, mpMATLAB, mpPython = 0, 1, 2
+
+
+class Float3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float)]
+
+
+class Float4(ctypes.Structure):
+    # 16-byte alignment of the C type is reproduced with explicit padding
+    # fields in Config (ctypes cannot raise a structure's alignment).
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float), ("w", ctypes.c_float)]
+
+
+class Uint3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint), ("y", ctypes.c_uint), ("z", ctypes.c_uint)]
+
+
+class Medium(ctypes.Structure):
+    _fields_ = [("mua", ctypes.c_float), ("mus", ctypes.c_float), ("g", ctypes.c_float), ("n", ctypes.c_float)]
+
+
+class History(ctypes.Structure):
+    _fields_ = [
+        ("magic", ctypes.c_char * 4), ("version", ctypes.c_uint), ("maxmedia", ctypes.c_uint),
+        ("detnum", ctypes.c_uint), ("colcount", ctypes.c_uint), ("totalphoton", ctypes.c_uint),
+        ("detected", ctypes.c_uint), ("savedphoton", ctypes.c_uint), ("unitinmm", ctypes.c_float),
+        ("seedbyte", ctypes.c_uint), ("normalizer", ctypes.c_float), ("respin", ctypes.c_int),
+        ("srcnum", ctypes.c_uint), ("savedetflag", ctypes.c_uint), ("totalsource", ctypes.c_uint),
+        ("reserved", ctypes.c_int * 1),
+    ]
+
+
+class Replay(ctypes.Structure):
+    _fields_ = [("detid", ctypes.c_void_p), ("seed", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("tof", ctypes.c_void_p)]
+
+
+class GPUInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * MAX_SESSION_LENGTH), ("id", ctypes.c_int), ("devcount", ctypes.c_int),
+        ("major", ctypes.c_int), ("minor", ctypes.c_int), ("globalmem", ctypes.c_size_t),
+        ("constmem", ctypes.c_size_t), ("sharedmem", ctypes.c_size_t), ("regcount", ctypes.c_int),
+        ("clock", ctypes.c_int), ("sm", ctypes.c_int), ("core", ctypes.c_int), ("autoblock", ctypes.c_int),
+        ("autothread", ctypes.c_int), ("maxgate", ctypes.c_uint), ("maxmpthread", ctypes.c_int),
+    ]
+
+
+class Config(ctypes.Structure):
+    """Mirror of Config (src/mcx_utils.h:172-285)."""
+    _fields_ = [
+        ("nphoton", ctypes.c_size_t), ("nblocksize", ctypes.c_uint), ("nthread", ctypes.c_uint),
+        ("seed", ctypes.c_int), ("_pad0", ctypes.c_char * 12),
+        ("srcpos", Float4), ("srcdir", Float4), ("srciquv", Float4),
+        ("tstart", ctypes.c_float), ("tstep", ctypes.c_float), ("tend", ctypes.c_float),
+        ("steps", Float3), ("dim", Uint3), ("crop0", Uint3), ("crop1", Uint3),
+        ("medianum", ctypes.c_uint), ("polmedianum", ctypes.c_uint), ("detnum", ctypes.c_uint),
+        ("maxdetphoton", ctypes.c_uint), ("detradius", ctypes.c_float), ("sradius", ctypes.c_float),
+        ("prop", ctypes.POINTER(Medium)), ("polprop", ctypes.c_void_p), ("detpos", ctypes.POINTER(Float4)),
+        ("smatrix", ctypes.c_void_p), ("maxgate", ctypes.c_uint), ("respin", ctypes.c_int),
+        ("printnum", ctypes.c_int), ("gpuid", ctypes.c_int), ("vol", ctypes.POINTER(ctypes.c_uint)),
+        ("session", ctypes.c_char * MAX_SESSION_LENGTH),
+        ("isrowmajor", ctypes.c_byte), ("isreflect", ctypes.c_byte), ("isref3", ctypes.c_byte),
+        ("isrefint", ctypes.c_byte), ("isnormalized", ctypes.c_byte), ("issavedet", ctypes.c_byte),
+        ("issave2pt", ctypes.c_byte), ("isgpuinfo", ctypes.c_byte), ("isspecular", ctypes.c_byte),
+        ("issrcfrom0", ctypes.c_byte), ("isdumpmask", ctypes.c_byte), ("autopilot", ctypes.c_byte),
+        ("issaveseed", ctypes.c_byte), ("issaveexit", ctypes.c_byte), ("issaveref", ctypes.c_byte),
+        ("ismomentum", ctypes.c_byte), ("istrajstokes", ctypes.c_byte), ("isdumpjson", ctypes.c_byte),
+        ("internalsrc", ctypes.c_byte),
+        ("zipid", ctypes.c_int), ("srctype", ctypes.c_byte), ("outputtype", ctypes.c_byte),
+        ("outputformat", ctypes.c_byte), ("faststep", ctypes.c_byte),
+        ("minenergy", ctypes.c_float), ("unitinmm", ctypes.c_float), ("omega", ctypes.c_float),
+        ("lambda_", ctypes.c_float), ("flog", ctypes.c_void_p), ("his", History),
+        ("exportfield", ctypes.POINTER(ctypes.c_float)), ("exportdetected", ctypes.POINTER(ctypes.c_float)),
+        ("detectedcount", ctypes.c_ulong), ("rootpath", ctypes.c_char * MAX_PATH_LENGTH),
+        ("shapedata", ctypes.c_void_p), ("extrajson", ctypes.c_void_p),
+        ("maxvoidstep", ctypes.c_int), ("voidtime", ctypes.c_int), ("_pad1", ctypes.c_char * 8),
+        ("srcparam1", Float4), ("srcparam2", Float4), ("srcnum", ctypes.c_uint),
+        ("srcpattern", ctypes.c_void_p), ("replay", Replay), ("seeddata", ctypes.c_void_p),
+        ("replaydet", ctypes.c_int), ("seedfile", ctypes.c_char * MAX_PATH_LENGTH),
+        ("jsonfile", ctypes.c_char * MAX_PATH_LENGTH), ("debuglevel", ctypes.c_uint),
+        ("savedetflag", ctypes.c_uint), ("deviceid", ctypes.c_char * MAX_DEVICE),
+        ("workload", ctypes.c_float * MAX_DEVICE), ("parentid", ctypes.c_int), ("runtime", ctypes.c_uint),
+        ("langid", ctypes.c_char * MAX_LANG_ID),
+        ("energytot", ctypes.c_double), ("energyabs", ctypes.c_double), ("energyesc", ctypes.c_double),
+        ("normalizer", ctypes.c_float), ("maxjumpdebug", ctypes.c_uint), ("debugdatalen", ctypes.c_uint),
+        ("gscatter", ctypes.c_uint), ("exportdebugdata", ctypes.c_void_p), ("mediabyte", ctypes.c_uint),
+        ("dx", ctypes.c_void_p), ("dy", ctypes.c_void_p), ("dz", ctypes.c_void_p),
+        ("bc", ctypes.c_char * 13), ("nphase", ctypes.c_uint), ("invcdf", ctypes.c_void_p),
+        ("nangle", ctypes.c_uint), ("angleinvcdf", ctypes.c_void_p), ("srcid", ctypes.c_int),
+        ("extrasrclen", ctypes.c_uint), ("srcdata", ctypes.c_void_p),
+    ]
+
+
+# every symbol include/mcx_abi.h and include/mcx_amd_ext.h declare
+ABI_SYMBOLS = [
+    "mcx_run_simulation", "mcx_list_gpu", "mcx_initcfg", "mcx_clearcfg", "mcx_preprocess", "mcx_maskdet",
+    "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
+    "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_last_error", "mcx_clear_error",
+    "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
+    "mcx_amd_abi_layout",
+]
+
+_lib = None
+
+
+def _setup(L):
+    P = ctypes.POINTER
+    L.mcx_initcfg.argtypes = [P(Config)]
+    L.mcx_clearcfg.argtypes = [P(Config)]
+    L.mcx_preprocess.argtypes = [P(Config)]
+    L.mcx_maskdet.argtypes = [P(Config)]
+    L.mcx_validatecfg.argtypes = [P(Config), P(ctypes.c_float), P(ctypes.c_int), ctypes.c_int]
+    L.mcx_list_gpu.argtypes = [P(Config), P(P(GPUInfo))]
+    L.mcx_list_gpu.restype = ctypes.c_int
+    L.mcx_run_simulation.argtypes = [P(Config), P(GPUInfo)]
+    L.mcx_cleargpuinfo.argtypes = [P(P(GPUInfo))]
+    L.mcx_readconfig.argtypes = [ctypes.c_char_p, P(Config)]
+    L.mcx_run_from_json.argtypes = [ctypes.c_char_p]
+    L.mcx_run_from_json.restype = ctypes.c_int
+    L.mcx_savedata.argtypes = [P(ctypes.c_float), ctypes.c_size_t, P(Config)]
+    L.mcx_savedetphoton.argtypes = [P(ctypes.c_float), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(Config)]
+    L.mcx_normalize.argtypes = [P(ctypes.c_float), ctypes.c_float, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int]
+    L.mcx_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.mcx_last_error.restype = ctypes.c_int
+    L.mcx_rng_test.argtypes = [ctypes.c_uint, P(ctypes.c_float), ctypes.c_uint]
+    L.mcx_rng_test.restype = ctypes.c_int
+    L.mcx_amd_set_tally_mode.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_tally_mode.restype = ctypes.c_int
+    L.mcx_amd_set_counters.argtypes = [ctypes.c_int]
+    L.mcx_amd_last_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
+    L.mcx_amd_last_stats.restype = ctypes.c_int
+    L.mcx_amd_abi_layout.argtypes = [P(ctypes.c_size_t), ctypes.c_int]
+    L.mcx_amd_abi_layout.restype = ctypes.c_int
+
+
+def check_layout(L):
+    """Compare the ctypes mirror with the compiled struct layout."""
+    out = (ctypes.c_size_t * 16)()
+    n = L.mcx_amd_abi_layout(out, 16)
+    want = [ctypes.sizeof(Config), ctypes.sizeof(GPUInfo), ctypes.sizeof(History), Config.srcpos.offset,
+            Config.session.offset, Config.flog.offset, Config.his.offset, Config.exportfield.offset,
+            Config.srcparam1.offset, Config.deviceid.offset, Config.energytot.offset, Config.bc.offset,
+            Config.srcdata.offset]
+    got = list(out[:n])
+    if got != want[:n]:
+        raise RuntimeError("Config layout mismatch between libmcx_amd.so %s and the ctypes mirror %s" % (got, want))
+
+
+def lib():
+    """Load libmcx_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmcx_amd.so is not built; run __graft_entry__.build() or make -C mcx_amd/csrc")
+        L = ctypes.CDLL(LIB_PATH)
+        _setup(L)
+        check_layout(L)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    code = lib().mcx_last_error(buf, 1024)
+    return code, buf.value.decode(errors="replace")
+
+
+def last_stats():
+    out = (ctypes.c_double * 11)()
+    n = lib().mcx_amd_last_stats(out, 11)
+    keys = ["kernel_ms", "wall_ms", "nstep", "ndeposit", "nscatter", "nlaunch", "vgprs", "nthread", "nblock",
+            "nlaunches", "errlanes"]
+    return dict(zip(keys, list(out[:n])))

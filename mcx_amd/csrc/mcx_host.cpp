@@ -1,0 +1,615 @@
+/*
+ * mcx_host.cpp -- launch/reduction layer: mcx_run_simulation, mcx_list_gpu.
+ *
+ * Follows the reference's host flow (src/mcx_core.cu:2740-4110): derive the
+ * launch constants, seed every virtual thread from the glibc rand() stream,
+ * loop over time-gate groups and respins, launch the transport kernel, then
+ * merge the tally, sum the energies in FP64 and normalise.  MI355X
+ * differences:
+ *  - one HIP stream, hipEvents for kernel time, steady_clock around
+ *    launch+sync for the reference's `runtime` definition;
+ *  - the tally halves (shadow or hi/lo) are merged on the GPU by a
+ *    vectorised kernel instead of a serial host loop (:3687-3696);
+ *  - the label volume is shipped as uint8 (bit 7 = detector mask) when the
+ *    labels fit, uint32 otherwise;
+ *  - autopilot sizes the grid from the kernel's register count so every
+ *    SIMD of every CU is filled once (256 CUs x 4 SIMDs x waves/SIMD).
+ */
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mcx_amd_ext.h"
+#include "mcx_internal.h"
+#include "mcx_kernel.h"
+
+extern "C" hipError_t mcxk_launch_transport(const KParam* kp, int ispencil, int isreflect, int tally, int media32,
+                                            unsigned int grid, unsigned int block, size_t shmem, hipStream_t stream);
+extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads);
+extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream);
+extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, size_t n, hipStream_t stream);
+
+namespace mcx {
+
+static std::mutex g_statmu;
+static double g_stats[11];
+static int g_tally_mode = MCX_TALLY_F32_SHADOW;
+static int g_counters = 0;
+
+#define HIP_CHECK(x)                                                                                      \
+    do {                                                                                                  \
+        hipError_t e_ = (x);                                                                              \
+        if (e_ != hipSuccess)                                                                             \
+            throw mcx::Error(-(int)e_, std::string("HIP error (") + hipGetErrorString(e_) + ") in " #x); \
+    } while (0)
+
+/* device allocation released on scope exit */
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        bytes = n;
+        if (n) HIP_CHECK(hipMalloc(&p, n));
+    }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+static bool supported_source(int t) {
+    switch (t) {
+        case MCX_SRC_PENCIL: case MCX_SRC_ISOTROPIC: case MCX_SRC_CONE: case MCX_SRC_GAUSSIAN: case MCX_SRC_PLANAR:
+        case MCX_SRC_FOURIER: case MCX_SRC_ARCSINE: case MCX_SRC_DISK: case MCX_SRC_PENCILARRAY: case MCX_SRC_RING:
+            return true;
+        default:
+            return false;
+    }
+}
+
+static void check_supported(const Config* cfg) {
+    if (cfg->polmedianum > 0) MCX_THROW(-1, "polarized photon simulation is not supported by this engine");
+    if (cfg->mediabyte > 4) MCX_THROW(-1, "only label-based media formats are supported by this engine");
+    if (cfg->seed == MCX_SEED_FROM_FILE) MCX_THROW(-1, "photon replay is not supported by this engine");
+    if (cfg->nphase || cfg->nangle) MCX_THROW(-1, "user-defined phase/angle inverse CDFs are not supported by this engine");
+    if (cfg->issaveref) MCX_THROW(-1, "diffuse reflectance output (issaveref) is not supported by this engine");
+    if (cfg->extrasrclen) MCX_THROW(-1, "multiple sources are not supported by this engine");
+    if (cfg->issaveseed) MCX_THROW(-1, "saving photon seeds is not supported by this engine");
+    if (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))
+        MCX_THROW(-1, "trajectory output (-D M) is not supported by this engine");
+    if (!supported_source(cfg->srctype)) MCX_THROW(-1, "the specified source type is not supported by this engine");
+    if (cfg->outputtype != otFlux && cfg->outputtype != otFluence && cfg->outputtype != otEnergy && cfg->outputtype != otL)
+        MCX_THROW(-1, "the specified output type is not supported by this engine");
+}
+
+/* waves per SIMD for a kernel using `vgprs` architectural VGPRs (gfx950:
+ * 512 per lane per SIMD, granule 8, at most 8 waves) */
+static int waves_per_simd(int vgprs) {
+    if (vgprs <= 0) return 8;
+    int g = (vgprs + 7) / 8 * 8;
+    return std::max(1, std::min(8, 512 / g));
+}
+
+static void rng_test_path(Config* cfg, GPUInfo* gpu, int gpuid, size_t fieldlen) {
+    (void)gpu;
+    (void)gpuid;
+    /* no srand() precedes this path in the reference: glibc default state */
+    GlibcRand r(1);
+    uint32_t seeds[4];
+    for (int i = 0; i < 4; i++) {
+        uint32_t a = (uint32_t)r.next(), b = (uint32_t)r.next(), c = (uint32_t)r.next();
+        seeds[i] = (a << 16) | (b << 1) | (c >> 14);
+    }
+    Stream st;
+    DevMem dseed, dout;
+    dseed.alloc(sizeof(seeds));
+    dout.alloc(fieldlen * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(dseed.p, seeds, sizeof(seeds), hipMemcpyHostToDevice, st.s));
+    HIP_CHECK(mcxk_launch_rngtest(dout.as<float>(), dseed.as<uint32_t>(), (uint32_t)fieldlen, st.s));
+    if (!cfg->exportfield) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(cfg->exportfield, dout.p, fieldlen * sizeof(float), hipMemcpyDeviceToHost, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    if (cfg->flog) fprintf(cfg->flog, "generated %zu random numbers\n", fieldlen);
+}
+
+static void run_impl(Config* cfg, GPUInfo* gpu) {
+    check_supported(cfg);
+    int threadid = 0;
+    if (cfg->deviceid[threadid] == '\0') return;
+    const int gpuid = cfg->deviceid[threadid] - 1;
+    if (gpuid < 0) MCX_THROW(-1, "GPU ID must be non-zero");
+    HIP_CHECK(hipSetDevice(gpuid));
+
+    const unsigned int partialdata = det_partialdata(cfg);
+    const unsigned int hostdetreclen = det_reclen(cfg);
+    const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+    const unsigned int totalgates = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
+    unsigned int is2d = (cfg->dim.x == 1 ? 1 : (cfg->dim.y == 1 ? 2 : (cfg->dim.z == 1 ? 3 : 0)));
+    if (is2d) is2d = is2d * ((cfg->dim.x > 1) + (cfg->dim.y > 1) + (cfg->dim.z > 1) == 2);
+    if (is2d) {
+        const float* vec = &cfg->srcdir.x;
+        if (fabsf(vec[is2d - 1]) > MCX_EPS)
+            MCX_THROW(-1, "input domain is 2D, the initial direction can not have non-zero value in the singular dimension");
+    }
+
+    GPUInfo& G = gpu[gpuid];
+    if (G.maxgate == 0 && nvox > 0) {
+        size_t need = nvox + (size_t)cfg->nthread * 64 + sizeof(float) * (size_t)cfg->maxdetphoton * hostdetreclen + (10u << 20);
+        size_t free_b = 0, total_b = 0;
+        HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        size_t mg = free_b > need ? (free_b - need) / (nvox * 2 * sizeof(float)) : 1;
+        G.maxgate = (unsigned int)std::min<size_t>(std::max<size_t>(mg, 1), (size_t)totalgates);
+    }
+    cfg->detectedcount = 0;
+    cfg->his.detected = 0;
+    cfg->his.respin = cfg->respin;
+    cfg->his.colcount = hostdetreclen;
+    cfg->energytot = cfg->energyabs = cfg->energyesc = 0.0;
+    cfg->runtime = 0;
+
+    const int tally = g_tally_mode;
+    const int ispencil = (cfg->srctype == MCX_SRC_PENCIL);
+    int isref = cfg->isreflect;
+    for (int i = 0; i < 6; i++)
+        if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) isref = 1;
+
+    /* media: uint8 with the detector bit in bit 7 when labels fit */
+    unsigned int maxlabel = 0;
+    for (size_t i = 0; i < nvox; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
+    const int media32 = (maxlabel > 127);
+
+    int numregs = 0, maxthreads = 0;
+    HIP_CHECK(mcxk_kernel_attrs(ispencil, isref > 0, tally, media32, &numregs, &maxthreads));
+    if (!cfg->autopilot) {
+        G.autothread = (int)cfg->nthread;
+        G.autoblock = (int)cfg->nblocksize;
+        if (cfg->maxgate == 0 || cfg->maxgate > totalgates) cfg->maxgate = totalgates;
+        G.maxgate = cfg->maxgate;
+    } else {
+        G.autoblock = 256;
+        G.autothread = G.sm * 4 * waves_per_simd(numregs) * 64;
+        if (cfg->maxgate && cfg->maxgate < G.maxgate) G.maxgate = cfg->maxgate;
+    }
+    if (G.autoblock <= 0 || G.autoblock > 1024 || G.autoblock > maxthreads) G.autoblock = std::min(256, maxthreads);
+    if (G.autothread % G.autoblock) G.autothread = (G.autothread / G.autoblock) * G.autoblock;
+    if (G.autothread <= 0) MCX_THROW(-1, "thread number must be at least one block");
+    const unsigned int maxgate = G.maxgate;
+    if (totalgates > maxgate && cfg->isnormalized) {
+        if (cfg->flog) fprintf(cfg->flog, "WARNING: GPU memory can not hold all time gates, disabling normalization to allow multiple runs\n");
+        cfg->isnormalized = 0;
+    }
+
+    const size_t fieldlen = nvox * maxgate;
+    if (cfg->exportfield == NULL && cfg->issave2pt) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
+    if (cfg->exportdetected == NULL) cfg->exportdetected = (float*)malloc(std::max<size_t>(1, (size_t)hostdetreclen * cfg->maxdetphoton) * sizeof(float));
+
+    if (cfg->debuglevel & MCX_DEBUG_RNG) {
+        rng_test_path(cfg, gpu, gpuid, fieldlen);
+        return;
+    }
+
+    /* work split (src/mcx_core.cu:2980-3022): one device takes all photons */
+    const size_t gpuphoton = cfg->nphoton;
+    if (gpuphoton == 0) return;
+    int threadphoton, oddphotons;
+    const unsigned int nthread = (unsigned int)G.autothread;
+    if (cfg->respin >= 1) {
+        threadphoton = (int)(gpuphoton / nthread);
+        oddphotons = (int)(gpuphoton - (size_t)threadphoton * nthread);
+    } else {
+        threadphoton = (int)(gpuphoton / nthread / (size_t)(-cfg->respin));
+        oddphotons = (int)(gpuphoton / (size_t)(-cfg->respin) - (size_t)threadphoton * nthread);
+    }
+
+    Stream st;
+    DevMem dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
+    std::vector<unsigned char> m8;
+    if (media32) {
+        dmedia.alloc(nvox * 4);
+        HIP_CHECK(hipMemcpyAsync(dmedia.p, cfg->vol, nvox * 4, hipMemcpyHostToDevice, st.s));
+    } else {
+        m8.resize(nvox);
+        for (size_t i = 0; i < nvox; i++) m8[i] = (unsigned char)((cfg->vol[i] & 0x7Fu) | ((cfg->vol[i] >> 24) & 0x80u));
+        dmedia.alloc(nvox);
+        HIP_CHECK(hipMemcpyAsync(dmedia.p, m8.data(), nvox, hipMemcpyHostToDevice, st.s));
+    }
+    const unsigned int ntable = cfg->medianum + cfg->detnum;
+    if (ntable > MCX_MAX_PROP_AND_DETECTORS) MCX_THROW(-1, "too many media and detectors");
+    std::vector<float> table(4 * ntable);
+    memcpy(table.data(), cfg->prop, sizeof(float) * 4 * cfg->medianum);
+    if (cfg->detnum) memcpy(table.data() + 4 * cfg->medianum, cfg->detpos, sizeof(float) * 4 * cfg->detnum);
+    dprop.alloc(table.size() * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(dprop.p, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, st.s));
+
+    const bool savedet = cfg->issavedet != 0;
+    if (tally == MCX_TALLY_F64) {
+        dfield64.alloc(fieldlen * sizeof(double));
+    } else {
+        dfield.alloc(fieldlen * sizeof(float));
+        dfield2.alloc(fieldlen * sizeof(float));
+    }
+    dmerged.alloc(fieldlen * sizeof(float));
+    dgenergy.alloc((size_t)nthread * 2 * sizeof(float));
+    HIP_CHECK(hipMemsetAsync(dgenergy.p, 0, dgenergy.bytes, st.s));
+    dseeds.alloc((size_t)nthread * 4 * sizeof(uint32_t));
+    if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
+    dcount.alloc(4 * sizeof(uint32_t));
+    dcounters.alloc(4 * sizeof(uint64_t));
+    HIP_CHECK(hipMemsetAsync(dcounters.p, 0, dcounters.bytes, st.s));
+
+    KParam kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.nx = cfg->dim.x;
+    kp.ny = cfg->dim.y;
+    kp.nz = cfg->dim.z;
+    kp.dimxy = cfg->dim.x * cfg->dim.y;
+    kp.nvox = (uint32_t)nvox;
+    kp.maxidx_x = (float)cfg->dim.x;
+    kp.maxidx_y = (float)cfg->dim.y;
+    kp.maxidx_z = (float)cfg->dim.z;
+    memcpy(kp.srcpos, &cfg->srcpos, 16);
+    memcpy(kp.srcdir, &cfg->srcdir, 16);
+    memcpy(kp.srcparam1, &cfg->srcparam1, 16);
+    memcpy(kp.srcparam2, &cfg->srcparam2, 16);
+    kp.srctype = cfg->srctype;
+    kp.Rtstep = 1.f / cfg->tstep;
+    kp.oneoverc0 = MCX_R_C0 * cfg->unitinmm;
+    kp.minaccumtime = 1.f * MCX_R_C0 * cfg->unitinmm;
+    kp.maxgate = maxgate;
+    kp.minenergy = cfg->minenergy;
+    kp.doreflect = (uint32_t)cfg->isreflect;
+    kp.isspecular = cfg->isspecular > 0;
+    kp.voidtime = (uint32_t)cfg->voidtime;
+    kp.maxvoidstep = (uint32_t)cfg->maxvoidstep;
+    kp.gscatter = cfg->gscatter;
+    kp.outputtype = (uint32_t)cfg->outputtype;
+    kp.save2pt = (uint32_t)cfg->issave2pt;
+    kp.is2d = is2d;
+    memcpy(kp.bc, cfg->bc, 12);
+    kp.savedet = (uint32_t)cfg->issavedet;
+    kp.savedetflag = cfg->savedetflag;
+    kp.maxdetphoton = cfg->maxdetphoton;
+    kp.reclen = hostdetreclen;
+    kp.partialdata = partialdata;
+    kp.maxmedia = cfg->medianum - 1;
+    kp.detnum = cfg->detnum;
+    kp.medianum = cfg->medianum;
+    kp.ntable = ntable;
+    kp.threadphoton = threadphoton;
+    kp.oddphotons = oddphotons;
+    kp.nthread = nthread;
+    kp.media = dmedia.p;
+    kp.gprop = dprop.as<float>();
+    kp.field = dfield.as<float>();
+    kp.field2 = dfield2.as<float>();
+    kp.field64 = dfield64.as<double>();
+    kp.genergy = dgenergy.as<float>();
+    kp.seeds = dseeds.as<uint32_t>();
+    kp.detdata = ddet.as<float>();
+    kp.detcount = dcount.as<uint32_t>();
+    kp.counters = g_counters ? dcounters.as<uint64_t>() : nullptr;
+
+    const unsigned int block = (unsigned int)G.autoblock;
+    const unsigned int grid = nthread / block;
+    const size_t shmem = (size_t)ntable * 16 + (size_t)block * partialdata * sizeof(float);
+    if (shmem > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
+
+    if (cfg->flog && cfg->printnum >= 0) {
+        fprintf(cfg->flog, "GPU=%d (%s) threadph=%d extra=%d np=%.0f nthread=%u maxgate=%u repetition=%d\n", gpuid + 1, G.name,
+                threadphoton, oddphotons, (double)gpuphoton, nthread, maxgate, abs(cfg->respin));
+        fflush(cfg->flog);
+    }
+
+    GlibcRand rnd((unsigned int)(cfg->seed > 0 ? cfg->seed + threadid : (int)time(0) + threadid));
+    std::vector<uint32_t> seeds((size_t)nthread * 4);
+    std::vector<float> field(cfg->issave2pt ? fieldlen : 0, 0.f);
+    std::vector<float> fieldacc((cfg->issave2pt && abs(cfg->respin) > 1) ? fieldlen : 0, 0.f);
+    std::vector<float> pdet;
+    double kernel_ms = 0.0, wall_ms = 0.0;
+    int nlaunch = 0;
+    uint32_t counts[4] = {0, 0, 0, 0};
+    uint32_t errlanes = 0;
+    EventPair ev;
+
+    for (unsigned int timegate = 0; timegate < totalgates; timegate += maxgate) {
+        kp.twin0 = cfg->tstart + cfg->tstep * timegate;
+        kp.twin1 = kp.twin0 + cfg->tstep * maxgate;
+        for (int iter = 0; iter < abs(cfg->respin); iter++) {
+            if (tally == MCX_TALLY_F64) {
+                HIP_CHECK(hipMemsetAsync(dfield64.p, 0, dfield64.bytes, st.s));
+            } else {
+                HIP_CHECK(hipMemsetAsync(dfield.p, 0, dfield.bytes, st.s));
+                HIP_CHECK(hipMemsetAsync(dfield2.p, 0, dfield2.bytes, st.s));
+            }
+            if (savedet) HIP_CHECK(hipMemsetAsync(ddet.p, 0, ddet.bytes, st.s));
+            HIP_CHECK(hipMemsetAsync(dcount.p, 0, dcount.bytes, st.s));
+            /* Pseed[i] = (rand()<<16) | (rand()<<1) | (rand()>>14), evaluated left to right */
+            for (size_t i = 0; i < seeds.size(); i++) {
+                uint32_t a = (uint32_t)rnd.next(), b = (uint32_t)rnd.next(), c = (uint32_t)rnd.next();
+                seeds[i] = (a << 16) | (b << 1) | (c >> 14);
+            }
+            HIP_CHECK(hipMemcpyAsync(dseeds.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st.s));
+            HIP_CHECK(hipStreamSynchronize(st.s));
+
+            auto tic = std::chrono::steady_clock::now();
+            HIP_CHECK(hipEventRecord(ev.a, st.s));
+            HIP_CHECK(mcxk_launch_transport(&kp, ispencil, isref > 0, tally, media32, grid, block, shmem, st.s));
+            HIP_CHECK(hipEventRecord(ev.b, st.s));
+            HIP_CHECK(hipMemcpyAsync(counts, dcount.p, sizeof(counts), hipMemcpyDeviceToHost, st.s));
+            HIP_CHECK(hipStreamSynchronize(st.s));
+            auto toc = std::chrono::steady_clock::now();
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev.a, ev.b));
+            kernel_ms += ms;
+            wall_ms += std::chrono::duration<double, std::milli>(toc - tic).count();
+            nlaunch++;
+            errlanes += counts[1];
+            uint32_t detected = counts[0];
+
+            if (savedet) {
+                if (detected > cfg->maxdetphoton) {
+                    if (cfg->flog)
+                        fprintf(cfg->flog, "WARNING: the detected photon number is more than what your have specified (%u > %u), please use the -H option to specify a greater number\n",
+                                detected, cfg->maxdetphoton);
+                } else if (cfg->flog) {
+                    fprintf(cfg->flog, "detected %u photons, total: %.0f\n", detected, (double)cfg->detectedcount + detected);
+                }
+                cfg->his.detected += detected;
+                detected = std::min(detected, cfg->maxdetphoton);
+                if (cfg->exportdetected) {
+                    cfg->exportdetected = (float*)realloc(cfg->exportdetected,
+                                                          std::max<size_t>(1, (cfg->detectedcount + detected) * (size_t)hostdetreclen) * sizeof(float));
+                    if (detected)
+                        HIP_CHECK(hipMemcpyAsync(cfg->exportdetected + cfg->detectedcount * hostdetreclen, ddet.p,
+                                                 (size_t)detected * hostdetreclen * sizeof(float), hipMemcpyDeviceToHost, st.s));
+                    cfg->detectedcount += detected;
+                }
+            }
+            if (cfg->issave2pt) {
+                const void* a = (tally == MCX_TALLY_F64) ? dfield64.p : dfield.p;
+                HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, fieldlen, st.s));
+                HIP_CHECK(hipMemcpyAsync(field.data(), dmerged.p, fieldlen * sizeof(float), hipMemcpyDeviceToHost, st.s));
+                HIP_CHECK(hipStreamSynchronize(st.s));
+                if (abs(cfg->respin) > 1) {
+#pragma omp parallel for schedule(static)
+                    for (long long i = 0; i < (long long)fieldlen; i++) fieldacc[i] += field[i];
+                }
+            }
+            HIP_CHECK(hipStreamSynchronize(st.s));
+        }
+        if (abs(cfg->respin) > 1 && cfg->issave2pt) field = fieldacc;
+
+        std::vector<float> energy((size_t)nthread * 2);
+        HIP_CHECK(hipMemcpyAsync(energy.data(), dgenergy.p, energy.size() * sizeof(float), hipMemcpyDeviceToHost, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
+        for (unsigned int i = 0; i < nthread; i++) {
+            cfg->energyesc += energy[2 * i];
+            cfg->energytot += energy[2 * i + 1];
+        }
+        if (cfg->exportfield && cfg->issave2pt) {
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)fieldlen; i++) cfg->exportfield[i] += field[i];
+        }
+        if (kp.twin1 < cfg->tend) HIP_CHECK(hipMemsetAsync(dgenergy.p, 0, dgenergy.bytes, st.s));
+        if (abs(cfg->respin) > 1) std::fill(fieldacc.begin(), fieldacc.end(), 0.f);
+    }
+    cfg->runtime = (unsigned int)llround(wall_ms);
+
+    uint64_t ctr[4] = {0, 0, 0, 0};
+    if (g_counters) {
+        HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_statmu);
+        g_stats[0] = kernel_ms;
+        g_stats[1] = wall_ms;
+        g_stats[2] = (double)ctr[0];
+        g_stats[3] = (double)ctr[1];
+        g_stats[4] = (double)ctr[2];
+        g_stats[5] = (double)ctr[3];
+        g_stats[6] = numregs;
+        g_stats[7] = nthread;
+        g_stats[8] = block;
+        g_stats[9] = nlaunch;
+        g_stats[10] = errlanes;
+    }
+
+    /* normalisation (src/mcx_core.cu:3823-3925) */
+    if (cfg->issave2pt && cfg->isnormalized) {
+        float scale = 1.f;
+        const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+        cfg->energyabs += cfg->energytot - cfg->energyesc;
+        if (cfg->outputtype == otFlux || cfg->outputtype == otFluence) {
+            scale = (float)(cfg->unitinmm / (cfg->energytot * Vvox * cfg->tstep));
+            if (cfg->outputtype == otFluence) scale *= cfg->tstep;
+        } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
+            scale = (float)(1.f / cfg->energytot);
+        }
+        cfg->normalizer = scale;
+        cfg->his.normalizer = scale;
+        if (cfg->flog) fprintf(cfg->flog, "source 1, normalization factor alpha=%f\n", scale);
+        mcx_normalize(cfg->exportfield, scale, fieldlen, cfg->isnormalized, 0, 1);
+    }
+    if (cfg->parentid == mpStandalone) {
+        if (cfg->issave2pt) mcx_savedata(cfg->exportfield, fieldlen, cfg);
+        if (cfg->issavedet && cfg->exportdetected) {
+            cfg->his.unitinmm = cfg->unitinmm;
+            cfg->his.savedphoton = (unsigned int)cfg->detectedcount;
+            cfg->his.totalphoton = (unsigned int)cfg->nphoton;
+            cfg->his.detected = (unsigned int)cfg->detectedcount;
+            mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, 0, cfg);
+        }
+    }
+    if (cfg->flog) {
+        const double np = (double)cfg->nphoton * ((cfg->respin > 1) ? cfg->respin : 1);
+        fprintf(cfg->flog, "simulated %.0f photons (%.0f) with %u threads (repeat x%d)\nMCX simulation speed: %.2f photon/ms\n", np, np,
+                nthread, abs(cfg->respin), np / std::max(1e-3, wall_ms));
+        fprintf(cfg->flog, "total simulated energy: %.2f\tabsorbed: %5.5f%%\n(loss due to initial specular reflection is excluded in the total)\n",
+                cfg->energytot, (cfg->energytot - cfg->energyesc) / cfg->energytot * 100.f);
+        fflush(cfg->flog);
+    }
+    cfg->energyabs = cfg->energytot - cfg->energyesc;
+    if (errlanes) MCX_THROW(-2, "transport kernel: a photon left the volume without being relaunched (internal error)");
+}
+
+static int list_gpu_impl(Config* cfg, GPUInfo** info) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        if (cfg->flog) fprintf(cfg->flog, "ERROR: No HIP-capable GPU device found\n");
+        *info = NULL;
+        return 0;
+    }
+    *info = (GPUInfo*)calloc((size_t)count, sizeof(GPUInfo));
+    if (cfg->gpuid && cfg->gpuid > count) {
+        if (cfg->flog) fprintf(cfg->flog, "ERROR: Specified GPU ID is out of range\n");
+        free(*info);
+        *info = NULL;
+        return 0;
+    }
+    int activedev = 0;
+    for (int dev = 0; dev < count; dev++) {
+        hipDeviceProp_t dp;
+        HIP_CHECK(hipGetDeviceProperties(&dp, dev));
+        if (cfg->isgpuinfo == 3) {
+            activedev++;
+        } else if (cfg->deviceid[dev] == '1') {
+            cfg->deviceid[dev] = '\0';
+            cfg->deviceid[activedev] = (char)(dev + 1);
+            activedev++;
+        }
+        GPUInfo& g = (*info)[dev];
+        strncpy(g.name, dp.name, MCX_MAX_SESSION_LENGTH - 1);
+        g.id = dev + 1;
+        g.devcount = count;
+        g.major = dp.major;
+        g.minor = dp.minor;
+        g.globalmem = dp.totalGlobalMem;
+        g.constmem = dp.totalConstMem;
+        g.sharedmem = dp.sharedMemPerBlock;
+        g.regcount = dp.regsPerBlock;
+        g.clock = dp.clockRate;
+        g.sm = dp.multiProcessorCount;
+        g.core = dp.multiProcessorCount * 4 * 16; /* 4 SIMDs x 16 lanes per CU */
+        g.maxmpthread = dp.maxThreadsPerMultiProcessor;
+        g.maxgate = cfg->maxgate;
+        g.autoblock = 256;
+        g.autothread = g.sm * 4 * 8 * 64;
+        if (cfg->isgpuinfo && cfg->flog) {
+            fprintf(cfg->flog, "=============================   GPU Information  ================================\n");
+            fprintf(cfg->flog, "Device %d of %d:\t\t%s (%s)\n", g.id, g.devcount, g.name, dp.gcnArchName);
+            fprintf(cfg->flog, "Global Memory:\t\t%.0f B\nCompute Units:\t\t%d\nClock Speed:\t\t%.2f GHz\n", (double)g.globalmem, g.sm,
+                    g.clock * 1e-6f);
+            fprintf(cfg->flog, "Auto-thread:\t\t%d\nAuto-block:\t\t%d\n", g.autothread, g.autoblock);
+        }
+    }
+    if (cfg->isgpuinfo == 2 && cfg->parentid == mpStandalone) exit(0);
+    if (activedev < MCX_MAX_DEVICE) cfg->deviceid[activedev] = '\0';
+    return activedev;
+}
+
+}  // namespace mcx
+
+using namespace mcx;
+
+extern "C" {
+
+void mcx_run_simulation(Config* cfg, GPUInfo* gpu) {
+    EntryGuard g;
+    GPUInfo* own = NULL;
+    try {
+        if (gpu == NULL) {
+            if (list_gpu_impl(cfg, &own) == 0) MCX_THROW(-1, "no active GPU device");
+            gpu = own;
+        }
+        run_impl(cfg, gpu);
+    } catch (const Error& e) {
+        if (own) free(own);
+        handle_error(e, cfg);
+        return;
+    }
+    if (own) free(own);
+}
+
+int mcx_list_gpu(Config* cfg, GPUInfo** info) {
+    EntryGuard g;
+    try {
+        return list_gpu_impl(cfg, info);
+    } catch (const Error& e) {
+        handle_error(e, cfg);
+        return 0;
+    }
+}
+
+int mcx_rng_test(unsigned int seed, float* out, unsigned int n) {
+    EntryGuard g;
+    try {
+        GlibcRand r(seed);
+        uint32_t seeds[4];
+        for (int i = 0; i < 4; i++) {
+            uint32_t a = (uint32_t)r.next(), b = (uint32_t)r.next(), c = (uint32_t)r.next();
+            seeds[i] = (a << 16) | (b << 1) | (c >> 14);
+        }
+        Stream st;
+        DevMem dseed, dout;
+        dseed.alloc(sizeof(seeds));
+        dout.alloc((size_t)n * sizeof(float));
+        HIP_CHECK(hipMemcpyAsync(dseed.p, seeds, sizeof(seeds), hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(mcxk_launch_rngtest(dout.as<float>(), dseed.as<uint32_t>(), n, st.s));
+        HIP_CHECK(hipMemcpyAsync(out, dout.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, st.s));
+        HIP_CHECK(hipStreamSynchronize(st.s));
+        return 0;
+    } catch (const Error& e) {
+        record_error(e.id, e.msg.c_str());
+        return e.id ? e.id : -1;
+    }
+}
+
+int mcx_amd_set_tally_mode(int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    int old = g_tally_mode;
+    g_tally_mode = mode;
+    return old;
+}
+
+void mcx_amd_set_counters(int enable) { g_counters = enable ? 1 : 0; }
+
+int mcx_amd_last_stats(double* out, int n) {
+    std::lock_guard<std::mutex> lk(g_statmu);
+    int k = std::min(n, 11);
+    for (int i = 0; i < k; i++) out[i] = g_stats[i];
+    return k;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+/*
+ * mcx_internal.h -- shared host-side helpers of the library (not ABI).
+ */
+#ifndef MCX_AMD_INTERNAL_H
+#define MCX_AMD_INTERNAL_H
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/mcx_abi.h"
+
+namespace mcx {
+
+/* Error raised inside the library; entry points translate it into the
+ * reference's convention (print + exit for standalone callers, recorded
+ * error for embedded ones). */
+struct Error : public std::exception {
+    int id;
+    std::string msg;
+    Error(int i, std::string m) : id(i), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+/* RAII marker: while alive, mcx_error() throws instead of exiting. */
+struct EntryGuard {
+    EntryGuard();
+    ~EntryGuard();
+};
+
+void record_error(int id, const char* msg);
+/* Called by entry points on a caught Error: standalone -> print + exit. */
+void handle_error(const Error& e, const Config* cfg);
+
+/* glibc rand()/srand() restated (TYPE_3 additive feedback, published
+ * algorithm) so that the per-thread seeds do not depend on the host libc's
+ * global state (src/mcx_core.cu:3247-3251, :3381-3387). */
+class GlibcRand {
+  public:
+    explicit GlibcRand(unsigned int seed);
+    int next();
+  private:
+    unsigned int ring_[31];
+    int pos_;
+};
+
+/* bytes of one detected-photon record (src/mcx_core.cu:2820-2830) */
+unsigned int det_reclen(const Config* cfg);
+unsigned int det_partialdata(const Config* cfg);
+
+/* rasterize a JSON Shapes description onto cfg->vol (mcx_io.cpp) */
+void apply_shapes(Config* cfg, const char* text);
+
+}  // namespace mcx
+
+#define MCX_THROW(id, msg) throw mcx::Error((id), (msg))
+
+#endif

@@ -1,0 +1,55 @@
+/*
+ * mcx_kernel.h -- launch parameters of the transport kernel.
+ *
+ * KParam is passed by value as the kernel argument block, so every field
+ * is wave-uniform and lands in scalar registers; it replaces the
+ * reference's __constant__ MCXParam (src/mcx_core.h:138-198) and the
+ * gproperty table (src/mcx_core.cu:180).  The property/detector table is
+ * read from global memory once per workgroup and kept in LDS.
+ */
+#ifndef MCX_AMD_KERNEL_H
+#define MCX_AMD_KERNEL_H
+
+#include <stdint.h>
+
+/* tally precision modes */
+enum {
+    MCX_TALLY_F32_SHADOW = 0, /* FP32 atomics + MAX_ACCUM shadow half (reference semantics) */
+    MCX_TALLY_F32_HILO = 1,   /* FP32 hi + FP32 lo (TwoSum error), merged in FP64 */
+    MCX_TALLY_F64 = 2         /* FP64 atomics (accuracy reference) */
+};
+
+struct KParam {
+    /* grid */
+    uint32_t nx, ny, nz, dimxy, nvox;
+    float maxidx_x, maxidx_y, maxidx_z;
+    /* source */
+    float srcpos[4], srcdir[4], srcparam1[4], srcparam2[4];
+    int32_t srctype;
+    /* time gates */
+    float twin0, twin1, Rtstep, oneoverc0, minaccumtime;
+    uint32_t maxgate;
+    /* physics */
+    float minenergy;
+    uint32_t doreflect, isspecular, voidtime, maxvoidstep, gscatter, outputtype, save2pt, is2d;
+    uint8_t bc[12];
+    /* detected photons */
+    uint32_t savedet, savedetflag, maxdetphoton, reclen, partialdata, maxmedia, detnum, medianum;
+    uint32_t ntable; /* medianum + detnum rows of float4 */
+    /* work split (src/mcx_core.cu:3014-3022) */
+    int32_t threadphoton, oddphotons;
+    uint32_t nthread;
+    /* buffers */
+    const void* media;            /* uint8 (label | det<<7) or uint32 (label | DET_MASK) */
+    const float* gprop;           /* ntable x 4 floats */
+    float* field;                 /* tally (FP32 modes) */
+    float* field2;                /* shadow (mode 0) or lo (mode 1) */
+    double* field64;              /* mode 2 */
+    float* genergy;               /* 2*nthread */
+    const uint32_t* seeds;        /* 4*nthread */
+    float* detdata;               /* maxdetphoton*reclen */
+    uint32_t* detcount;           /* [0]=detected, [1]=error exits, [2]=lanes done */
+    uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
+};
+
+#endif

@@ -1,0 +1,810 @@
+/*
+ * mcx_kernel.hip -- photon-transport kernel for MI355X (gfx950, wave64).
+ *
+ * One lane owns one xorshift128+ stream and runs its share of photons one
+ * after another, exactly as the reference's virtual threads do
+ * (src/mcx_core.cu:1767-2528, :3014-3022); the per-step float work follows
+ * the reference's order of operations, with the elementary functions of
+ * mcx_math.h, so a lane's trajectories are reproducible bit for bit by the
+ * CPU restatement used in the tests.
+ *
+ * MI355X choices:
+ *  - all launch constants arrive in the kernel argument block (scalar
+ *    registers, uniform branches) instead of constant memory;
+ *  - the media/detector property table is copied once per workgroup into
+ *    LDS and read with ds_read_b128 by the per-lane medium index;
+ *  - the label volume is uint8 (label | detector bit 7) when the labels fit,
+ *    4x less traffic than the reference's uint32 volume;
+ *  - detector partial paths live in LDS in lane-interleaved (SoA) order,
+ *    so a wave's accesses hit 64 consecutive dwords (no bank conflicts);
+ *  - fluence deposits use hardware global_atomic_add_f32 (returning only
+ *    where the MAX_ACCUM shadow needs the old value) or f64 atomics;
+ *  - detected photons reserve record slots with one atomic per wave
+ *    (ballot + prefix count) instead of one per lane.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mcx_kernel.h"
+#include "mcx_math.h"
+
+namespace mcxk {
+
+constexpr float EPSF = 1.1920928955078125e-07f;
+constexpr uint32_t DET_MASK = 0x80000000u;
+constexpr uint32_t MED_MASK = 0x7FFFFFFFu;
+constexpr uint32_t OUT_MIN = 0xFFFFFFFFu;
+constexpr uint32_t OUT_MAX = 0x7FFFFFFFu;
+constexpr float MAX_ACCUM = 1000.f;
+constexpr float ROULETTE = 10.f;
+enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
+enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL };
+
+struct F4 { float x, y, z, w; };
+
+template <typename MT>
+__device__ __forceinline__ uint32_t load_media(const void* m, uint32_t i);
+
+template <>
+__device__ __forceinline__ uint32_t load_media<uint8_t>(const void* m, uint32_t i) {
+    uint32_t b = ((const uint8_t*)m)[i];
+    return (b & 0x7Fu) | ((b & 0x80u) << 24);
+}
+
+template <>
+__device__ __forceinline__ uint32_t load_media<uint32_t>(const void* m, uint32_t i) {
+    return ((const uint32_t*)m)[i];
+}
+
+__device__ __forceinline__ float sgnf(float x) { return (float)((x > 0.f) - (x < 0.f)); }
+
+__device__ __forceinline__ float nextafter_mcx(float a, int dir, uint32_t maxvoidstep) {
+    float f = a + (float)maxvoidstep;
+    uint32_t i = mcx_f2bits(f);
+    i += (uint32_t)dir ^ (i & 0x80000000u);
+    return mcx_bits2f(i) - (float)maxvoidstep;
+}
+
+template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
+struct Lane {
+    const KParam& kp;
+    const F4* tab;        /* LDS: properties then detectors */
+    float* pp;            /* LDS: this lane's partial-path column */
+    uint32_t stride;      /* blockDim.x */
+    uint32_t idx;
+    int nphoton;
+
+    F4 p, v, f;           /* f = {pscat, t, pathlen, ndone} */
+    float rv0, rv1, rv2;
+    F4 prop;
+    uint64_t t0, t1;
+    int fd0, fd1, fd2, fd3;
+    uint32_t idx1d, mediaid;
+    float w0;
+    float esc, launched, winit;
+    uint32_t nstep, ndep, nscat, nlaunch;
+
+    __device__ __forceinline__ Lane(const KParam& k, const F4* tb, float* ppb, uint32_t s, uint32_t id)
+        : kp(k), tab(tb), pp(ppb), stride(s), idx(id) {}
+
+    __device__ __forceinline__ float& ppath(uint32_t i) { return pp[i * stride]; }
+
+    __device__ __forceinline__ float rnd() {
+        uint64_t s1 = t0;
+        const uint64_t s0 = t1;
+        t0 = s0;
+        s1 ^= s1 << 23;
+        t1 = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+        uint32_t lo = (uint32_t)(t1 + s0);
+        return mcx_bits2f(0x3F800000u | (lo >> 9)) - 1.0f;
+    }
+    __device__ __forceinline__ float scatlen() { return -mcx_logf(rnd() + EPSF); }
+
+    __device__ __forceinline__ void setprop(uint32_t m) { prop = tab[m & MED_MASK]; }
+    __device__ __forceinline__ void calc_rv() {
+        rv0 = 1.f / v.x;
+        rv1 = 1.f / v.y;
+        rv2 = 1.f / v.z;
+    }
+
+    /* src/mcx_core.cu:493-507 */
+    __device__ __forceinline__ float hitgrid(const F4& q, int& i0, int& i1, int& i2, int& face, float r0, float r1, float r2) const {
+        float h0 = fabsf(((float)(i0 + (v.x > 0.f)) - q.x) * r0);
+        float h1 = fabsf(((float)(i1 + (v.y > 0.f)) - q.y) * r1);
+        float h2 = fabsf(((float)(i2 + (v.z > 0.f)) - q.z) * r2);
+        float dist = fminf(fminf(h0, h1), h2);
+        face = (dist == h0 ? 0 : (dist == h1 ? 1 : 2));
+        return dist;
+    }
+
+    /* src/mcx_core.cu:1011-1031 */
+    __device__ __forceinline__ void rotatevector(float stheta, float ctheta, float sphi, float cphi) {
+        if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+            float tmp0 = 1.f - v.z * v.z;
+            float tmp1 = stheta / __builtin_sqrtf(tmp0);
+            float nx = tmp1 * (v.x * v.z * cphi - v.y * sphi) + v.x * ctheta;
+            float ny = tmp1 * (v.y * v.z * cphi + v.x * sphi) + v.y * ctheta;
+            float nz = -tmp1 * tmp0 * cphi + v.z * ctheta;
+            v.x = nx; v.y = ny; v.z = nz;
+        } else {
+            float nz = (v.z > 0.f) ? ctheta : -ctheta;
+            v.x = stheta * cphi;
+            v.y = stheta * sphi;
+            v.z = nz;
+        }
+        float tmp0 = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+    }
+
+    /* src/mcx_core.cu:970-999 */
+    __device__ __forceinline__ void rotatevector2d(float stheta, float ctheta) {
+        if (kp.is2d == 1) {
+            float ny = v.y * ctheta - v.z * stheta, nz = v.y * stheta + v.z * ctheta;
+            v.x = 0.f; v.y = ny; v.z = nz;
+        } else if (kp.is2d == 2) {
+            float nx = v.x * ctheta - v.z * stheta, nz = v.x * stheta + v.z * ctheta;
+            v.x = nx; v.y = 0.f; v.z = nz;
+        } else {
+            float nx = v.x * ctheta - v.y * stheta, ny = v.x * stheta + v.y * ctheta;
+            v.x = nx; v.y = ny; v.z = 0.f;
+        }
+        float tmp0 = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+    }
+
+    __device__ __forceinline__ float vcomp(int i) const { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+    /* src/mcx_core.cu:523-537 */
+    __device__ __forceinline__ void transmit(float n1, float n2, int face) {
+        float tmp0 = n1 / n2;
+        v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+        if (face == 0) {
+            tmp0 = v.y * v.y + v.z * v.z;
+            v.x = (tmp0 < 1.f) ? __builtin_sqrtf(1.f - tmp0) * sgnf(v.x) : 0.f;
+        } else if (face == 1) {
+            tmp0 = v.x * v.x + v.z * v.z;
+            v.y = (tmp0 < 1.f) ? __builtin_sqrtf(1.f - tmp0) * sgnf(v.y) : 0.f;
+        } else {
+            tmp0 = v.x * v.x + v.y * v.y;
+            v.z = (tmp0 < 1.f) ? __builtin_sqrtf(1.f - tmp0) * sgnf(v.z) : 0.f;
+        }
+        tmp0 = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+    }
+
+    /* src/mcx_core.cu:552-570 */
+    __device__ __forceinline__ float reflectcoeff(float n1, float n2, int face) const {
+        float Icos = fabsf(vcomp(face));
+        float tmp0 = n1 * n1;
+        float tmp1 = n2 * n2;
+        float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+        if (tmp2 > 0.f) {
+            float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+            tmp2 = __builtin_sqrtf(tmp2);
+            float Im = 2.f * n1 * n2 * Icos * tmp2;
+            float Rtotal = (Re - Im) / (Re + Im);
+            Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+            Rtotal = (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+            return Rtotal;
+        }
+        return 1.f;
+    }
+
+    /* src/mcx_core.cu:258-270 */
+    __device__ __forceinline__ uint32_t finddetector() const {
+        for (uint32_t i = kp.maxmedia + 1; i < kp.maxmedia + kp.detnum + 1; i++) {
+            const F4 d = tab[i];
+            if ((d.x - p.x) * (d.x - p.x) + (d.y - p.y) * (d.y - p.y) + (d.z - p.z) * (d.z - p.z) < d.w * d.w)
+                return (d.w < 0.f) ? 0 : i - kp.maxmedia;
+        }
+        return 0;
+    }
+
+    /* src/mcx_core.cu:365-417, slot reservation aggregated per wave */
+    __device__ __forceinline__ void savedet(bool want, uint32_t isdet) {
+        int detid = 0;
+        if (want) detid = (isdet == OUT_MIN) ? -1 : (int)finddetector();
+        const bool hit = want && detid != 0;
+        const uint64_t mask = __ballot(hit);
+        if (mask == 0) return;
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(kp.detcount, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (!hit) return;
+        const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (slot < kp.maxdetphoton) {
+            float* rec = kp.detdata + (size_t)slot * kp.reclen;
+            uint32_t k = 0;
+            const uint32_t flag = kp.savedetflag;
+            if (flag & 1) rec[k++] = (float)detid;
+            for (uint32_t i = 0; i < kp.partialdata; i++) rec[k++] = ppath(i);
+            if ((flag >> 4) & 1) { rec[k++] = p.x; rec[k++] = p.y; rec[k++] = p.z; }
+            if ((flag >> 5) & 1) { rec[k++] = v.x; rec[k++] = v.y; rec[k++] = v.z; }
+            if ((flag >> 6) & 1) rec[k++] = winit;
+        } else if (kp.savedet == 3) {
+            atomicSub(kp.detcount, 1u);
+        }
+    }
+
+    /* src/mcx_core.cu:866-956 */
+    __device__ int skipvoid() {
+        int count = 1, i0, i1, i2, face = -1;
+        i0 = (int)floorf(p.x); i1 = (int)floorf(p.y); i2 = (int)floorf(p.z);
+        while (1) {
+            if ((uint32_t)i0 < kp.nx && (uint32_t)i1 < kp.ny && (uint32_t)i2 < kp.nz) {
+                uint32_t id = i2 * kp.dimxy + i1 * kp.nx + i0;
+                if (load_media<MT>(kp.media, id) & MED_MASK) {
+                    p.x -= v.x; p.y -= v.y; p.z -= v.z;
+                    i0 = (int)floorf(p.x); i1 = (int)floorf(p.y); i2 = (int)floorf(p.z);
+                    f.y -= kp.minaccumtime;
+                    id = i2 * kp.dimxy + i1 * kp.nx + i0;
+                    count = 0;
+                    while (!((uint32_t)i0 < kp.nx && (uint32_t)i1 < kp.ny && (uint32_t)i2 < kp.nz) ||
+                           !(load_media<MT>(kp.media, id) & MED_MASK)) {
+                        float dist = hitgrid(p, i0, i1, i2, face, rv0, rv1, rv2);
+                        f.y += kp.minaccumtime * dist;
+                        p.x = p.x + dist * v.x; p.y = p.y + dist * v.y; p.z = p.z + dist * v.z;
+                        if (face == 0) i0 += (v.x > 0.f ? 1 : -1);
+                        if (face == 1) i1 += (v.y > 0.f ? 1 : -1);
+                        if (face == 2) i2 += (v.z > 0.f ? 1 : -1);
+                        id = i2 * kp.dimxy + i1 * kp.nx + i0;
+                        if (count++ > 3) break;
+                    }
+                    f.y = kp.voidtime ? f.y : 0.f;
+                    const F4 h = tab[load_media<MT>(kp.media, id) & MED_MASK];
+                    if (kp.isspecular && h.w != tab[0].w) {
+                        p.w *= 1.f - reflectcoeff(tab[0].w, h.w, face);
+                        if (p.w > EPSF) transmit(tab[0].w, h.w, face);
+                    }
+                    return (int)id;
+                }
+            }
+            if ((p.x < 0.f && v.x <= 0.f) || (p.x >= kp.maxidx_x && v.x >= 0.f) ||
+                (p.y < 0.f && v.y <= 0.f) || (p.y >= kp.maxidx_y && v.y >= 0.f) ||
+                (p.z < 0.f && v.z <= 0.f) || (p.z >= kp.maxidx_z && v.z >= 0.f))
+                return -1;
+            p.x = p.x + v.x; p.y = p.y + v.y; p.z = p.z + v.z;
+            i0 = (int)floorf(p.x); i1 = (int)floorf(p.y); i2 = (int)floorf(p.z);
+            f.y += kp.minaccumtime;
+            if (count++ > (int)kp.maxvoidstep) return -1;
+        }
+    }
+
+    __device__ __forceinline__ void locate_launch() {
+        idx1d = (uint32_t)((int)floorf(p.z) * (int)kp.dimxy + (int)floorf(p.y) * (int)kp.nx + (int)floorf(p.x));
+        if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= kp.maxidx_x || p.y >= kp.maxidx_y || p.z >= kp.maxidx_z)
+            mediaid = 0;
+        else
+            mediaid = load_media<MT>(kp.media, idx1d);
+    }
+
+    /* source sampling for non-pencil sources (src/mcx_core.cu:1261-1641) */
+    __device__ void launch_source(bool& skip) {
+        const float* s1 = kp.srcparam1;
+        const float* s2 = kp.srcparam2;
+        int canfocus = 1;
+        float ox = kp.srcpos[0], oy = kp.srcpos[1], oz = kp.srcpos[2];
+        skip = false;
+        switch (kp.srctype) {
+            case 4: case 6: case 14: {
+                float rx = rnd();
+                float ry = rnd();
+                p.x = p.x + rx * s1[0] + ry * s2[0];
+                p.y = p.y + rx * s1[1] + ry * s2[1];
+                p.z = p.z + rx * s1[2] + ry * s2[2];
+                if (kp.srctype == 6) {
+                    float s, c;
+                    float ang = (floorf(s1[3]) * rx + floorf(s2[3]) * ry + s1[3] - floorf(s1[3]));
+                    mcx_sincos2pi(ang, &s, &c);
+                    p.w = kp.srcpos[3] * (c * (1.f - s2[3] + floorf(s2[3])) + 1.f) * 0.5f;
+                } else if (kp.srctype == 14) {
+                    p.x = kp.srcpos[0] + floorf(rx * s1[3]) * s1[0] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[0] / (s2[3] - 1.f);
+                    p.y = kp.srcpos[1] + floorf(rx * s1[3]) * s1[1] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[1] / (s2[3] - 1.f);
+                    p.z = kp.srcpos[2] + floorf(rx * s1[3]) * s1[2] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[2] / (s2[3] - 1.f);
+                }
+                locate_launch();
+                ox = ox + (s1[0] + s2[0]) * 0.5f;
+                oy = oy + (s1[1] + s2[1]) * 0.5f;
+                oz = oz + (s1[2] + s2[2]) * 0.5f;
+                break;
+            }
+            case 8: case 17: case 3: {
+                float sphi, cphi, r;
+                if (kp.srctype == 17 && (s1[2] > 0.f || s1[3] > 0.f)) {
+                    float phi = fabsf(s1[2] - s1[3]) * rnd() + fminf(s1[2], s1[3]);
+                    mcx_sincos2pi(phi * 0.159154943091895336f, &sphi, &cphi);
+                } else {
+                    mcx_sincos2pi(rnd(), &sphi, &cphi);
+                }
+                if (kp.srctype == 8 || kp.srctype == 17) {
+                    r = __builtin_sqrtf(rnd() * fabsf(s1[0] * s1[0] - s1[1] * s1[1]) + s1[1] * s1[1]);
+                } else if (fabsf(kp.srcdir[3]) < 1e-5f || fabsf(s1[1]) < 1e-5f) {
+                    r = __builtin_sqrtf(0.5f * scatlen()) * s1[0];
+                } else {
+                    r = s1[0] * s1[0] * 3.14159265358979f / s1[1];
+                    r = __builtin_sqrtf(0.5f * scatlen() * (1.f + (kp.srcdir[3] * kp.srcdir[3] / (r * r)))) * s1[0];
+                }
+                if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+                    float tmp0 = 1.f - v.z * v.z;
+                    float tmp1 = r / __builtin_sqrtf(tmp0);
+                    p.x = p.x + tmp1 * (v.x * v.z * cphi - v.y * sphi);
+                    p.y = p.y + tmp1 * (v.y * v.z * cphi + v.x * sphi);
+                    p.z = p.z - tmp1 * tmp0 * cphi;
+                } else {
+                    p.x += r * cphi;
+                    p.y += r * sphi;
+                }
+                locate_launch();
+                break;
+            }
+            case 1: case 2: case 7: {
+                float sphi, cphi, stheta, ctheta;
+                mcx_sincos2pi(rnd(), &sphi, &cphi);
+                if (kp.srctype == 2) {
+                    if (s1[1] > 0.f) {
+                        float ang = rnd() * s1[0];
+                        mcx_sincos2pi(ang * 0.159154943091895336f, &stheta, &ctheta);
+                    } else {
+                        float cmax, dummy;
+                        mcx_sincos2pi(s1[0] * 0.159154943091895336f, &dummy, &cmax);
+                        ctheta = rnd() * (1.f - cmax) + cmax;
+                        stheta = mcx_sin_from_cos(ctheta);
+                    }
+                } else if (kp.srctype == 1) {
+                    ctheta = 2.f * rnd() - 1.f;
+                    stheta = mcx_sin_from_cos(ctheta);
+                } else {
+                    mcx_sincos2pi(rnd() * 0.5f, &stheta, &ctheta);
+                }
+                rotatevector(stheta, ctheta, sphi, cphi);
+                canfocus = 0;
+                break;
+            }
+            default:
+                break;
+        }
+        if (fabsf(p.w) <= kp.minenergy) {
+            skip = true;
+            return;
+        }
+        if (canfocus) {
+            const float fw = kp.srcdir[3];
+            if (__builtin_isnan(fw)) {
+                float sphi, cphi;
+                mcx_sincos2pi(rnd(), &sphi, &cphi);
+                float ctheta = 2.f * rnd() - 1.f;
+                rotatevector(mcx_sin_from_cos(ctheta), ctheta, sphi, cphi);
+            } else if (fw < 0.f && __builtin_isinf(fw)) {
+                float sphi, cphi;
+                mcx_sincos2pi(rnd(), &sphi, &cphi);
+                float stheta = __builtin_sqrtf(rnd());
+                float ctheta = __builtin_sqrtf(1.f - stheta * stheta);
+                rotatevector(stheta, ctheta, sphi, cphi);
+            } else if (fw != 0.f) {
+                float Rn2 = (float)((fw > 0.f) - (fw < 0.f));
+                ox += fw * v.x;
+                oy += fw * v.y;
+                oz += fw * v.z;
+                v.x = Rn2 * (ox - p.x);
+                v.y = Rn2 * (oy - p.y);
+                v.z = Rn2 * (oz - p.z);
+                Rn2 = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                v.x *= Rn2; v.y *= Rn2; v.z *= Rn2;
+            }
+        }
+    }
+
+    /* src/mcx_core.cu:1089-1717: terminate the current photon, launch the
+     * next one.  Returns 0 launched, 1 lane finished, -1 launch failed. */
+    __device__ int launch(uint32_t isdet) {
+        w0 = 1.f;
+        if (kp.savedet == 3 && __hip_atomic_load(kp.detcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= kp.maxdetphoton)
+            return 1;
+        const bool live = fabsf(p.w) >= 0.f;
+        if (live) esc += p.w;
+        if (kp.savedet) {
+            savedet(live && (isdet & DET_MASK) == DET_MASK && mediaid == 0, isdet);
+            for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;
+        }
+        if ((int)f.w >= nphoton - 1) return 1;
+        do {
+            p = {kp.srcpos[0], kp.srcpos[1], kp.srcpos[2], kp.srcpos[3]};
+            v = {kp.srcdir[0], kp.srcdir[1], kp.srcdir[2], kp.srcdir[3]};
+            f = {0.f, 0.f, kp.minaccumtime, f.w};
+            idx1d = mcx_f2bits(kp.srcparam2[2]);
+            mediaid = mcx_f2bits(kp.srcparam2[3]);
+            if (!ISPENCIL) {
+                bool skip;
+                launch_source(skip);
+                if (skip) continue;
+            }
+            calc_rv();
+            if ((mediaid & MED_MASK) == 0) {
+                int i1 = skipvoid();
+                if (i1 >= 0) {
+                    idx1d = (uint32_t)i1;
+                    mediaid = load_media<MT>(kp.media, idx1d);
+                }
+            }
+            fd0 = (int)floorf(p.x); fd1 = (int)floorf(p.y); fd2 = (int)floorf(p.z);
+            w0 += 1.f;
+            if (w0 > (float)kp.maxvoidstep) return -1;
+        } while ((mediaid & MED_MASK) == 0 || fabsf(p.w) <= kp.minenergy);
+        f.w += 1.f;
+        setprop(mediaid);
+        launched += p.w;
+        w0 = p.w;
+        winit = p.w;
+        v.w = EPSF;
+        f.z = 0.f;
+        nlaunch++;
+        return 0;
+    }
+
+    __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight) {
+        const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
+        ndep++;
+        if (TALLY == MCX_TALLY_F32_SHADOW) {
+            float oldval = unsafeAtomicAdd(kp.field + i, weight);
+            if (fabsf(oldval) > MAX_ACCUM) {
+                unsafeAtomicAdd(kp.field + i, (oldval > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
+                unsafeAtomicAdd(kp.field2 + i, (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
+            }
+        } else if (TALLY == MCX_TALLY_F32_HILO) {
+            /* TwoSum of (old, weight): the atomic stored s = fl(old + weight);
+             * the rounding error e is exact and goes to the lo half. */
+            float a = unsafeAtomicAdd(kp.field + i, weight);
+            float s = a + weight;
+            float bb = s - a;
+            float e = (a - (s - bb)) + (weight - bb);
+            if (e != 0.f) unsafeAtomicAdd(kp.field2 + i, e);
+        } else {
+            unsafeAtomicAdd(kp.field64 + i, (double)weight);
+        }
+    }
+
+    /* src/mcx_core.cu:1767-2528, one lane */
+    __device__ void run() {
+        const uint32_t flag = kp.savedetflag;
+        const uint32_t NSCAT = (flag >> 1) & 1, PPATH = (flag >> 2) & 1, MOM = (flag >> 3) & 1;
+        p = {0.f, 0.f, 0.f, __builtin_nanf("")};
+        v = {0.f, 0.f, 0.f, 0.f};
+        f = {0.f, 0.f, 0.f, -1.f};
+        fd0 = fd1 = fd2 = 0;
+        fd3 = -1;
+        mediaid = mcx_f2bits(kp.srcparam2[3]);
+        idx1d = 0;
+        esc = kp.genergy[2 * idx];
+        launched = kp.genergy[2 * idx + 1];
+        winit = 0.f;
+        nstep = ndep = nscat = nlaunch = 0;
+        uint32_t idx1dold, mediaidold = 0, isdet = 0;
+        float n1, len, slen;
+        prop = tab[1];
+        t0 = ((uint64_t)kp.seeds[4 * idx] << 32) | kp.seeds[4 * idx + 1];
+        t1 = ((uint64_t)kp.seeds[4 * idx + 2] << 32) | kp.seeds[4 * idx + 3];
+        if (launch(0)) return;
+        calc_rv();
+        isdet = mediaid & DET_MASK;
+        mediaid &= MED_MASK;
+        while (f.w < (float)nphoton) {
+            nstep++;
+            if (f.x <= 0.f) {
+                f.x = scatlen();
+                if (v.w != EPSF) {
+                    float cphi = 1.f, sphi = 0.f, stheta, ctheta, tmp0;
+                    if (!kp.is2d) mcx_sincos2pi(rnd(), &sphi, &cphi);
+                    tmp0 = (v.w > (float)kp.gscatter) ? 0.f : prop.z;
+                    if (fabsf(tmp0) > EPSF) {
+                        const float g = prop.z;
+                        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * rnd());
+                        tmp0 *= tmp0;
+                        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
+                        ctheta = tmp0;
+                        stheta = mcx_sin_from_cos(ctheta);
+                    } else {
+                        ctheta = 2.f * rnd() - 1.f;
+                        stheta = mcx_sin_from_cos(ctheta);
+                    }
+                    if (kp.savedet) {
+                        if (NSCAT) ppath((mediaid & MED_MASK) - 1) += 1.f;
+                        if (MOM) ppath(kp.maxmedia * (NSCAT + PPATH) + (mediaid & MED_MASK) - 1) += 1.f - ctheta;
+                    }
+                    if (kp.is2d)
+                        rotatevector2d((rnd() > 0.5f ? stheta : -stheta), ctheta);
+                    else
+                        rotatevector(stheta, ctheta, sphi, cphi);
+                    v.w += 1.f;
+                    calc_rv();
+                    nscat++;
+                }
+                v.w = (float)(int)v.w;
+            }
+            n1 = prop.w;
+            setprop(mediaid);
+            len = hitgrid(p, fd0, fd1, fd2, fd3, rv0, rv1, rv2);
+            const bool gs = (v.w + 1.f > (float)kp.gscatter);
+            slen = len * prop.y * (gs ? (1.f - prop.z) : 1.f);
+            slen = fminf(slen, f.x);
+            len = ((prop.y == 0.f) ? len : (slen / (prop.y * (gs ? (1.f - prop.z) : 1.f))));
+            p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
+            const bool stop = (slen == f.x);
+            if (fd3 == 0) fd0 += stop ? 0 : (v.x > 0.f ? 1 : -1);
+            if (fd3 == 1) fd1 += stop ? 0 : (v.y > 0.f ? 1 : -1);
+            if (fd3 == 2) fd2 += stop ? 0 : (v.z > 0.f ? 1 : -1);
+            p.w *= mcx_expf(-prop.x * len);
+            f.x -= slen;
+            f.y += len * prop.w * kp.oneoverc0;
+            f.z += len;
+            if (kp.savedet && PPATH) ppath(kp.maxmedia * NSCAT + (mediaid & MED_MASK) - 1) += len;
+            mediaidold = mediaid | (isdet & DET_MASK);
+            idx1dold = idx1d;
+            idx1d = (uint32_t)(fd2 * (int)kp.dimxy + fd1 * (int)kp.nx + fd0);
+            if ((uint32_t)fd0 >= kp.nx || (uint32_t)fd1 >= kp.ny || (uint32_t)fd2 >= kp.nz) {
+                mediaid = 0;
+                idx1d = (fd0 < 0 || fd1 < 0 || fd2 < 0) ? OUT_MIN : OUT_MAX;
+                isdet = kp.bc[(idx1d == OUT_MAX) * 3 + fd3];
+            } else {
+                mediaid = load_media<MT>(kp.media, idx1d);
+                isdet = mediaid & DET_MASK;
+                mediaid &= MED_MASK;
+            }
+            if (idx1d != idx1dold && mediaidold) {
+                if (kp.save2pt && f.y >= kp.twin0 && f.y < kp.twin1) {
+                    float weight = 0.f;
+                    int tshift = (int)floorf((f.y - kp.twin0) * kp.Rtstep);
+                    if (tshift > (int)kp.maxgate - 1) tshift = (int)kp.maxgate - 1;
+                    if (kp.outputtype == otEnergy)
+                        weight = w0 - p.w;
+                    else if (kp.outputtype == otFluence || kp.outputtype == otFlux)
+                        weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
+                    else if (kp.outputtype == otL)
+                        weight = w0 * f.z;
+                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
+                }
+                w0 = p.w;
+                f.z = 0.f;
+            } else {
+                mediaid = mediaidold;
+            }
+            const uint32_t bcf = isdet & 0xF;
+            if ((mediaid == 0 &&
+                 ((!ISREFLECT || (ISREFLECT && n1 == tab[0].w)) || (bcf == bcUnknown && !kp.doreflect) ||
+                  bcf == bcAbsorb || bcf == bcCyclic) &&
+                 bcf != bcMirror) ||
+                f.y > kp.twin1) {
+                if (isdet == bcCyclic) {
+                    if (fd3 == 0) {
+                        p.x = nextafter_mcx(roundf(p.x + ((idx1d == OUT_MIN) ? kp.maxidx_x : -kp.maxidx_x)), (v.x > 0.f) - (v.x < 0.f), kp.maxvoidstep);
+                        fd0 = (int)floorf(p.x);
+                    }
+                    if (fd3 == 1) {
+                        p.y = nextafter_mcx(roundf(p.y + ((idx1d == OUT_MIN) ? kp.maxidx_y : -kp.maxidx_y)), (v.y > 0.f) - (v.y < 0.f), kp.maxvoidstep);
+                        fd1 = (int)floorf(p.y);
+                    }
+                    if (fd3 == 2) {
+                        p.z = nextafter_mcx(roundf(p.z + ((idx1d == OUT_MIN) ? kp.maxidx_z : -kp.maxidx_z)), (v.z > 0.f) - (v.z < 0.f), kp.maxvoidstep);
+                        fd2 = (int)floorf(p.z);
+                    }
+                    if ((uint32_t)fd0 < kp.nx && (uint32_t)fd1 < kp.ny && (uint32_t)fd2 < kp.nz) {
+                        idx1d = (uint32_t)(fd2 * (int)kp.dimxy + fd1 * (int)kp.nx + fd0);
+                        mediaid = load_media<MT>(kp.media, idx1d);
+                        isdet = mediaid & DET_MASK;
+                        mediaid &= MED_MASK;
+                        continue;
+                    }
+                }
+                const uint32_t detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                                             ? OUT_MIN : (mediaidold & DET_MASK));
+                if (launch(detarg)) break;
+                isdet = mediaid & DET_MASK;
+                mediaid &= MED_MASK;
+                continue;
+            }
+            if (fabsf(p.w) < kp.minenergy) {
+                if (rnd() * ROULETTE <= 1.f) {
+                    p.w *= ROULETTE;
+                } else {
+                    if (launch(mediaidold & DET_MASK)) break;
+                    isdet = mediaid & DET_MASK;
+                    mediaid &= MED_MASK;
+                    continue;
+                }
+            }
+            if (ISREFLECT) {
+                setprop(mediaid);
+                if (((mediaid && kp.doreflect) ||
+                     (mediaid == 0 && ((bcf == bcUnknown && kp.doreflect) || bcf == bcReflect || bcf == bcMirror))) &&
+                    (bcf == bcMirror || n1 != prop.w)) {
+                    float Rtotal = 1.f;
+                    float cphi, sphi, stheta, ctheta, tmp0, tmp1;
+                    tmp0 = n1 * n1;
+                    tmp1 = prop.w * prop.w;
+                    cphi = fabsf(vcomp(fd3));
+                    sphi = 1.f - cphi * cphi;
+                    len = 1.f - tmp0 / tmp1 * sphi;
+                    if (len > 0.f && bcf != bcMirror) {
+                        ctheta = tmp0 * cphi * cphi + tmp1 * len;
+                        stheta = 2.f * n1 * prop.w * cphi * __builtin_sqrtf(len);
+                        Rtotal = (ctheta - stheta) / (ctheta + stheta);
+                        ctheta = tmp1 * cphi * cphi + tmp0 * len;
+                        Rtotal = (Rtotal + (ctheta - stheta) / (ctheta + stheta)) * 0.5f;
+                    }
+                    if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && rnd() > Rtotal) {
+                        transmit(n1, prop.w, fd3);
+                        if (mediaid == 0) {
+                            const uint32_t detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                                                         ? OUT_MIN : (mediaidold & DET_MASK));
+                            if (launch(detarg)) break;
+                            isdet = mediaid & DET_MASK;
+                            mediaid &= MED_MASK;
+                            continue;
+                        }
+                        calc_rv();
+                    } else {
+                        if (fd3 == 0) {
+                            v.x = -v.x;
+                        } else if (fd3 == 1) {
+                            v.y = -v.y;
+                        } else {
+                            v.z = -v.z;
+                        }
+                        calc_rv();
+                        if (fd3 == 0) {
+                            p.x = nextafter_mcx(__builtin_rintf(p.x), (v.x > 0.f) - (v.x < 0.f), kp.maxvoidstep);
+                            fd0 = (int)floorf(p.x);
+                        } else if (fd3 == 1) {
+                            p.y = nextafter_mcx(__builtin_rintf(p.y), (v.y > 0.f) - (v.y < 0.f), kp.maxvoidstep);
+                            fd1 = (int)floorf(p.y);
+                        } else {
+                            p.z = nextafter_mcx(__builtin_rintf(p.z), (v.z > 0.f) - (v.z < 0.f), kp.maxvoidstep);
+                            fd2 = (int)floorf(p.z);
+                        }
+                        idx1d = idx1dold;
+                        mediaid = load_media<MT>(kp.media, idx1d) & MED_MASK;
+                        setprop(mediaid);
+                        n1 = prop.w;
+                    }
+                } else {
+                    setprop(mediaidold);
+                }
+            }
+            if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
+                atomicAdd(kp.detcount + 1, 1u);
+                return;
+            }
+        }
+        kp.genergy[2 * idx] = esc;
+        kp.genergy[2 * idx + 1] = launched;
+    }
+};
+
+}  // namespace mcxk
+
+using namespace mcxk;
+
+template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
+__global__ void __launch_bounds__(256) mcx_transport(KParam kp) {
+    extern __shared__ F4 lds[];
+    for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
+        const float* g = kp.gprop + 4 * i;
+        lds[i] = F4{g[0], g[1], g[2], g[3]};
+    }
+    __syncthreads();
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
+    if ((long long)idx >= total || idx >= kp.nthread) return;
+    float* ppbase = (float*)(lds + kp.ntable) + threadIdx.x;
+    Lane<ISPENCIL, ISREFLECT, TALLY, MT> L(kp, lds, ppbase, blockDim.x, idx);
+    L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
+    for (uint32_t i = 0; i < kp.partialdata; i++) L.ppath(i) = 0.f;
+    L.run();
+    if (kp.counters) {
+        atomicAdd((unsigned long long*)kp.counters + 0, (unsigned long long)L.nstep);
+        atomicAdd((unsigned long long*)kp.counters + 1, (unsigned long long)L.ndep);
+        atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)L.nscat);
+        atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)L.nlaunch);
+    }
+}
+
+/* src/mcx_core.cu:1731-1742: n draws of one lane's stream */
+__global__ void mcx_test_rng(float* out, const uint32_t* seeds, uint32_t n) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t t0 = ((uint64_t)seeds[0] << 32) | seeds[1];
+    uint64_t t1 = ((uint64_t)seeds[2] << 32) | seeds[3];
+    for (uint32_t i = 0; i < n; i++) {
+        uint64_t s1 = t0;
+        const uint64_t s0 = t1;
+        t0 = s0;
+        s1 ^= s1 << 23;
+        t1 = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+        uint32_t lo = (uint32_t)(t1 + s0);
+        out[i] = mcx_bits2f(0x3F800000u | (lo >> 9)) - 1.0f;
+    }
+}
+
+/* Tally merge, vectorised: out = raw + shadow (mode 0, FP32 as the
+ * reference's host loop src/mcx_core.cu:3687-3696), or the FP64 sum of the
+ * hi/lo pair rounded once to FP32 (mode 1), or FP64 rounded (mode 2). */
+__global__ void mcx_merge_f32(float* __restrict__ out, const float* __restrict__ a, const float* __restrict__ b, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = a[i] + b[i];
+}
+
+__global__ void mcx_merge_hilo(float* __restrict__ out, const float* __restrict__ hi, const float* __restrict__ lo, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)((double)hi[i] + (double)lo[i]);
+}
+
+__global__ void mcx_merge_f64(float* __restrict__ out, const double* __restrict__ d, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)d[i];
+}
+
+/* ---- host-side dispatch (called from mcx_host.cpp) ------------------- */
+
+typedef void (*mcx_kernel_fn)(KParam);
+
+template <int P, int R, int T>
+static mcx_kernel_fn pick_mt(int media32) {
+    return media32 ? (mcx_kernel_fn)mcx_transport<P, R, T, uint32_t> : (mcx_kernel_fn)mcx_transport<P, R, T, uint8_t>;
+}
+
+template <int P, int R>
+static mcx_kernel_fn pick_tally(int tally, int media32) {
+    switch (tally) {
+        case MCX_TALLY_F32_HILO: return pick_mt<P, R, MCX_TALLY_F32_HILO>(media32);
+        case MCX_TALLY_F64: return pick_mt<P, R, MCX_TALLY_F64>(media32);
+        default: return pick_mt<P, R, MCX_TALLY_F32_SHADOW>(media32);
+    }
+}
+
+extern "C" hipError_t mcxk_launch_transport(const KParam* kp, int ispencil, int isreflect, int tally, int media32,
+                                            unsigned int grid, unsigned int block, size_t shmem, hipStream_t stream) {
+    mcx_kernel_fn fn;
+    if (ispencil)
+        fn = isreflect ? pick_tally<1, 1>(tally, media32) : pick_tally<1, 0>(tally, media32);
+    else
+        fn = isreflect ? pick_tally<0, 1>(tally, media32) : pick_tally<0, 0>(tally, media32);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), shmem, stream, *kp);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads) {
+    mcx_kernel_fn fn;
+    if (ispencil)
+        fn = isreflect ? pick_tally<1, 1>(tally, media32) : pick_tally<1, 0>(tally, media32);
+    else
+        fn = isreflect ? pick_tally<0, 1>(tally, media32) : pick_tally<0, 0>(tally, media32);
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, (const void*)fn);
+    if (e == hipSuccess) {
+        *numregs = a.numRegs;
+        *maxthreads = a.maxThreadsPerBlock;
+    }
+    return e;
+}
+
+extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(mcx_test_rng, dim3(1), dim3(64), 0, stream, out, seeds, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, size_t n, hipStream_t stream) {
+    const unsigned int block = 256;
+    size_t g = (n + block - 1) / block;
+    unsigned int grid = (unsigned int)(g > 65536 ? 65536 : (g ? g : 1));
+    if (tally == MCX_TALLY_F32_HILO)
+        hipLaunchKernelGGL(mcx_merge_hilo, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b, n);
+    else if (tally == MCX_TALLY_F64)
+        hipLaunchKernelGGL(mcx_merge_f64, dim3(grid), dim3(block), 0, stream, out, (const double*)a, n);
+    else
+        hipLaunchKernelGGL(mcx_merge_f32, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,303 @@
+"""pmcx-compatible Python interface: run(cfg) -> dict, gpuinfo(), version().
+
+Mirrors the reference's binding (src/pmcx.cpp): parse_config (:400-1052)
+turns the dict into a Config through the C ABI, mcx_list_gpu (:1089) and
+mcx_validatecfg (:1096) prepare it, mcx_run_simulation (:1143-1170) runs the
+HIP engine, and the result dict carries the same keys and array layouts:
+flux float32 [srcnum*Nx, Ny, Nz, Ngates] (F-order), detp float32
+[reclen, ndetected] (F-order), vol (with dumpmask), and stat {runtime,
+nphoton, energytot, energyabs, normalizer, unitinmm, workload}.
+Errors raise ValueError for bad fields and RuntimeError for engine errors,
+as the reference does (:1327-1351).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, Float4, GPUInfo, Medium, mpPython
+
+_libc = ctypes.CDLL(None)
+_libc.malloc.restype = ctypes.c_void_p
+_libc.malloc.argtypes = [ctypes.c_size_t]
+_libc.calloc.restype = ctypes.c_void_p
+_libc.calloc.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
+_c_stderr = ctypes.c_void_p.in_dll(_libc, "stderr")
+
+SRCTYPES = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier", "arcsine", "disk", "fourierx",
+            "fourierx2d", "zgaussian", "line", "slit", "pencilarray", "pattern3d", "hyperboloid", "ring"]
+OUTPUTTYPES = ["flux", "fluence", "energy", "jacobian", "nscat", "wl", "wp", "wm", "rf", "length", "rfmus", "wltof",
+               "wptof"]
+SAVEFLAGS = "DSPMXVWI"
+DEBUGFLAGS = "RMP"
+
+_INT_FIELDS = ["nphoton", "nblocksize", "nthread", "maxdetphoton", "maxgate", "respin", "isreflect", "isref3",
+               "isrefint", "isnormalized", "issrcfrom0", "autopilot", "printnum", "voidtime", "issaveref",
+               "replaydet", "maxvoidstep", "maxjumpdebug", "gscatter", "srcnum", "srcid", "issave2pt", "issavedet",
+               "issaveseed"]
+_BOOL_FIELDS = ["issaveexit", "ismomentum", "isspecular", "istrajstokes", "faststep"]
+_FLOAT_FIELDS = ["tstart", "tstep", "tend", "sradius", "minenergy", "unitinmm", "omega"]
+
+
+def _cmalloc(nbytes):
+    p = _libc.malloc(max(int(nbytes), 1))
+    if not p:
+        raise MemoryError("malloc failed")
+    return p
+
+
+def _copy_to_c(arr, ctype):
+    a = np.ascontiguousarray(arr)
+    p = _cmalloc(a.nbytes)
+    ctypes.memmove(p, a.ctypes.data, a.nbytes)
+    return ctypes.cast(p, ctypes.POINTER(ctype))
+
+
+def _set_vec(dst, values, n, name):
+    v = np.asarray(values, dtype=np.float64).ravel()
+    if v.size < 3 or v.size > n:
+        raise ValueError("the '%s' field must have 3%s elements" % (name, " or 4" if n == 4 else ""))
+    for i, k in enumerate("xyzw"[: v.size]):
+        setattr(dst, k, float(v[i]))
+
+
+def _check():
+    code, msg = _lib.last_error()
+    if code:
+        _lib.lib().mcx_clear_error()
+        raise RuntimeError("MCX ERROR(%d): %s" % (code, msg))
+
+
+def _parse_config(user_cfg, cfg):
+    L = _lib.lib()
+    L.mcx_initcfg(ctypes.byref(cfg))
+    cfg.parentid = mpPython
+    cfg.flog = None if user_cfg.get("quiet", False) else _c_stderr.value
+    for k in _INT_FIELDS:
+        if k in user_cfg:
+            setattr(cfg, k, int(user_cfg[k]))
+    for k in _BOOL_FIELDS:
+        if k in user_cfg:
+            setattr(cfg, k, 1 if bool(user_cfg[k]) else 0)
+    for k in _FLOAT_FIELDS:
+        if k in user_cfg:
+            setattr(cfg, k, float(user_cfg[k]))
+    if "lambda" in user_cfg:
+        cfg.lambda_ = float(user_cfg["lambda"])
+    if "steps" in user_cfg:
+        _set_vec(cfg.steps, user_cfg["steps"], 3, "steps")
+    if "vol" not in user_cfg:
+        raise ValueError("Configuration must specify a 2/3/4D volume.")
+    vol = np.asarray(user_cfg["vol"])
+    if vol.ndim == 2:
+        vol = vol[:, :, None]
+    if vol.ndim != 3:
+        raise ValueError("only 2D/3D label volumes are supported by this engine")
+    if vol.dtype in (np.int8, np.uint8, np.bool_):
+        cfg.mediabyte = 1
+    elif vol.dtype in (np.int16, np.uint16):
+        cfg.mediabyte = 2
+    elif vol.dtype in (np.int32, np.uint32, np.int64, np.uint64):
+        cfg.mediabyte = 4
+    elif vol.dtype in (np.float32, np.float64):
+        if not np.all(vol == np.round(vol)):
+            raise ValueError("floating-point media formats are not supported by this engine")
+        cfg.mediabyte = 4
+    else:
+        raise ValueError("unsupported volume data type %s" % vol.dtype)
+    cfg.dim.x, cfg.dim.y, cfg.dim.z = (int(s) for s in vol.shape)
+    flat = np.asfortranarray(vol).ravel(order="F").astype(np.uint32)
+    cfg.vol = _copy_to_c(flat, ctypes.c_uint)
+    if "srcpos" in user_cfg:
+        sp = np.asarray(user_cfg["srcpos"], dtype=np.float64)
+        if sp.ndim > 1:
+            if sp.shape[0] > 1:
+                raise ValueError("multiple sources are not supported by this engine")
+            sp = sp[0]
+        if sp.size < 3 or sp.size > 4:
+            raise ValueError("the 'srcpos' field must have 3 or 4 columns (x,y,z,w0)")
+        _set_vec(cfg.srcpos, sp, 4, "srcpos")
+    if "srcdir" in user_cfg:
+        sd = np.asarray(user_cfg["srcdir"], dtype=np.float64).ravel()
+        if sd.size < 3 or sd.size > 4:
+            raise ValueError("the 'srcdir' field must have 3 or 4 columns")
+        _set_vec(cfg.srcdir, sd, 4, "srcdir")
+    for k in ("srcparam1", "srcparam2", "srciquv"):
+        if k in user_cfg:
+            v = np.asarray(user_cfg[k], dtype=np.float64).ravel()
+            dst = getattr(cfg, k)
+            for i, c in enumerate("xyzw"[: min(4, v.size)]):
+                setattr(dst, c, float(v[i]))
+    if "detpos" in user_cfg:
+        dp = np.atleast_2d(np.asarray(user_cfg["detpos"], dtype=np.float32))
+        if dp.size:
+            if dp.shape[1] < 3 or dp.shape[1] > 4:
+                raise ValueError("the 'detpos' field must have 3 or 4 columns (x,y,z,radius)")
+            d4 = np.zeros((dp.shape[0], 4), dtype=np.float32)
+            d4[:, : dp.shape[1]] = dp
+            if dp.shape[1] == 3:
+                d4[:, 3] = cfg.detradius
+            cfg.detnum = dp.shape[0]
+            cfg.detpos = _copy_to_c(d4, Float4)
+    if "prop" in user_cfg:
+        pr = np.atleast_2d(np.asarray(user_cfg["prop"], dtype=np.float32))
+        if pr.shape[1] != 4:
+            raise ValueError("the 'prop' field must have 4 columns (mua,mus,g,n)")
+        cfg.medianum = pr.shape[0]
+        cfg.prop = _copy_to_c(pr, Medium)
+    if "polprop" in user_cfg:
+        raise ValueError("polarized photon simulation is not supported by this engine")
+    if "session" in user_cfg:
+        s = str(user_cfg["session"])
+        if not s or len(s) > 255:
+            raise ValueError("the 'session' field must be a non-empty string")
+        cfg.session = s.encode()
+    if "srctype" in user_cfg:
+        s = str(user_cfg["srctype"]).lower()
+        if s not in SRCTYPES:
+            raise ValueError("the specified source type is not supported")
+        cfg.srctype = SRCTYPES.index(s)
+    if "outputtype" in user_cfg:
+        s = str(user_cfg["outputtype"]).lower()
+        if s not in OUTPUTTYPES:
+            raise ValueError("the specified output type is not supported")
+        t = OUTPUTTYPES.index(s)
+        cfg.outputtype = t - 2 if t >= 5 else t
+    if "debuglevel" in user_cfg:
+        s = str(user_cfg["debuglevel"]).upper()
+        lvl = sum(1 << i for i, c in enumerate(DEBUGFLAGS) if c in s)
+        if lvl == 0:
+            raise ValueError("the specified debuglevel is not supported")
+        cfg.debuglevel = lvl
+    if "savedetflag" in user_cfg:
+        v = user_cfg["savedetflag"]
+        if isinstance(v, str):
+            cfg.savedetflag = sum(1 << i for i, c in enumerate(SAVEFLAGS) if c in v.upper())
+        else:
+            cfg.savedetflag = int(v)
+    if "srcpattern" in user_cfg:
+        raise ValueError("pattern sources are not supported by this engine")
+    if "invcdf" in user_cfg or "angleinvcdf" in user_cfg:
+        raise ValueError("user-defined inverse CDFs are not supported by this engine")
+    if "shapes" in user_cfg:
+        s = str(user_cfg["shapes"]).encode() + b"\0"
+        p = _cmalloc(len(s) + 1)
+        ctypes.memmove(p, s, len(s))
+        cfg.shapedata = p
+    if "bc" in user_cfg:
+        s = str(user_cfg["bc"])
+        if not s or len(s) > 12:
+            raise ValueError("the 'bc' field must be a non-empty string with at most 12 characters")
+        cfg.bc = s.encode()
+    if "detphotons" in user_cfg:
+        raise ValueError("photon replay is not supported by this engine")
+    if "seed" in user_cfg:
+        sv = user_cfg["seed"]
+        if isinstance(sv, (int, np.integer, float, np.floating)):
+            cfg.seed = int(sv)
+        else:
+            raise ValueError("photon replay (array seeds) is not supported by this engine")
+    if "gpuid" in user_cfg:
+        g = user_cfg["gpuid"]
+        cfg.deviceid = b"\0" * 256
+        if isinstance(g, (int, np.integer)):
+            if not (0 < int(g) < 256):
+                raise ValueError("GPU id must be positive and can not be more than 256")
+            cfg.gpuid = int(g)
+            cfg.deviceid = b"0" * (int(g) - 1) + b"1"
+        else:
+            cfg.deviceid = str(g).encode()[:256]
+    if "workload" in user_cfg:
+        w = np.asarray(user_cfg["workload"], dtype=np.float32).ravel()
+        for i, x in enumerate(w[:256]):
+            cfg.workload[i] = float(x)
+
+
+def _reclen(cfg):
+    f = cfg.savedetflag
+    partial = (cfg.medianum - 1) * (((f >> 1) & 1) + ((f >> 2) & 1) + ((f >> 3) & 1))
+    return partial + (f & 1) + 3 * (((f >> 4) & 1) + ((f >> 5) & 1)) + ((f >> 6) & 1) + 4 * ((f >> 7) & 1)
+
+
+def run(cfg=None, **kwargs):
+    """Run one simulation; cfg is a dict of pmcx fields (or keyword args)."""
+    user_cfg = dict(cfg or {})
+    user_cfg.update(kwargs)
+    L = _lib.lib()
+    c = Config()
+    gpu = ctypes.POINTER(GPUInfo)()
+    output = {}
+    try:
+        _parse_config(user_cfg, c)
+        ndev = L.mcx_list_gpu(ctypes.byref(c), ctypes.byref(gpu))
+        _check()
+        if not ndev:
+            raise RuntimeError("No GPU device found")
+        L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+        _check()
+        if not c.vol or c.medianum == 0:
+            raise ValueError("You must define 'vol' and 'prop' field.")
+        ngates = int((c.tend - c.tstart) / c.tstep + 0.5)
+        nvox = c.dim.x * c.dim.y * c.dim.z
+        reclen = _reclen(c)
+        if c.issave2pt:
+            c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates * c.srcnum, 4), ctypes.POINTER(ctypes.c_float))
+        if c.issavedet:
+            c.exportdetected = ctypes.cast(_cmalloc(reclen * c.maxdetphoton * 4), ctypes.POINTER(ctypes.c_float))
+        if user_cfg.get("dumpmask", False):
+            output["vol"] = np.ctypeslib.as_array(c.vol, shape=(nvox,)).copy().reshape(
+                (c.dim.x, c.dim.y, c.dim.z), order="F")
+        L.mcx_run_simulation(ctypes.byref(c), gpu)
+        _check()
+        if c.issavedet:
+            n = int(c.detectedcount)
+            if n > 0:
+                d = np.ctypeslib.as_array(c.exportdetected, shape=(n * reclen,)).copy()
+                output["detp"] = np.asfortranarray(d.reshape((n, reclen)).T)
+        if c.issave2pt:
+            f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum,)).copy()
+            output["flux"] = f.reshape((c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates), order="F")
+            output["stat"] = {
+                "runtime": int(c.runtime),
+                "nphoton": int(c.nphoton) * (c.respin if c.respin > 1 else 1),
+                "energytot": float(c.energytot),
+                "energyabs": float(c.energyabs),
+                "normalizer": float(c.normalizer),
+                "unitinmm": float(c.unitinmm),
+                "workload": [float(c.workload[i]) for i in range(max(ndev, 1))],
+            }
+    finally:
+        if gpu:
+            L.mcx_cleargpuinfo(ctypes.byref(gpu))
+        L.mcx_clearcfg(ctypes.byref(c))
+    return output
+
+
+def gpuinfo():
+    """List HIP devices as dicts (src/pmcx.cpp:1404-1439)."""
+    L = _lib.lib()
+    c = Config()
+    L.mcx_initcfg(ctypes.byref(c))
+    c.parentid = mpPython
+    c.flog = None
+    c.isgpuinfo = 3
+    gpu = ctypes.POINTER(GPUInfo)()
+    out = []
+    n = L.mcx_list_gpu(ctypes.byref(c), ctypes.byref(gpu))
+    try:
+        if n and gpu:
+            for i in range(gpu[0].devcount):
+                g = gpu[i]
+                out.append({"name": g.name.decode(errors="replace"), "id": g.id, "devcount": g.devcount,
+                            "major": g.major, "minor": g.minor, "globalmem": g.globalmem, "constmem": g.constmem,
+                            "sharedmem": g.sharedmem, "regcount": g.regcount, "clock": g.clock, "sm": g.sm,
+                            "core": g.core, "autoblock": g.autoblock, "autothread": g.autothread,
+                            "maxgate": g.maxgate})
+    finally:
+        if gpu:
+            L.mcx_cleargpuinfo(ctypes.byref(gpu))
+    return out
+
+
+def version():
+    return "v2025.10-mi355x"

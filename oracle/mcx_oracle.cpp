@@ -1,0 +1,837 @@
+/*
+ * mcx_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the reference's transport kernel (label media,
+ * non-polarised, no split voxels), one OpenMP iteration per reference
+ * virtual thread, each owning its xorshift128+ stream.  Used by tests/ to
+ * check the HIP kernel and by bench.py's cpu_baseline leg.  The product
+ * never links this file.
+ *
+ * Followed files (reference, read as text):
+ *   src/mcx_core.cu:1767-2528  mcx_main_loop        -> run_thread()
+ *   src/mcx_core.cu:1089-1717  launchnewphoton      -> launch()
+ *   src/mcx_core.cu:866-956    skipvoid             -> skipvoid()
+ *   src/mcx_core.cu:493-507    hitgrid              -> hitgrid()
+ *   src/mcx_core.cu:523-537    transmit             -> transmit()
+ *   src/mcx_core.cu:552-570    reflectcoeff         -> reflectcoeff()
+ *   src/mcx_core.cu:470-478    mcx_nextafterf       -> nextafter_mcx()
+ *   src/mcx_core.cu:1011-1031  rotatevector         -> rotatevector()
+ *   src/mcx_core.cu:970-999    rotatevector2d       -> rotatevector2d()
+ *   src/mcx_core.cu:258-270    finddetector         -> finddetector()
+ *   src/mcx_core.cu:365-417    savedetphoton        -> savedet()
+ *   src/mcx_core.cu:2191-2292  fluence tally        -> deposit()
+ *   src/mcx_rand_xorshift128p.cu:56-124  RNG         -> Rng
+ *   src/mcx_core.cu:3247-3251, :3381-3387  host seeding -> mcx_oracle_seedwords()
+ *   src/mcx_utils.c:4214-4340  mcx_maskdet          -> mcx_oracle_maskdet()
+ *
+ * Float arithmetic is written operation by operation (no contraction, see
+ * the Makefile) and the elementary functions come from oracle_math.h.
+ */
+#include "mcx_oracle.h"
+#include "oracle_math.h"
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const float EPSF = 1.1920928955078125e-07f; /* FLT_EPSILON */
+const unsigned int DET_MASK = 0x80000000u;
+const unsigned int MED_MASK = 0x7FFFFFFFu;
+const unsigned int OUT_MIN = 0xFFFFFFFFu;
+const unsigned int OUT_MAX = 0x7FFFFFFFu;
+const float TWO_PI = 6.28318530717959f;
+const float MAX_ACCUM = 1000.f;
+const float ROULETTE = 10.f;
+enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
+enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL };
+
+/* ---- glibc rand(): TYPE_3 additive feedback, r[i] = r[i-3] + r[i-31] --- */
+struct GlibcRand {
+    int32_t r[34];
+    uint32_t ring[31];
+    int pos;
+    explicit GlibcRand(uint32_t seed) {
+        if (seed == 0) seed = 1;
+        std::vector<uint32_t> st(400);
+        int32_t w = (int32_t)seed;
+        st[0] = (uint32_t)w;
+        for (int i = 1; i < 31; i++) {
+            /* 16807 * w mod (2^31-1) by Schrage's decomposition */
+            int32_t hi = w / 127773, lo = w % 127773;
+            w = 16807 * lo - 2836 * hi;
+            if (w < 0) w += 2147483647;
+            st[i] = (uint32_t)w;
+        }
+        for (int i = 31; i < 34; i++) st[i] = st[i - 31];
+        for (int i = 34; i < 344; i++) st[i] = st[i - 31] + st[i - 3];
+        for (int i = 0; i < 31; i++) ring[i] = st[313 + i]; /* r[313..343] */
+        pos = 0;
+    }
+    /* next output: r[i] = r[i-31] + r[i-3]; ring holds r[i-31..i-1] */
+    int next() {
+        uint32_t v = ring[pos] + ring[(pos + 28) % 31];
+        ring[pos] = v;
+        pos = (pos + 1) % 31;
+        return (int)(v >> 1);
+    }
+};
+
+/* ---- xorshift128+ (src/mcx_rand_xorshift128p.cu:56-72) ----------------- */
+struct Rng {
+    uint64_t t0, t1;
+    void seed(const unsigned int* s) {
+        t0 = ((uint64_t)s[0] << 32) | s[1];
+        t1 = ((uint64_t)s[2] << 32) | s[3];
+    }
+    float next() {
+        uint64_t s1 = t0;
+        const uint64_t s0 = t1;
+        t0 = s0;
+        s1 ^= s1 << 23;
+        t1 = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+        uint32_t lo = (uint32_t)(t1 + s0);
+        return om_bits2f(0x3F800000u | (lo >> 9)) - 1.0f;
+    }
+    float scatlen() { return -om_logf(next() + EPSF); }
+};
+
+struct Vec4 { float x, y, z, w; };
+
+struct Ctx {
+    const OracleParams* P;
+    OracleOut* O;
+    unsigned int dimlen_x, dimlen_y, dimlen_z;
+    float maxidx_x, maxidx_y, maxidx_z;
+    unsigned int maxmedia, partialdata, w0offset, reclen;
+    std::atomic<unsigned int>* detected;
+    const Vec4* gprop;   /* media then detectors */
+};
+
+inline float hitgrid(const Vec4& p, const Vec4& v, const float* rv, int* id) {
+    float h0 = fabsf(((float)(id[0] + (v.x > 0.f)) - p.x) * rv[0]);
+    float h1 = fabsf(((float)(id[1] + (v.y > 0.f)) - p.y) * rv[1]);
+    float h2 = fabsf(((float)(id[2] + (v.z > 0.f)) - p.z) * rv[2]);
+    float dist = fminf(fminf(h0, h1), h2);
+    id[3] = (dist == h0 ? 0 : (dist == h1 ? 1 : 2));
+    return dist;
+}
+
+inline void rotatevector(Vec4& v, float stheta, float ctheta, float sphi, float cphi) {
+    if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+        float tmp0 = 1.f - v.z * v.z;
+        float tmp1 = stheta / sqrtf(tmp0);
+        float nx = tmp1 * (v.x * v.z * cphi - v.y * sphi) + v.x * ctheta;
+        float ny = tmp1 * (v.y * v.z * cphi + v.x * sphi) + v.y * ctheta;
+        float nz = -tmp1 * tmp0 * cphi + v.z * ctheta;
+        v.x = nx; v.y = ny; v.z = nz;
+    } else {
+        float nz = (v.z > 0.f) ? ctheta : -ctheta;
+        v.x = stheta * cphi;
+        v.y = stheta * sphi;
+        v.z = nz;
+    }
+    float tmp0 = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+}
+
+inline void rotatevector2d(Vec4& v, int is2d, float stheta, float ctheta) {
+    if (is2d == 1) {
+        float ny = v.y * ctheta - v.z * stheta, nz = v.y * stheta + v.z * ctheta;
+        v.x = 0.f; v.y = ny; v.z = nz;
+    } else if (is2d == 2) {
+        float nx = v.x * ctheta - v.z * stheta, nz = v.x * stheta + v.z * ctheta;
+        v.x = nx; v.y = 0.f; v.z = nz;
+    } else if (is2d == 3) {
+        float nx = v.x * ctheta - v.y * stheta, ny = v.x * stheta + v.y * ctheta;
+        v.x = nx; v.y = ny; v.z = 0.f;
+    }
+    float tmp0 = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+}
+
+inline float comp(const Vec4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+inline float sgnf(float x) { return (float)((x > 0.f) - (x < 0.f)); }
+
+inline void transmit(Vec4& v, float n1, float n2, int flipdir) {
+    float tmp0 = n1 / n2;
+    v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+    if (flipdir == 0) {
+        tmp0 = v.y * v.y + v.z * v.z;
+        v.x = (tmp0 < 1.f) ? sqrtf(1.f - tmp0) * sgnf(v.x) : 0.f;
+    } else if (flipdir == 1) {
+        tmp0 = v.x * v.x + v.z * v.z;
+        v.y = (tmp0 < 1.f) ? sqrtf(1.f - tmp0) * sgnf(v.y) : 0.f;
+    } else {
+        tmp0 = v.x * v.x + v.y * v.y;
+        v.z = (tmp0 < 1.f) ? sqrtf(1.f - tmp0) * sgnf(v.z) : 0.f;
+    }
+    tmp0 = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
+}
+
+inline float reflectcoeff(const Vec4& v, float n1, float n2, int flipdir) {
+    float Icos = fabsf(comp(v, flipdir));
+    float tmp0 = n1 * n1;
+    float tmp1 = n2 * n2;
+    float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+    if (tmp2 > 0.f) {
+        float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+        tmp2 = sqrtf(tmp2);
+        float Im = 2.f * n1 * n2 * Icos * tmp2;
+        float Rtotal = (Re - Im) / (Re + Im);
+        Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+        Rtotal = (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+        return Rtotal;
+    }
+    return 1.f;
+}
+
+inline float nextafter_mcx(float a, int dir, int maxvoidstep) {
+    float f = a + (float)maxvoidstep;
+    uint32_t i = om_f2bits(f);
+    i += (uint32_t)dir ^ (i & 0x80000000u);
+    return om_bits2f(i) - (float)maxvoidstep;
+}
+
+inline void atomic_add_double(double* a, double v) {
+#pragma omp atomic
+    *a += v;
+}
+
+inline float atomic_add_float_ret(float* a, float v) {
+    std::atomic_ref<float> r(*a);
+    return r.fetch_add(v);
+}
+
+/* One virtual thread's state (the reference keeps these in registers). */
+struct Thread {
+    Ctx* C;
+    unsigned int idx;
+    int nphoton;          /* threadphoton + (idx < oddphotons) */
+    Vec4 p, v, f;         /* f = {pscat, t, pathlen, ndone} */
+    float rv[3];
+    Vec4 prop;
+    Rng t;
+    int flipdir[4];
+    unsigned int idx1d, mediaid;
+    float w0;
+    float* ppath;
+    unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
+    int err = 0;
+
+    const OracleParams& P() const { return *C->P; }
+
+    unsigned int finddetector() const {
+        const Ctx& c = *C;
+        for (unsigned int i = c.maxmedia + 1; i < c.maxmedia + P().detnum + 1; i++) {
+            const Vec4& d = c.gprop[i];
+            if ((d.x - p.x) * (d.x - p.x) + (d.y - p.y) * (d.y - p.y) + (d.z - p.z) * (d.z - p.z) < d.w * d.w)
+                return (d.w < 0.f) ? 0 : i - c.maxmedia;
+        }
+        return 0;
+    }
+
+    void savedet(unsigned int isdet) {
+        int detid = (isdet == OUT_MIN) ? -1 : (int)finddetector();
+        if (!detid) return;
+        unsigned int base = C->detected->fetch_add(1);
+        if (base < P().maxdetphoton) {
+            if (!C->O->detdata) return;
+            float* rec = C->O->detdata + (size_t)base * C->reclen;
+            unsigned int k = 0, flag = P().savedetflag;
+            if (flag & 1) rec[k++] = (float)detid;
+            for (unsigned int i = 0; i < C->partialdata; i++) rec[k++] = ppath[i];
+            if ((flag >> 4) & 1) { rec[k++] = p.x; rec[k++] = p.y; rec[k++] = p.z; }
+            if ((flag >> 5) & 1) { rec[k++] = v.x; rec[k++] = v.y; rec[k++] = v.z; }
+            if ((flag >> 6) & 1) rec[k++] = ppath[C->w0offset - 2];
+        } else if (P().savedet == 3) {
+            C->detected->fetch_sub(1);
+        }
+    }
+
+    void setprop(unsigned int m) { prop = C->gprop[m & MED_MASK]; }
+
+    /* src/mcx_core.cu:866-956 */
+    int skipvoid() {
+        const OracleParams& Q = P();
+        int count = 1, id[4] = {0, 0, 0, -1};
+        id[0] = (int)floorf(p.x); id[1] = (int)floorf(p.y); id[2] = (int)floorf(p.z);
+        while (1) {
+            if ((unsigned)id[0] < Q.nx && (unsigned)id[1] < Q.ny && (unsigned)id[2] < Q.nz) {
+                unsigned int i1 = id[2] * C->dimlen_y + id[1] * C->dimlen_x + id[0];
+                if (Q.media[i1] & MED_MASK) {
+                    p.x -= v.x; p.y -= v.y; p.z -= v.z;
+                    id[0] = (int)floorf(p.x); id[1] = (int)floorf(p.y); id[2] = (int)floorf(p.z);
+                    f.y -= Q.minaccumtime;
+                    i1 = id[2] * C->dimlen_y + id[1] * C->dimlen_x + id[0];
+                    count = 0;
+                    while (!((unsigned)id[0] < Q.nx && (unsigned)id[1] < Q.ny && (unsigned)id[2] < Q.nz) ||
+                           !(Q.media[i1] & MED_MASK)) {
+                        float dist = hitgrid(p, v, rv, id);
+                        f.y += Q.minaccumtime * dist;
+                        p.x = p.x + dist * v.x; p.y = p.y + dist * v.y; p.z = p.z + dist * v.z;
+                        if (id[3] == 0) id[0] += (v.x > 0.f ? 1 : -1);
+                        if (id[3] == 1) id[1] += (v.y > 0.f ? 1 : -1);
+                        if (id[3] == 2) id[2] += (v.z > 0.f ? 1 : -1);
+                        i1 = id[2] * C->dimlen_y + id[1] * C->dimlen_x + id[0];
+                        if (count++ > 3) break;
+                    }
+                    f.y = Q.voidtime ? f.y : 0.f;
+                    Vec4 h = C->gprop[Q.media[i1] & MED_MASK];
+                    if (Q.isspecular && h.w != C->gprop[0].w) {
+                        p.w *= 1.f - reflectcoeff(v, C->gprop[0].w, h.w, id[3]);
+                        if (p.w > EPSF) transmit(v, C->gprop[0].w, h.w, id[3]);
+                    }
+                    return (int)i1;
+                }
+            }
+            if ((p.x < 0.f && v.x <= 0.f) || (p.x >= C->maxidx_x && v.x >= 0.f) ||
+                (p.y < 0.f && v.y <= 0.f) || (p.y >= C->maxidx_y && v.y >= 0.f) ||
+                (p.z < 0.f && v.z <= 0.f) || (p.z >= C->maxidx_z && v.z >= 0.f))
+                return -1;
+            p.x = p.x + v.x; p.y = p.y + v.y; p.z = p.z + v.z;
+            id[0] = (int)floorf(p.x); id[1] = (int)floorf(p.y); id[2] = (int)floorf(p.z);
+            f.y += Q.minaccumtime;
+            if (count++ > Q.maxvoidstep) return -1;
+        }
+    }
+
+    void locate_launch() {
+        const OracleParams& Q = P();
+        idx1d = (unsigned)((int)floorf(p.z) * (int)C->dimlen_y + (int)floorf(p.y) * (int)C->dimlen_x + (int)floorf(p.x));
+        if (p.x < 0.f || p.y < 0.f || p.z < 0.f || p.x >= C->maxidx_x || p.y >= C->maxidx_y || p.z >= C->maxidx_z)
+            mediaid = 0;
+        else
+            mediaid = Q.media[idx1d];
+    }
+
+    /* src/mcx_core.cu:1089-1717; returns 0 launched, 1 done, -1 failed */
+    int launch(unsigned int isdet) {
+        const OracleParams& Q = P();
+        w0 = 1.f;
+        int canfocus = 1;
+        if (Q.savedet == 3 && C->detected->load() >= Q.maxdetphoton) return 1;
+        if (fabsf(p.w) >= 0.f) {
+            ppath[C->partialdata] += p.w;
+            if (Q.savedet && (isdet & DET_MASK) == DET_MASK && mediaid == 0) savedet(isdet);
+        }
+        if (Q.savedet)
+            for (unsigned int i = 0; i < C->partialdata; i++) ppath[i] = 0.f;
+        if ((int)f.w >= nphoton - 1) return 1;
+        float* pp = ppath + C->partialdata;
+        const bool ispencil = (Q.srctype == 0);
+        do {
+            p = {Q.srcpos[0], Q.srcpos[1], Q.srcpos[2], Q.srcpos[3]};
+            v = {Q.srcdir[0], Q.srcdir[1], Q.srcdir[2], Q.srcdir[3]};
+            f = {0.f, 0.f, Q.minaccumtime, f.w};
+            memcpy(&idx1d, &Q.srcparam2[2], 4);
+            memcpy(&mediaid, &Q.srcparam2[3], 4);
+            float org[3] = {Q.srcpos[0], Q.srcpos[1], Q.srcpos[2]};
+            if (!ispencil) {
+                const float* s1 = Q.srcparam1;
+                const float* s2 = Q.srcparam2;
+                switch (Q.srctype) {
+                    case 4: /* planar */
+                    case 6: /* fourier */
+                    case 14: { /* pencilarray */
+                        float rx = t.next();
+                        float ry = t.next();
+                        p.x = p.x + rx * s1[0] + ry * s2[0];
+                        p.y = p.y + rx * s1[1] + ry * s2[1];
+                        p.z = p.z + rx * s1[2] + ry * s2[2];
+                        if (Q.srctype == 6) {
+                            float s, c;
+                            float ang = (floorf(s1[3]) * rx + floorf(s2[3]) * ry + s1[3] - floorf(s1[3]));
+                            om_sincos2pi(ang, &s, &c);
+                            p.w = Q.srcpos[3] * (c * (1.f - s2[3] + floorf(s2[3])) + 1.f) * 0.5f;
+                        } else if (Q.srctype == 14) {
+                            p.x = Q.srcpos[0] + floorf(rx * s1[3]) * s1[0] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[0] / (s2[3] - 1.f);
+                            p.y = Q.srcpos[1] + floorf(rx * s1[3]) * s1[1] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[1] / (s2[3] - 1.f);
+                            p.z = Q.srcpos[2] + floorf(rx * s1[3]) * s1[2] / (s1[3] - 1.f) + floorf(ry * s2[3]) * s2[2] / (s2[3] - 1.f);
+                        }
+                        locate_launch();
+                        org[0] = org[0] + (s1[0] + s2[0]) * 0.5f;
+                        org[1] = org[1] + (s1[1] + s2[1]) * 0.5f;
+                        org[2] = org[2] + (s1[2] + s2[2]) * 0.5f;
+                        break;
+                    }
+                    case 8:  /* disk */
+                    case 17: /* ring */
+                    case 3: { /* gaussian */
+                        float sphi, cphi, r;
+                        if (Q.srctype == 17 && (s1[2] > 0.f || s1[3] > 0.f)) {
+                            float phi = fabsf(s1[2] - s1[3]) * t.next() + fminf(s1[2], s1[3]);
+                            om_sincos2pi(phi * 0.159154943091895336f, &sphi, &cphi);
+                        } else {
+                            om_sincos2pi(t.next(), &sphi, &cphi);
+                        }
+                        if (Q.srctype == 8 || Q.srctype == 17) {
+                            r = sqrtf(t.next() * fabsf(s1[0] * s1[0] - s1[1] * s1[1]) + s1[1] * s1[1]);
+                        } else if (fabsf(Q.srcdir[3]) < 1e-5f || fabsf(s1[1]) < 1e-5f) {
+                            r = sqrtf(0.5f * t.scatlen()) * s1[0];
+                        } else {
+                            r = s1[0] * s1[0] * 3.14159265358979f / s1[1];
+                            r = sqrtf(0.5f * t.scatlen() * (1.f + (Q.srcdir[3] * Q.srcdir[3] / (r * r)))) * s1[0];
+                        }
+                        if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+                            float tmp0 = 1.f - v.z * v.z;
+                            float tmp1 = r / sqrtf(tmp0);
+                            p.x = p.x + tmp1 * (v.x * v.z * cphi - v.y * sphi);
+                            p.y = p.y + tmp1 * (v.y * v.z * cphi + v.x * sphi);
+                            p.z = p.z - tmp1 * tmp0 * cphi;
+                        } else {
+                            p.x += r * cphi;
+                            p.y += r * sphi;
+                        }
+                        locate_launch();
+                        break;
+                    }
+                    case 1:  /* isotropic */
+                    case 2:  /* cone */
+                    case 7: { /* arcsine */
+                        float sphi, cphi, stheta, ctheta;
+                        om_sincos2pi(t.next(), &sphi, &cphi);
+                        if (Q.srctype == 2) {
+                            if (s1[1] > 0.f) {
+                                float ang = t.next() * s1[0];
+                                om_sincos2pi(ang * 0.159154943091895336f, &stheta, &ctheta);
+                            } else {
+                                float cmax;
+                                float dummy;
+                                om_sincos2pi(s1[0] * 0.159154943091895336f, &dummy, &cmax);
+                                ctheta = t.next() * (1.f - cmax) + cmax;
+                                stheta = om_sin_from_cos(ctheta);
+                            }
+                        } else if (Q.srctype == 1) {
+                            ctheta = 2.f * t.next() - 1.f;
+                            stheta = om_sin_from_cos(ctheta);
+                        } else {
+                            om_sincos2pi(t.next() * 0.5f, &stheta, &ctheta);
+                        }
+                        rotatevector(v, stheta, ctheta, sphi, cphi);
+                        canfocus = 0;
+                        break;
+                    }
+                    default:
+                        break;
+                }
+                if (fabsf(p.w) <= Q.minenergy) continue;
+                if (canfocus) {
+                    if (std::isnan(Q.srcdir[3])) {
+                        float sphi, cphi;
+                        om_sincos2pi(t.next(), &sphi, &cphi);
+                        float ctheta = 2.f * t.next() - 1.f;
+                        rotatevector(v, om_sin_from_cos(ctheta), ctheta, sphi, cphi);
+                    } else if (Q.srcdir[3] < 0.f && std::isinf(Q.srcdir[3])) {
+                        float sphi, cphi;
+                        om_sincos2pi(t.next(), &sphi, &cphi);
+                        float stheta = sqrtf(t.next());
+                        float ctheta = sqrtf(1.f - stheta * stheta);
+                        rotatevector(v, stheta, ctheta, sphi, cphi);
+                    } else if (Q.srcdir[3] != 0.f) {
+                        float Rn2 = (float)((Q.srcdir[3] > 0.f) - (Q.srcdir[3] < 0.f));
+                        org[0] += Q.srcdir[3] * v.x;
+                        org[1] += Q.srcdir[3] * v.y;
+                        org[2] += Q.srcdir[3] * v.z;
+                        v.x = Rn2 * (org[0] - p.x);
+                        v.y = Rn2 * (org[1] - p.y);
+                        v.z = Rn2 * (org[2] - p.z);
+                        Rn2 = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                        v.x *= Rn2; v.y *= Rn2; v.z *= Rn2;
+                    }
+                }
+            }
+            rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+            if ((mediaid & MED_MASK) == 0) {
+                int i1 = skipvoid();
+                if (i1 >= 0) {
+                    idx1d = (unsigned)i1;
+                    mediaid = Q.media[idx1d];
+                }
+            }
+            flipdir[0] = (int)floorf(p.x); flipdir[1] = (int)floorf(p.y); flipdir[2] = (int)floorf(p.z);
+            w0 += 1.f;
+            if (w0 > (float)Q.maxvoidstep) return -1;
+        } while ((mediaid & MED_MASK) == 0 || fabsf(p.w) <= Q.minenergy);
+        f.w += 1.f;
+        setprop(mediaid);
+        pp[1] += p.w;
+        w0 = p.w;
+        pp[2] = p.w;
+        v.w = EPSF;
+        f.z = 0.f;
+        nlaunch++;
+        return 0;
+    }
+
+    void deposit(unsigned int vox, int tshift, float weight) {
+        const OracleParams& Q = P();
+        size_t nvox = (size_t)C->dimlen_z;
+        size_t i = (size_t)vox + (size_t)tshift * nvox;
+        ndep++;
+        if (C->O->field) atomic_add_double(C->O->field + i, (double)weight);
+        if (C->O->field32) {
+            float* fld = C->O->field32;
+            float oldval = atomic_add_float_ret(fld + i, weight);
+            if (fabsf(oldval) > MAX_ACCUM) {
+                atomic_add_float_ret(fld + i, (oldval > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
+                atomic_add_float_ret(fld + i + nvox * Q.maxgate, (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
+            }
+        }
+    }
+
+    /* src/mcx_core.cu:1767-2528 for one virtual thread */
+    void run() {
+        const OracleParams& Q = P();
+        const Ctx& c = *C;
+        const unsigned int flag = Q.savedetflag;
+        const unsigned int NSCAT = (flag >> 1) & 1, PPATH = (flag >> 2) & 1, MOM = (flag >> 3) & 1;
+        p = {0.f, 0.f, 0.f, NAN};
+        v = {0.f, 0.f, 0.f, 0.f};
+        f = {0.f, 0.f, 0.f, -1.f};
+        flipdir[0] = flipdir[1] = flipdir[2] = 0; flipdir[3] = -1;
+        memcpy(&mediaid, &Q.srcparam2[3], 4);
+        idx1d = 0;
+        unsigned int idx1dold, mediaidold = 0, isdet = 0;
+        float n1, len, slen;
+        prop = c.gprop[1];
+        t.seed(Q.seeds + (size_t)idx * 4);
+        if (launch(0)) return;
+        rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+        isdet = mediaid & DET_MASK;
+        mediaid &= MED_MASK;
+        while (f.w < (float)nphoton) {
+            nstep++;
+            if (f.x <= 0.f) {
+                f.x = t.scatlen();
+                if (v.w != EPSF) {
+                    float cphi = 1.f, sphi = 0.f, stheta, ctheta, tmp0;
+                    if (!Q.is2d) om_sincos2pi(t.next(), &sphi, &cphi);
+                    tmp0 = (v.w > (float)Q.gscatter) ? 0.f : prop.z;
+                    if (fabsf(tmp0) > EPSF) {
+                        float g = prop.z;
+                        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * t.next());
+                        tmp0 *= tmp0;
+                        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
+                        ctheta = tmp0;
+                        stheta = om_sin_from_cos(ctheta);
+                    } else {
+                        ctheta = 2.f * t.next() - 1.f;
+                        stheta = om_sin_from_cos(ctheta);
+                    }
+                    if (Q.savedet) {
+                        if (NSCAT) ppath[(mediaid & MED_MASK) - 1] += 1.f;
+                        if (MOM) ppath[c.maxmedia * (NSCAT + PPATH) + (mediaid & MED_MASK) - 1] += 1.f - ctheta;
+                    }
+                    if (Q.is2d)
+                        rotatevector2d(v, Q.is2d, (t.next() > 0.5f ? stheta : -stheta), ctheta);
+                    else
+                        rotatevector(v, stheta, ctheta, sphi, cphi);
+                    v.w += 1.f;
+                    rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+                    nscat++;
+                }
+                v.w = (float)(int)v.w;
+            }
+            n1 = prop.w;
+            setprop(mediaid);
+            len = hitgrid(p, v, rv, flipdir);
+            const bool gs = (v.w + 1.f > (float)Q.gscatter);
+            slen = len * prop.y * (gs ? (1.f - prop.z) : 1.f);
+            slen = fminf(slen, f.x);
+            len = ((prop.y == 0.f) ? len : (slen / (prop.y * (gs ? (1.f - prop.z) : 1.f))));
+            p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
+            if (flipdir[3] == 0) flipdir[0] += (slen == f.x) ? 0 : (v.x > 0.f ? 1 : -1);
+            if (flipdir[3] == 1) flipdir[1] += (slen == f.x) ? 0 : (v.y > 0.f ? 1 : -1);
+            if (flipdir[3] == 2) flipdir[2] += (slen == f.x) ? 0 : (v.z > 0.f ? 1 : -1);
+            p.w *= om_expf(-prop.x * len);
+            f.x -= slen;
+            f.y += len * prop.w * Q.oneoverc0;
+            f.z += len;
+            if (Q.savedet && PPATH) ppath[c.maxmedia * NSCAT + (mediaid & MED_MASK) - 1] += len;
+            mediaidold = mediaid | (isdet & DET_MASK);
+            idx1dold = idx1d;
+            idx1d = (unsigned)(flipdir[2] * (int)c.dimlen_y + flipdir[1] * (int)c.dimlen_x + flipdir[0]);
+            if ((unsigned)flipdir[0] >= Q.nx || (unsigned)flipdir[1] >= Q.ny || (unsigned)flipdir[2] >= Q.nz) {
+                mediaid = 0;
+                idx1d = (flipdir[0] < 0 || flipdir[1] < 0 || flipdir[2] < 0) ? OUT_MIN : OUT_MAX;
+                isdet = Q.bc[(idx1d == OUT_MAX) * 3 + flipdir[3]];
+            } else {
+                mediaid = Q.media[idx1d];
+                isdet = mediaid & DET_MASK;
+                mediaid &= MED_MASK;
+            }
+            if (idx1d != idx1dold && mediaidold) {
+                if (Q.save2pt && f.y >= Q.twin0 && f.y < Q.twin1) {
+                    float weight = 0.f;
+                    int tshift = (int)floorf((f.y - Q.twin0) * Q.Rtstep);
+                    if (tshift > (int)Q.maxgate - 1) tshift = (int)Q.maxgate - 1;
+                    if (Q.outputtype == otEnergy)
+                        weight = w0 - p.w;
+                    else if (Q.outputtype == otFluence || Q.outputtype == otFlux)
+                        weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
+                    else if (Q.outputtype == otL)
+                        weight = w0 * f.z;
+                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
+                }
+                w0 = p.w;
+                f.z = 0.f;
+            } else {
+                mediaid = mediaidold;
+            }
+            const unsigned int bcf = isdet & 0xF;
+            if ((mediaid == 0 &&
+                 ((!Q.isreflect || (Q.isreflect && n1 == c.gprop[0].w)) || (bcf == bcUnknown && !Q.doreflect) ||
+                  bcf == bcAbsorb || bcf == bcCyclic) &&
+                 bcf != bcMirror) ||
+                f.y > Q.twin1) {
+                if (isdet == bcCyclic) {
+                    if (flipdir[3] == 0) {
+                        p.x = nextafter_mcx(roundf(p.x + ((idx1d == OUT_MIN) ? c.maxidx_x : -c.maxidx_x)), (v.x > 0.f) - (v.x < 0.f), Q.maxvoidstep);
+                        flipdir[0] = (int)floorf(p.x);
+                    }
+                    if (flipdir[3] == 1) {
+                        p.y = nextafter_mcx(roundf(p.y + ((idx1d == OUT_MIN) ? c.maxidx_y : -c.maxidx_y)), (v.y > 0.f) - (v.y < 0.f), Q.maxvoidstep);
+                        flipdir[1] = (int)floorf(p.y);
+                    }
+                    if (flipdir[3] == 2) {
+                        p.z = nextafter_mcx(roundf(p.z + ((idx1d == OUT_MIN) ? c.maxidx_z : -c.maxidx_z)), (v.z > 0.f) - (v.z < 0.f), Q.maxvoidstep);
+                        flipdir[2] = (int)floorf(p.z);
+                    }
+                    if ((unsigned)flipdir[0] < Q.nx && (unsigned)flipdir[1] < Q.ny && (unsigned)flipdir[2] < Q.nz) {
+                        idx1d = (unsigned)(flipdir[2] * (int)c.dimlen_y + flipdir[1] * (int)c.dimlen_x + flipdir[0]);
+                        mediaid = Q.media[idx1d];
+                        isdet = mediaid & DET_MASK;
+                        mediaid &= MED_MASK;
+                        continue;
+                    }
+                }
+                unsigned int detarg = (((idx1d == OUT_MAX && Q.bc[9 + flipdir[3]]) || (idx1d == OUT_MIN && Q.bc[6 + flipdir[3]]))
+                                           ? OUT_MIN : (mediaidold & DET_MASK));
+                if (launch(detarg)) break;
+                isdet = mediaid & DET_MASK;
+                mediaid &= MED_MASK;
+                continue;
+            }
+            if (fabsf(p.w) < Q.minenergy) {
+                if (t.next() * ROULETTE <= 1.f) {
+                    p.w *= ROULETTE;
+                } else {
+                    if (launch(mediaidold & DET_MASK)) break;
+                    isdet = mediaid & DET_MASK;
+                    mediaid &= MED_MASK;
+                    continue;
+                }
+            }
+            if (Q.isreflect) {
+                setprop(mediaid);
+                if (((mediaid && Q.doreflect) ||
+                     (mediaid == 0 && ((bcf == bcUnknown && Q.doreflect) || bcf == bcReflect || bcf == bcMirror))) &&
+                    (bcf == bcMirror || n1 != prop.w)) {
+                    float Rtotal = 1.f;
+                    float cphi, sphi, stheta, ctheta, tmp0, tmp1;
+                    setprop(mediaid);
+                    tmp0 = n1 * n1;
+                    tmp1 = prop.w * prop.w;
+                    cphi = fabsf(comp(v, flipdir[3]));
+                    sphi = 1.f - cphi * cphi;
+                    len = 1.f - tmp0 / tmp1 * sphi;
+                    if (len > 0.f && bcf != bcMirror) {
+                        ctheta = tmp0 * cphi * cphi + tmp1 * len;
+                        stheta = 2.f * n1 * prop.w * cphi * sqrtf(len);
+                        Rtotal = (ctheta - stheta) / (ctheta + stheta);
+                        ctheta = tmp1 * cphi * cphi + tmp0 * len;
+                        Rtotal = (Rtotal + (ctheta - stheta) / (ctheta + stheta)) * 0.5f;
+                    }
+                    if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && t.next() > Rtotal) {
+                        transmit(v, n1, prop.w, flipdir[3]);
+                        if (mediaid == 0) {
+                            unsigned int detarg = (((idx1d == OUT_MAX && Q.bc[9 + flipdir[3]]) || (idx1d == OUT_MIN && Q.bc[6 + flipdir[3]]))
+                                                       ? OUT_MIN : (mediaidold & DET_MASK));
+                            if (launch(detarg)) break;
+                            isdet = mediaid & DET_MASK;
+                            mediaid &= MED_MASK;
+                            continue;
+                        }
+                        rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+                    } else {
+                        int fd = flipdir[3];
+                        if (fd == 0) v.x = -v.x; else if (fd == 1) v.y = -v.y; else v.z = -v.z;
+                        rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+                        if (fd == 0) {
+                            p.x = nextafter_mcx(rintf(p.x), (v.x > 0.f) - (v.x < 0.f), Q.maxvoidstep);
+                            flipdir[0] = (int)floorf(p.x);
+                        } else if (fd == 1) {
+                            p.y = nextafter_mcx(rintf(p.y), (v.y > 0.f) - (v.y < 0.f), Q.maxvoidstep);
+                            flipdir[1] = (int)floorf(p.y);
+                        } else {
+                            p.z = nextafter_mcx(rintf(p.z), (v.z > 0.f) - (v.z < 0.f), Q.maxvoidstep);
+                            flipdir[2] = (int)floorf(p.z);
+                        }
+                        idx1d = idx1dold;
+                        mediaid = Q.media[idx1d] & MED_MASK;
+                        setprop(mediaid);
+                        n1 = prop.w;
+                    }
+                } else {
+                    setprop(mediaidold);
+                }
+            }
+            if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
+                err++;
+                return;
+            }
+        }
+        C->O->genergy[idx * 2] = ppath[c.partialdata];
+        C->O->genergy[idx * 2 + 1] = ppath[c.partialdata + 1];
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+float mcx_oracle_expf(float x) { return om_expf(x); }
+float mcx_oracle_logf(float x) { return om_logf(x); }
+void mcx_oracle_sincos2pi(float u, float* s, float* c) { om_sincos2pi(u, s, c); }
+
+void mcx_oracle_glibc_rand(unsigned int seed, unsigned int n, int* out) {
+    GlibcRand g(seed);
+    for (unsigned int i = 0; i < n; i++) out[i] = g.next();
+}
+
+void mcx_oracle_seedwords(unsigned int seed, unsigned int nwords, unsigned int* words) {
+    GlibcRand g(seed);
+    for (unsigned int i = 0; i < nwords; i++) {
+        uint32_t a = (uint32_t)g.next();
+        uint32_t b = (uint32_t)g.next();
+        uint32_t c = (uint32_t)g.next();
+        words[i] = (a << 16) | (b << 1) | (c >> 14);
+    }
+}
+
+void mcx_oracle_rng_floats(const unsigned int* seed4, unsigned int n, float* out) {
+    Rng r;
+    r.seed(seed4);
+    for (unsigned int i = 0; i < n; i++) out[i] = r.next();
+}
+
+void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, unsigned int nz,
+                        const float* detpos, unsigned int detnum) {
+    const float corners[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    unsigned int dx = nx + 2, dy = ny + 2, dz = nz + 2;
+    int isonecube = (nx == 1) && (ny == 1) && (nz == 1);
+    std::vector<unsigned int> pad((size_t)dx * dy * dz, 0);
+    for (unsigned int z = 1; z <= nz; z++)
+        for (unsigned int y = 1; y <= ny; y++)
+            memcpy(&pad[(size_t)z * dy * dx + y * dx + 1], vol + (size_t)(z - 1) * ny * nx + (y - 1) * nx, nx * 4);
+    for (unsigned int d = 0; d < detnum; d++) {
+        const float* dp = detpos + 4 * d;
+        float radius = fabsf(dp[3]);
+        float d2max = (radius + 1.7321f) * (radius + 1.7321f);
+        for (float z = -radius - 1.f; z <= radius + 1.f; z += 0.5f) {
+            float iz = z + dp[2];
+            for (float y = -radius - 1.f; y <= radius + 1.f; y += 0.5f) {
+                float iy = y + dp[1];
+                for (float x = -radius - 1.f; x <= radius + 1.f; x += 0.5f) {
+                    float ix = x + dp[0];
+                    if (iz < 0 || ix < 0 || iy < 0 || ix >= nx || iy >= ny || iz >= nz ||
+                        x * x + y * y + z * z > (radius + 1.f) * (radius + 1.f))
+                        continue;
+                    float mind2 = 1.f / EPSF;
+                    for (int c = 0; c < 8; c++) {
+                        float rx = (int)ix - dp[0] + corners[c][0];
+                        float ry = (int)iy - dp[1] + corners[c][1];
+                        float rz = (int)iz - dp[2] + corners[c][2];
+                        float d2 = rx * rx + ry * ry + rz * rz;
+                        if (d2 > d2max) { mind2 = 1.f / EPSF; break; }
+                        if (d2 < mind2) mind2 = d2;
+                    }
+                    float lim = radius + 0.5f * (1.f + isonecube);
+                    if (mind2 == 1.f / EPSF || mind2 >= lim * lim) continue;
+                    size_t i = (size_t)(int)(iz + 1.f) * dy * dx + (size_t)(int)(iy + 1.f) * dx + (size_t)(int)(ix + 1.f);
+                    if (!pad[i]) continue;
+                    bool interior = true;
+                    for (int oz = -1; oz <= 1 && interior; oz++)
+                        for (int oy = -1; oy <= 1 && interior; oy++)
+                            for (int ox = -1; ox <= 1; ox++) {
+                                int nzero = (ox != 0) + (oy != 0) + (oz != 0);
+                                if (nzero == 0) continue;
+                                size_t j = i + (long)oz * dy * dx + (long)oy * dx + ox;
+                                if (!pad[j]) { interior = false; break; }
+                            }
+                    if (!interior)
+                        vol[(size_t)(int)iz * ny * nx + (size_t)(int)iy * nx + (int)ix] |= DET_MASK;
+                }
+            }
+        }
+    }
+}
+
+int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
+    if (!P || !O || !P->media || !P->prop || !P->seeds || !O->genergy) return -1;
+    if (P->medianum < 2 || P->medianum + P->detnum > 4000) return -2;
+    Ctx c;
+    c.P = P;
+    c.O = O;
+    c.dimlen_x = P->nx;
+    c.dimlen_y = P->nx * P->ny;
+    c.dimlen_z = P->nx * P->ny * P->nz;
+    c.maxidx_x = (float)P->nx; c.maxidx_y = (float)P->ny; c.maxidx_z = (float)P->nz;
+    c.maxmedia = P->medianum - 1;
+    unsigned int f = P->savedetflag;
+    c.partialdata = c.maxmedia * (((f >> 1) & 1) + ((f >> 2) & 1) + ((f >> 3) & 1));
+    c.w0offset = c.partialdata + 4;
+    c.reclen = c.partialdata + (f & 1) + 3 * (((f >> 4) & 1) + ((f >> 5) & 1)) + ((f >> 6) & 1);
+    O->reclen = c.reclen;
+    std::vector<Vec4> gprop(P->medianum + P->detnum);
+    for (unsigned int i = 0; i < P->medianum; i++) gprop[i] = {P->prop[4 * i], P->prop[4 * i + 1], P->prop[4 * i + 2], P->prop[4 * i + 3]};
+    for (unsigned int i = 0; i < P->detnum; i++)
+        gprop[P->medianum + i] = {P->detpos[4 * i], P->detpos[4 * i + 1], P->detpos[4 * i + 2], P->detpos[4 * i + 3]};
+    c.gprop = gprop.data();
+    std::atomic<unsigned int> detected(O->detected);
+    c.detected = &detected;
+    unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
+    int errors = 0;
+    O->errors = 0;
+    const unsigned int pplen = c.w0offset + 1;
+    long tb = P->thread_begin, te = P->thread_end ? P->thread_end : P->nthread;
+    if (te > (long)P->nthread) te = P->nthread;
+    const long total_needed = (long)P->threadphoton * (long)P->nthread + P->oddphotons;
+    auto t0 = std::chrono::steady_clock::now();
+#ifdef _OPENMP
+    int nt = P->ompthreads > 0 ? P->ompthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) reduction(+ : nstep, ndep, nscat, nlaunch, errors)
+#endif
+    for (long i = tb; i < te; i++) {
+        if (i >= total_needed) continue;
+        std::vector<float> pp(pplen, 0.f);
+        Thread T;
+        T.C = &c;
+        T.idx = (unsigned)i;
+        T.nphoton = P->threadphoton + (i < P->oddphotons);
+        T.ppath = pp.data();
+        T.ppath[c.partialdata] = O->genergy[2 * i];
+        T.ppath[c.partialdata + 1] = O->genergy[2 * i + 1];
+        T.run();
+        nstep += T.nstep; ndep += T.ndep; nscat += T.nscat; nlaunch += T.nlaunch;
+        errors += T.err;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    O->seconds = std::chrono::duration<double>(t1 - t0).count();
+    O->detected = detected.load();
+    O->nstep = nstep; O->ndeposit = ndep; O->nscatter = nscat; O->nlaunch = nlaunch;
+    O->errors += errors;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+/*
+ * mcx_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU (OpenMP) restatement of the reference's photon-transport loop,
+ * used by tests/ as the checker and by bench.py's cpu_baseline leg as the
+ * host-core baseline.  Nothing in the product (mcx_amd/) links or calls it.
+ *
+ * Inputs are the *preprocessed* quantities the reference's kernel sees
+ * (src/mcx_core.cu:2853-2876 MCXParam, :3316-3321 gproperty): grid-unit
+ * media properties, 0-based source/detector positions, column-major
+ * uint32 labels with DET_MASK in bit 31, and the per-thread seed words.
+ */
+#ifndef MCX_ORACLE_H
+#define MCX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OracleParams {
+    /* domain */
+    unsigned int nx, ny, nz;
+    const unsigned int* media;   /* nx*ny*nz labels, x fastest, DET_MASK in bit 31 */
+    const float* prop;           /* medianum x {mua,mus,g,n}, grid units */
+    unsigned int medianum;
+    const float* detpos;         /* detnum x {x,y,z,r}, 0-based grid units */
+    unsigned int detnum;
+    /* source */
+    int srctype;
+    float srcpos[4], srcdir[4], srcparam1[4], srcparam2[4];
+    /* time gates */
+    float twin0, twin1, Rtstep, oneoverc0, minaccumtime;
+    unsigned int maxgate;
+    /* physics flags */
+    float minenergy;
+    int isreflect;               /* template isreflect (cfg.isreflect or any r/m bc) */
+    int doreflect;               /* cfg.isreflect */
+    int isspecular, voidtime, maxvoidstep;
+    unsigned int gscatter;
+    int outputtype;              /* otFlux / otFluence / otEnergy / otL */
+    int save2pt;
+    unsigned char bc[12];        /* numeric bc codes and det flags */
+    int is2d;
+    /* detected photons */
+    int savedet;                 /* cfg.issavedet (0,1,3) */
+    unsigned int savedetflag, maxdetphoton;
+    /* launch layout */
+    unsigned int nthread;        /* virtual threads (RNG streams) */
+    int threadphoton, oddphotons;
+    const unsigned int* seeds;   /* 4*nthread seed words */
+    unsigned int thread_begin, thread_end;  /* run streams [begin,end) */
+    int ompthreads;              /* 0 = OpenMP default */
+} OracleParams;
+
+typedef struct OracleOut {
+    double* field;        /* maxgate*nx*ny*nz FP64 tally, or NULL */
+    float*  field32;      /* 2*maxgate*nx*ny*nz FP32 tally + MAX_ACCUM shadow, or NULL */
+    float*  genergy;      /* 2*nthread: escaped, launched (must be pre-zeroed) */
+    float*  detdata;      /* maxdetphoton*reclen records, or NULL */
+    unsigned int detected;
+    unsigned int reclen;
+    int errors;           /* "should never happen" exits */
+    unsigned long long nstep, ndeposit, nscatter, nlaunch;
+    double seconds;       /* wall time of the transport loop */
+} OracleOut;
+
+/* Run the transport loop.  Returns 0 on success, <0 on bad input. */
+int mcx_oracle_run(const OracleParams* p, OracleOut* o);
+
+/* glibc rand() stream (TYPE_3 additive feedback) after srand(seed); seed 0
+ * and 1 give the default state.  Writes n outputs. */
+void mcx_oracle_glibc_rand(unsigned int seed, unsigned int n, int* out);
+
+/* The reference's per-thread seed words (src/mcx_core.cu:3381-3387):
+ * words[i] = (r<<16) | (r'<<1) | (r''>>14), three successive rand() draws. */
+void mcx_oracle_seedwords(unsigned int seed, unsigned int nwords, unsigned int* words);
+
+/* xorshift128+ floats for lane 0 seeded by the first four seed words
+ * (src/mcx_rand_xorshift128p.cu:56-82). */
+void mcx_oracle_rng_floats(const unsigned int* seed4, unsigned int n, float* out);
+
+/* The detector mask (src/mcx_utils.c:4214-4340) on a uint32 column-major
+ * label volume, in place. */
+void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, unsigned int nz,
+                        const float* detpos, unsigned int detnum);
+
+/* Math entry points for tests. */
+float mcx_oracle_expf(float x);
+float mcx_oracle_logf(float x);
+void  mcx_oracle_sincos2pi(float u, float* s, float* c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
