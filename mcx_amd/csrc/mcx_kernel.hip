@@ -252,6 +252,9 @@ struct Lane {
                         id = i2 * kp.dimxy + i1 * kp.nx + i0;
                         if (count++ > 3) break;
                     }
+                    /* the entry search gave up outside the grid: no tissue
+                     * entry (the reference would read an out-of-range label) */
+                    if (!((uint32_t)i0 < kp.nx && (uint32_t)i1 < kp.ny && (uint32_t)i2 < kp.nz)) return -1;
                     f.y = kp.voidtime ? f.y : 0.f;
                     const F4 h = tab[load_media<MT>(kp.media, id) & MED_MASK];
                     if (kp.isspecular && h.w != tab[0].w) {

@@ -284,6 +284,8 @@ struct Thread {
                         i1 = id[2] * C->dimlen_y + id[1] * C->dimlen_x + id[0];
                         if (count++ > 3) break;
                     }
+                    /* entry search gave up outside the grid: no tissue entry */
+                    if (!((unsigned)id[0] < Q.nx && (unsigned)id[1] < Q.ny && (unsigned)id[2] < Q.nz)) return -1;
                     f.y = Q.voidtime ? f.y : 0.f;
                     Vec4 h = C->gprop[Q.media[i1] & MED_MASK];
                     if (Q.isspecular && h.w != C->gprop[0].w) {

@@ -1,0 +1,265 @@
+"""CPU tests of the C-ABI library: it loads and exports every declared
+symbol, its struct layout matches the reference's, and the host-side logic
+(defaults, validation, preprocessing, JSON input, writers, error channel)
+behaves as the reference's does.  No kernel is launched here."""
+import base64
+import ctypes
+import json
+import os
+import re
+import struct
+import subprocess
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+from mcx_amd import _lib
+from mcx_amd._lib import Config
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+L = _lib.lib()
+
+
+def _declared_symbols():
+    names = set()
+    for h in ("include/mcx_abi.h", "include/mcx_amd_ext.h"):
+        text = open(os.path.join(ROOT, h)).read()
+        names |= set(re.findall(r"^\s*(?:void|int|float|double)\s+\**(mcx_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    declared = _declared_symbols()
+    assert len(declared) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (mcx_\w+)", out))
+    assert declared <= exported, declared - exported
+    assert declared == set(_lib.ABI_SYMBOLS)
+
+
+def test_layout_matches_reference():
+    _lib.check_layout(L)
+    assert ctypes.sizeof(Config) == 5296
+    assert ctypes.sizeof(_lib.GPUInfo) == 328
+    assert ctypes.sizeof(_lib.History) == 64
+
+
+def _cfg():
+    c = Config()
+    L.mcx_initcfg(ctypes.byref(c))
+    c.parentid = _lib.mpPython
+    c.flog = None
+    return c
+
+
+def test_initcfg_defaults():
+    """src/mcx_utils.c:248-390"""
+    c = _cfg()
+    assert c.nthread == 16384 and c.nblocksize == 64 and c.seed == 0x623F9A9E
+    assert c.maxdetphoton == 1000000 and c.savedetflag == 5 and c.gscatter == 1000000000
+    assert c.sradius == -2.0 and c.isreflect == 1 and c.mediabyte == 1 and c.maxvoidstep == 1000
+    assert c.outputformat == 5 and c.respin == 1 and c.autopilot == 1 and c.issave2pt == 1
+    assert c.his.magic == b"MCXH" and c.his.version == 1 and c.srcpos.w == 1.0
+    assert np.float32(c.tstep) == np.float32(1.0 / np.finfo(np.float32).eps)
+
+
+def _fill(c, vol, prop, srcpos, srcdir, detpos=None):
+    vol = np.asfortranarray(vol.astype(np.uint32))
+    c.dim.x, c.dim.y, c.dim.z = vol.shape
+    flat = vol.ravel(order="F")
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+
+    def cpy(a, t):
+        a = np.ascontiguousarray(a)
+        p = libc.malloc(a.nbytes)
+        ctypes.memmove(p, a.ctypes.data, a.nbytes)
+        return ctypes.cast(p, ctypes.POINTER(t))
+
+    c.vol = cpy(flat, ctypes.c_uint)
+    pr = np.asarray(prop, dtype=np.float32)
+    c.medianum = pr.shape[0]
+    c.prop = cpy(pr, _lib.Medium)
+    for k, v in zip("xyz", srcpos):
+        setattr(c.srcpos, k, v)
+    for k, v in zip("xyz", srcdir):
+        setattr(c.srcdir, k, v)
+    if detpos is not None:
+        d = np.asarray(detpos, dtype=np.float32)
+        c.detnum = d.shape[0]
+        c.detpos = cpy(d, _lib.Float4)
+    c.tstart, c.tend, c.tstep = 0.0, 5e-9, 5e-9
+
+
+def test_validatecfg_preprocess():
+    """1-based -> 0-based, unit scaling, mus=0 -> EPS & g=1, point-source
+    voxel precompute, DET_MASK (src/mcx_utils.c:3748-3859, :1525-1820)."""
+    c = _cfg()
+    vol = np.ones((20, 20, 20), np.uint8)
+    vol[:, :, 15:] = 2
+    _fill(c, vol, [[0, 0, 1, 1], [0.01, 1.0, 0.9, 1.37], [0.02, 0.0, 0.5, 1.4]], [10, 11, 1], [0, 0, 1],
+          detpos=[[10, 15, 1, 2]])
+    c.unitinmm = 0.5
+    L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+    assert _lib.last_error()[0] == 0
+    assert (c.srcpos.x, c.srcpos.y, c.srcpos.z) == (9, 10, 0)
+    assert (c.srcdir.x, c.srcdir.y, c.srcdir.z) == (0, 0, 1)
+    assert (c.detpos[0].x, c.detpos[0].y, c.detpos[0].z) == (9, 14, 0)
+    assert np.float32(c.prop[1].mua) == np.float32(0.01) * np.float32(0.5)
+    assert np.float32(c.prop[1].mus) == np.float32(1.0) * np.float32(0.5)
+    assert c.prop[2].mus == np.float32(np.finfo(np.float32).eps) and c.prop[2].g == 1.0
+    assert c.prop[0].mus == np.float32(np.finfo(np.float32).eps)
+    idx = struct.unpack("<I", struct.pack("<f", c.srcparam2.z))[0]
+    lab = struct.unpack("<I", struct.pack("<f", c.srcparam2.w))[0]
+    assert idx == 0 * 400 + 10 * 20 + 9 and lab == 1
+    v = np.ctypeslib.as_array(c.vol, shape=(8000,))
+    assert ((v & 0x80000000) != 0).sum() > 0
+    assert c.his.maxmedia == 2 and c.his.detnum == 1 and c.his.colcount == 1 + 2
+    L.mcx_clearcfg(ctypes.byref(c))
+
+
+def test_embedded_error_is_reported_not_fatal():
+    c = _cfg()
+    _fill(c, np.ones((4, 4, 4), np.uint8), [[0, 0, 1, 1], [0.01, 1, 0.9, 1]], [1, 1, 1], [0, 0, 1])
+    c.tend = 0.0
+    L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+    code, msg = _lib.last_error()
+    assert code == -6 and "time gate" in msg
+    L.mcx_clear_error()
+    assert _lib.last_error()[0] == 0
+    L.mcx_clearcfg(ctypes.byref(c))
+
+
+def test_label_beyond_prop_rejected():
+    c = _cfg()
+    _fill(c, np.full((4, 4, 4), 3, np.uint8), [[0, 0, 1, 1], [0.01, 1, 0.9, 1]], [1, 1, 1], [0, 0, 1])
+    L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+    code, msg = _lib.last_error()
+    assert code == -4 and "less than the labels" in msg
+    L.mcx_clear_error()
+    L.mcx_clearcfg(ctypes.byref(c))
+
+
+def test_standalone_error_exits_with_reference_message():
+    code = r"""
+import ctypes, sys
+sys.path.insert(0, %r)
+from mcx_amd import _lib
+L = _lib.lib(); c = _lib.Config(); L.mcx_initcfg(ctypes.byref(c))
+c.dim.x = c.dim.y = c.dim.z = 0; c.medianum = 0
+L.mcx_preprocess(ctypes.byref(c))
+print("not reached")
+""" % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
+    assert r.returncode == (-4) % 256
+    assert "MCX ERROR(-4)" in r.stdout and "not reached" not in r.stdout
+
+
+CUBE60B = {
+    "Session": {"ID": "cube60b", "Photons": 1e6, "RNGSeed": 1648335518, "DoMismatch": True},
+    "Domain": {"Dim": [60, 60, 60], "OriginType": 1,
+               "Media": [{"mua": 0.0, "mus": 0.0, "g": 1.0, "n": 1.0}, {"mua": 0.005, "mus": 1.0, "g": 0.01, "n": 1.37},
+                         {"mua": 0.002, "mus": 5.0, "g": 0.9, "n": 1.0}]},
+    "Shapes": [{"Name": "cube60b"}, {"Origin": [0, 0, 0]}, {"Grid": {"Tag": 1, "Size": [60, 60, 60]}},
+               {"Sphere": {"Tag": 2, "O": [30, 30, 30], "R": 10}}],
+    "Forward": {"T0": 0.0, "T1": 5.0e-09, "Dt": 5.0e-09},
+    "Optode": {"Source": {"Type": "pencil", "Pos": [29.0, 29.0, 0.0], "Dir": [0.0, 0.0, 1.0]},
+               "Detector": [{"Pos": [29.0, 19.0, 0.0], "R": 1.0}, {"Pos": [29.0, 39.0, 0.0], "R": 1.0}]},
+}
+
+
+def test_readconfig_json_and_shapes():
+    """Input surface of the reference's benchmark JSON (src/mcx_bench.c:105-150)."""
+    c = _cfg()
+    L.mcx_readconfig(json.dumps(CUBE60B).encode(), ctypes.byref(c))
+    assert _lib.last_error()[0] == 0, _lib.last_error()
+    assert (c.dim.x, c.dim.y, c.dim.z) == (60, 60, 60) and c.medianum == 3 and c.detnum == 2
+    assert c.nphoton == 1000000 and c.seed == 1648335518 and c.isreflect == 1
+    assert (c.srcpos.x, c.srcpos.y, c.srcpos.z) == (29, 29, 0) and c.issrcfrom0 == 1
+    assert (c.detpos[1].x, c.detpos[1].y, c.detpos[1].w) == (29, 39, 1)
+    assert c.session == b"cube60b" and c.maxgate == 1
+    v = np.ctypeslib.as_array(c.vol, shape=(216000,)).reshape((60, 60, 60), order="F") & 0x7FFFFFFF
+    c_ = np.arange(60) + 0.5 - 30
+    inside = (c_[:, None, None] ** 2 + c_[None, :, None] ** 2 + c_[None, None, :] ** 2) <= 100
+    assert np.array_equal(v == 2, inside) and np.all(v[~inside] == 1)
+    L.mcx_clearcfg(ctypes.byref(c))
+
+
+@pytest.mark.parametrize("shape,check", [
+    ({"ZLayers": [[1, 10, 1], [11, 20, 2]]}, lambda v: np.all(v[:, :, :10] == 1) and np.all(v[:, :, 10:20] == 2)),
+    ({"Box": {"Tag": 2, "O": [2, 2, 2], "Size": [4, 4, 4]}}, lambda v: (v == 2).sum() == 64),
+    ({"Subgrid": {"Tag": 2, "O": [3, 3, 3], "Size": [2, 2, 2]}}, lambda v: (v == 2).sum() == 27),
+    ({"Cylinder": {"Tag": 2, "C0": [10, 10, 0], "C1": [10, 10, 20], "R": 3}},
+     lambda v: np.all(v[9:11, 9:11, :] == 2) and v[0, 0, 0] == 0),
+    ({"XSlabs": {"Tag": 2, "Bound": [5, 8]}}, lambda v: np.all(v[4:8] == 2) and np.all(v[:4] == 0)),
+    ({"UpperSpace": {"Tag": 2, "Coef": [0, 0, 1, 10]}}, lambda v: np.all(v[:, :, 10:] == 2) and np.all(v[:, :, :10] == 0)),
+])
+def test_shape_rasterizers(shape, check):
+    """src/mcx_shapes.c rules: 1-based inclusive layers, voxel-centre tests."""
+    doc = {"Session": {"Photons": 10}, "Domain": {"Dim": [20, 20, 20], "OriginType": 1,
+                                                   "Media": [[0, 0, 1, 1], [0.01, 1, 0.9, 1.37], [0.01, 1, 0.9, 1.37]]},
+           "Forward": {"T0": 0, "T1": 1e-9, "Dt": 1e-9},
+           "Optode": {"Source": {"Pos": [10, 10, 0], "Dir": [0, 0, 1]}},
+           "Shapes": [{"Grid": {"Tag": 0, "Size": [20, 20, 20]}}, shape]}
+    c = _cfg()
+    L.mcx_readconfig(json.dumps(doc).encode(), ctypes.byref(c))
+    assert _lib.last_error()[0] == 0, _lib.last_error()
+    v = np.ctypeslib.as_array(c.vol, shape=(8000,)).reshape((20, 20, 20), order="F") & 0x7FFFFFFF
+    assert check(v), np.unique(v, return_counts=True)
+    L.mcx_clearcfg(ctypes.byref(c))
+
+
+def test_jnii_writer_roundtrip(tmp_path):
+    """.jnii: NIFTIData with zlib+base64 column-major float32 (src/mcx_utils.c:791-905, :4538-4614)."""
+    c = _cfg()
+    c.dim.x, c.dim.y, c.dim.z = 3, 4, 5
+    c.maxgate, c.srcnum, c.tstep = 2, 1, 1e-9
+    c.session = str(tmp_path / "out").encode()
+    c.outputformat = 5
+    data = np.arange(3 * 4 * 5 * 2, dtype=np.float32) * 0.5
+    L.mcx_savedata(data.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), data.size, ctypes.byref(c))
+    assert _lib.last_error()[0] == 0
+    doc = json.load(open(tmp_path / "out.jnii"))
+    nd = doc["NIFTIData"]
+    assert nd["_ArraySize_"] == [3, 4, 5, 2, 1] and nd["_ArrayOrder_"] == "c" and nd["_ArrayZipType_"] == "zlib"
+    raw = np.frombuffer(zlib.decompress(base64.b64decode(nd["_ArrayZipData_"])), dtype="<f4")
+    assert np.array_equal(raw, data)
+    assert doc["NIFTIHeader"]["Dim"] == [3, 4, 5, 2, 1]
+
+
+def test_mch_writer_header_matches_reference_format(tmp_path):
+    """.mch = 64-byte History header + float32 rows (src/mcx_utils.c:982-1017;
+    header layout as in pmcx/test/test_all.py:383-428)."""
+    c = _cfg()
+    c.session = str(tmp_path / "d").encode()
+    c.outputformat = 0
+    c.his.maxmedia, c.his.detnum, c.his.colcount = 2, 2, 6
+    c.his.totalphoton, c.his.detected, c.his.savedphoton = 1000, 100, 4
+    c.his.unitinmm, c.his.normalizer = 0.1, 1e6
+    rows = np.array([[1, 5, 3, 1.0, 2.0, 1.0], [1, 4, 2, 1.5, 1.8, 0.9], [2, 6, 4, 0.8, 2.5, 0.8],
+                     [2, 3, 5, 1.2, 2.2, 0.7]], dtype=np.float32)
+    L.mcx_savedetphoton(rows.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), None, 4, 0, ctypes.byref(c))
+    b = open(tmp_path / "d.mch", "rb").read()
+    fixture = (b"MCXH" + struct.pack("<7I", 1, 2, 2, 6, 1000, 100, 4) + struct.pack("<f", 0.1) +
+               struct.pack("<I", 0) + struct.pack("<f", 1e6) + struct.pack("<i", 1) + struct.pack("<I", 1) +
+               struct.pack("<I", 5) + struct.pack("<I", 1) + struct.pack("<I", 0))
+    assert len(b) == 64 + rows.nbytes
+    assert b[:64] == fixture
+    assert np.array_equal(np.frombuffer(b[64:], dtype="<f4").reshape(4, 6), rows)
+
+
+def test_run_without_gpu_raises_cleanly():
+    import mcx_amd
+    from mcx_amd import workloads
+    if mcx_amd.gpuinfo():
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError):
+        mcx_amd.run(dict(workloads.cube60(100), quiet=True))
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
