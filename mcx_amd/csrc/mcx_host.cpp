@@ -30,7 +30,8 @@
 #include "mcx_internal.h"
 #include "mcx_kernel.h"
 
-extern "C" hipError_t mcxk_launch_transport(const KParam* kp, int ispencil, int isreflect, int tally, int media32,
+/* dkp: the launch parameters in device memory */
+extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int isreflect, int tally, int media32,
                                             unsigned int grid, unsigned int block, size_t shmem, hipStream_t stream);
 extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads);
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream);
@@ -233,7 +234,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
 
     Stream st;
-    DevMem dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
+    DevMem dkp, dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
     std::vector<unsigned char> m8;
     if (media32) {
         dmedia.alloc(nvox * 4);
@@ -266,6 +267,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
     dcount.alloc(4 * sizeof(uint32_t));
     dcounters.alloc(4 * sizeof(uint64_t));
+    dkp.alloc(sizeof(KParam));
     HIP_CHECK(hipMemsetAsync(dcounters.p, 0, dcounters.bytes, st.s));
 
     KParam kp;
@@ -360,11 +362,12 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
                 seeds[i] = (a << 16) | (b << 1) | (c >> 14);
             }
             HIP_CHECK(hipMemcpyAsync(dseeds.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st.s));
+            HIP_CHECK(hipMemcpyAsync(dkp.p, &kp, sizeof(kp), hipMemcpyHostToDevice, st.s));
             HIP_CHECK(hipStreamSynchronize(st.s));
 
             auto tic = std::chrono::steady_clock::now();
             HIP_CHECK(hipEventRecord(ev.a, st.s));
-            HIP_CHECK(mcxk_launch_transport(&kp, ispencil, isref > 0, tally, media32, grid, block, shmem, st.s));
+            HIP_CHECK(mcxk_launch_transport(dkp.as<KParam>(), ispencil, isref > 0, tally, media32, grid, block, shmem, st.s));
             HIP_CHECK(hipEventRecord(ev.b, st.s));
             HIP_CHECK(hipMemcpyAsync(counts, dcount.p, sizeof(counts), hipMemcpyDeviceToHost, st.s));
             HIP_CHECK(hipStreamSynchronize(st.s));

@@ -152,7 +152,13 @@ struct Lane {
         v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
     }
 
-    __device__ __forceinline__ float vcomp(int i) const { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+    /* component `i` of v, selected with bit masks: a select between
+     * struct members would become a dynamically indexed load and keep the
+     * whole lane state in scratch memory */
+    __device__ __forceinline__ float vcomp(int i) const {
+        const uint32_t mx = 0u - (uint32_t)(i == 0), my = 0u - (uint32_t)(i == 1), mz = ~(mx | my);
+        return mcx_bits2f((mcx_f2bits(v.x) & mx) | (mcx_f2bits(v.y) & my) | (mcx_f2bits(v.z) & mz));
+    }
 
     /* src/mcx_core.cu:523-537 */
     __device__ __forceinline__ void transmit(float n1, float n2, int face) {
@@ -229,7 +235,7 @@ struct Lane {
     }
 
     /* src/mcx_core.cu:866-956 */
-    __device__ int skipvoid() {
+    __device__ __forceinline__ int skipvoid() {
         int count = 1, i0, i1, i2, face = -1;
         i0 = (int)floorf(p.x); i1 = (int)floorf(p.y); i2 = (int)floorf(p.z);
         while (1) {
@@ -284,7 +290,7 @@ struct Lane {
     }
 
     /* source sampling for non-pencil sources (src/mcx_core.cu:1261-1641) */
-    __device__ void launch_source(bool& skip) {
+    __device__ __forceinline__ void launch_source(bool& skip) {
         const float* s1 = kp.srcparam1;
         const float* s2 = kp.srcparam2;
         int canfocus = 1;
@@ -401,7 +407,7 @@ struct Lane {
 
     /* src/mcx_core.cu:1089-1717: terminate the current photon, launch the
      * next one.  Returns 0 launched, 1 lane finished, -1 launch failed. */
-    __device__ int launch(uint32_t isdet) {
+    __device__ __forceinline__ int launch(uint32_t isdet) {
         w0 = 1.f;
         if (kp.savedet == 3 && __hip_atomic_load(kp.detcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= kp.maxdetphoton)
             return 1;
@@ -469,7 +475,7 @@ struct Lane {
     }
 
     /* src/mcx_core.cu:1767-2528, one lane */
-    __device__ void run() {
+    __device__ __forceinline__ void run() {
         const uint32_t flag = kp.savedetflag;
         const uint32_t NSCAT = (flag >> 1) & 1, PPATH = (flag >> 2) & 1, MOM = (flag >> 3) & 1;
         p = {0.f, 0.f, 0.f, __builtin_nanf("")};
@@ -488,11 +494,27 @@ struct Lane {
         prop = tab[1];
         t0 = ((uint64_t)kp.seeds[4 * idx] << 32) | kp.seeds[4 * idx + 1];
         t1 = ((uint64_t)kp.seeds[4 * idx + 2] << 32) | kp.seeds[4 * idx + 3];
-        if (launch(0)) return;
-        calc_rv();
-        isdet = mediaid & DET_MASK;
-        mediaid &= MED_MASK;
-        while (f.w < (float)nphoton) {
+        /* Every termination sets `relaunch` and restarts the loop, so the
+         * (large) launch code exists once; the operations and their order
+         * are those of the reference, where launchnewphoton is called in
+         * place at each termination site. */
+        bool first = true, relaunch = true;
+        uint32_t detarg = 0;
+        while (true) {
+            if (relaunch) {
+                if (launch(detarg)) {
+                    if (first) return;
+                    break;
+                }
+                if (first) {
+                    calc_rv();
+                    first = false;
+                }
+                isdet = mediaid & DET_MASK;
+                mediaid &= MED_MASK;
+                relaunch = false;
+            }
+            if (!(f.w < (float)nphoton)) break;
             nstep++;
             if (f.x <= 0.f) {
                 f.x = scatlen();
@@ -600,20 +622,17 @@ struct Lane {
                         continue;
                     }
                 }
-                const uint32_t detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
-                                             ? OUT_MIN : (mediaidold & DET_MASK));
-                if (launch(detarg)) break;
-                isdet = mediaid & DET_MASK;
-                mediaid &= MED_MASK;
+                detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                              ? OUT_MIN : (mediaidold & DET_MASK));
+                relaunch = true;
                 continue;
             }
             if (fabsf(p.w) < kp.minenergy) {
                 if (rnd() * ROULETTE <= 1.f) {
                     p.w *= ROULETTE;
                 } else {
-                    if (launch(mediaidold & DET_MASK)) break;
-                    isdet = mediaid & DET_MASK;
-                    mediaid &= MED_MASK;
+                    detarg = mediaidold & DET_MASK;
+                    relaunch = true;
                     continue;
                 }
             }
@@ -639,11 +658,9 @@ struct Lane {
                     if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && rnd() > Rtotal) {
                         transmit(n1, prop.w, fd3);
                         if (mediaid == 0) {
-                            const uint32_t detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
-                                                         ? OUT_MIN : (mediaidold & DET_MASK));
-                            if (launch(detarg)) break;
-                            isdet = mediaid & DET_MASK;
-                            mediaid &= MED_MASK;
+                            detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                                          ? OUT_MIN : (mediaidold & DET_MASK));
+                            relaunch = true;
                             continue;
                         }
                         calc_rv();
@@ -689,8 +706,12 @@ struct Lane {
 
 using namespace mcxk;
 
+/* The parameter block is read through a device pointer: every field is a
+ * wave-uniform load served by the scalar cache.  (Binding a by-value kernel
+ * argument to a reference would copy the whole block to scratch.) */
 template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
-__global__ void __launch_bounds__(256) mcx_transport(KParam kp) {
+__global__ void __launch_bounds__(256) mcx_transport(const KParam* __restrict__ kpp) {
+    const KParam& kp = *kpp;
     extern __shared__ F4 lds[];
     for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
         const float* g = kp.gprop + 4 * i;
@@ -752,7 +773,7 @@ __global__ void mcx_merge_f64(float* __restrict__ out, const double* __restrict_
 
 /* ---- host-side dispatch (called from mcx_host.cpp) ------------------- */
 
-typedef void (*mcx_kernel_fn)(KParam);
+typedef void (*mcx_kernel_fn)(const KParam*);
 
 template <int P, int R, int T>
 static mcx_kernel_fn pick_mt(int media32) {
@@ -768,14 +789,14 @@ static mcx_kernel_fn pick_tally(int tally, int media32) {
     }
 }
 
-extern "C" hipError_t mcxk_launch_transport(const KParam* kp, int ispencil, int isreflect, int tally, int media32,
+extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int isreflect, int tally, int media32,
                                             unsigned int grid, unsigned int block, size_t shmem, hipStream_t stream) {
     mcx_kernel_fn fn;
     if (ispencil)
         fn = isreflect ? pick_tally<1, 1>(tally, media32) : pick_tally<1, 0>(tally, media32);
     else
         fn = isreflect ? pick_tally<0, 1>(tally, media32) : pick_tally<0, 0>(tally, media32);
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), shmem, stream, *kp);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), shmem, stream, dkp);
     return hipGetLastError();
 }
 
