@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nphoton", type=float, default=1e8)
     ap.add_argument("--workload", default="sphere100", choices=["sphere100", "cube60"])
-    ap.add_argument("--tally", type=int, default=0, help="0 FP32+shadow, 1 FP32 hi/lo, 2 FP64")
+    ap.add_argument("--tally", type=int, default=2, help="0 FP32+shadow, 1 FP32 hi/lo, 2 FP64, 3 FP64 per-XCD copies")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -104,7 +104,11 @@ def main():
     steps_pp = stats_w["nstep"] / nph
     dep_pp = stats_w["ndeposit"] / nph
     reclen = 3 if args.workload == "sphere100" else 0
-    bytes_pp = steps_pp * 1.0 + dep_pp * (16.0 if args.tally == 1 else 8.0) + reclen * 4.0 * detected_w / nph
+    # algorithmic bytes: 1 B label load per step; per deposit the tally
+    # word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8, FP64: 8+8);
+    # detected-photon records
+    dep_bytes = {0: 8.0, 1: 16.0, 2: 16.0, 3: 16.0}[args.tally]
+    bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
     line = {
@@ -125,7 +129,7 @@ def main():
                 args.workload,
                 "100^3 sphere R=15, 50 gates x 0.1 ns, 4 detectors DP, seed 12345" if args.workload == "sphere100"
                 else "60^3 homogeneous, 1 gate 0-5 ns, seed 1648335518",
-                nph, ["FP32+shadow", "FP32 hi/lo", "FP64"][args.tally]),
+                nph, ["FP32+shadow", "FP32 hi/lo", "FP64", "FP64 per-XCD"][args.tally]),
             "kernel_ms": round(k_ms, 3),
             "launch_sync_ms": round(sum(wall_ms) / len(wall_ms), 3),
             "lanes": int(stats_w["nthread"]),

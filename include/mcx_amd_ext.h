@@ -15,10 +15,14 @@ extern "C" {
 #endif
 
 /* Fluence tally precision used by subsequent mcx_run_simulation calls:
- *  0  FP32 atomics with the reference's MAX_ACCUM shadow half (default,
- *     reference semantics, src/mcx_core.cu:2246-2253)
+ *  0  FP32 atomics with the reference's MAX_ACCUM shadow half (reference
+ *     semantics, src/mcx_core.cu:2246-2253)
  *  1  FP32 hi + FP32 lo compensated (TwoSum) tally, merged in FP64
- *  2  FP64 atomics (accuracy reference)
+ *  2  FP64 no-return atomics (default: same 8 bytes per voxel-gate as the
+ *     two FP32 halves, the most accurate, and the fastest because no
+ *     deposit waits for an atomic's return value)
+ *  3  FP64, one private copy per XCD updated with workgroup-scope atomics
+ *     (completed in that XCD's L2), the copies summed in FP64 at merge
  * Returns the previous mode, or -1 for an invalid mode. */
 int  mcx_amd_set_tally_mode(int mode);
 
@@ -35,6 +39,15 @@ void mcx_amd_set_counters(int enable);
  *  out[10] lanes that hit the "should never happen" exit
  * Returns the number of values written (<= n). */
 int  mcx_amd_last_stats(double* out, int n);
+
+/* Scattered-atomic rate of the device (the roof of the fluence tally):
+ * nblocks x 256 lanes each add 1 at `nper` uniformly random indices of an
+ * `nelem`-element array.  mode 0: FP64 agent scope, no return; 1: FP32 no
+ * return; 2: FP32 returning; 3: FP64 workgroup scope into one private copy
+ * per XCD.  out[0] atomics/s, out[1] ms, out[2] array sum / atomics issued
+ * (1 = no lost update), out[3..10] workgroups per XCC id.  Returns 0 on
+ * success. */
+int  mcx_amd_atomic_bench(int mode, size_t nelem, unsigned int nper, unsigned int nblocks, double* out);
 
 /* sizeof/offsetof values of the ABI structs as compiled into the library:
  * sizeof(Config), sizeof(GPUInfo), sizeof(History), then the offsets of

@@ -114,7 +114,7 @@ ABI_SYMBOLS = [
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
-    "mcx_amd_abi_layout",
+    "mcx_amd_abi_layout", "mcx_amd_atomic_bench",
 ]
 
 _lib = None
@@ -148,6 +148,8 @@ def _setup(L):
     L.mcx_amd_last_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
     L.mcx_amd_last_stats.restype = ctypes.c_int
     L.mcx_amd_abi_layout.argtypes = [P(ctypes.c_size_t), ctypes.c_int]
+    L.mcx_amd_atomic_bench.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P(ctypes.c_double)]
+    L.mcx_amd_atomic_bench.restype = ctypes.c_int
     L.mcx_amd_abi_layout.restype = ctypes.c_int
 
 
@@ -181,6 +183,15 @@ def last_error():
     buf = ctypes.create_string_buffer(1024)
     code = lib().mcx_last_error(buf, 1024)
     return code, buf.value.decode(errors="replace")
+
+
+def atomic_bench(mode, nelem, nper, nblocks):
+    """Scattered-atomic rate microbenchmark (see include/mcx_amd_ext.h)."""
+    out = (ctypes.c_double * 11)()
+    rc = lib().mcx_amd_atomic_bench(mode, nelem, nper, nblocks, out)
+    if rc != 0:
+        raise RuntimeError("atomic bench failed (%d)" % rc)
+    return {"atomics_per_s": out[0], "ms": out[1], "sum_ratio": out[2], "xcc_workgroups": list(out[3:11])}
 
 
 def last_stats():

@@ -41,7 +41,7 @@ namespace mcx {
 
 static std::mutex g_statmu;
 static double g_stats[11];
-static int g_tally_mode = MCX_TALLY_F32_SHADOW;
+static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
 
 #define HIP_CHECK(x)                                                                                      \
@@ -256,6 +256,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const bool savedet = cfg->issavedet != 0;
     if (tally == MCX_TALLY_F64) {
         dfield64.alloc(fieldlen * sizeof(double));
+    } else if (tally == MCX_TALLY_F64_XCD) {
+        dfield64.alloc(fieldlen * MCX_NUM_XCD * sizeof(double));
     } else {
         dfield.alloc(fieldlen * sizeof(float));
         dfield2.alloc(fieldlen * sizeof(float));
@@ -316,6 +318,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.field = dfield.as<float>();
     kp.field2 = dfield2.as<float>();
     kp.field64 = dfield64.as<double>();
+    kp.fieldlen = fieldlen;
     kp.genergy = dgenergy.as<float>();
     kp.seeds = dseeds.as<uint32_t>();
     kp.detdata = ddet.as<float>();
@@ -348,7 +351,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         kp.twin0 = cfg->tstart + cfg->tstep * timegate;
         kp.twin1 = kp.twin0 + cfg->tstep * maxgate;
         for (int iter = 0; iter < abs(cfg->respin); iter++) {
-            if (tally == MCX_TALLY_F64) {
+            if (tally == MCX_TALLY_F64 || tally == MCX_TALLY_F64_XCD) {
                 HIP_CHECK(hipMemsetAsync(dfield64.p, 0, dfield64.bytes, st.s));
             } else {
                 HIP_CHECK(hipMemsetAsync(dfield.p, 0, dfield.bytes, st.s));
@@ -400,7 +403,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
                 }
             }
             if (cfg->issave2pt) {
-                const void* a = (tally == MCX_TALLY_F64) ? dfield64.p : dfield.p;
+                const void* a = (tally == MCX_TALLY_F64 || tally == MCX_TALLY_F64_XCD) ? dfield64.p : dfield.p;
                 HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, fieldlen, st.s));
                 HIP_CHECK(hipMemcpyAsync(field.data(), dmerged.p, fieldlen * sizeof(float), hipMemcpyDeviceToHost, st.s));
                 HIP_CHECK(hipStreamSynchronize(st.s));
@@ -600,7 +603,7 @@ int mcx_rng_test(unsigned int seed, float* out, unsigned int n) {
 }
 
 int mcx_amd_set_tally_mode(int mode) {
-    if (mode < 0 || mode > 2) return -1;
+    if (mode < 0 || mode > 3) return -1;
     int old = g_tally_mode;
     g_tally_mode = mode;
     return old;

@@ -16,8 +16,11 @@
 enum {
     MCX_TALLY_F32_SHADOW = 0, /* FP32 atomics + MAX_ACCUM shadow half (reference semantics) */
     MCX_TALLY_F32_HILO = 1,   /* FP32 hi + FP32 lo (TwoSum error), merged in FP64 */
-    MCX_TALLY_F64 = 2         /* FP64 atomics (accuracy reference) */
+    MCX_TALLY_F64 = 2,        /* FP64 no-return atomics */
+    MCX_TALLY_F64_XCD = 3     /* FP64, one private copy per XCD, workgroup-scope atomics */
 };
+
+#define MCX_NUM_XCD 8
 
 struct KParam {
     /* grid */
@@ -44,12 +47,22 @@ struct KParam {
     const float* gprop;           /* ntable x 4 floats */
     float* field;                 /* tally (FP32 modes) */
     float* field2;                /* shadow (mode 0) or lo (mode 1) */
-    double* field64;              /* mode 2 */
+    double* field64;              /* modes 2, 3 (mode 3: MCX_NUM_XCD copies of fieldlen) */
+    uint64_t fieldlen;            /* voxels x gates of one tally copy */
     float* genergy;               /* 2*nthread */
     const uint32_t* seeds;        /* 4*nthread */
     float* detdata;               /* maxdetphoton*reclen */
     uint32_t* detcount;           /* [0]=detected, [1]=error exits, [2]=lanes done */
     uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
+};
+
+/* One instance of a batched launch (device side; parameters in KParam). */
+struct BatchInst {
+    float* out;        /* merged, normalised FP32 fluence (fieldlen) */
+    double* energy;    /* [0] escaped, [1] launched: FP64 sums over lanes */
+    double scale_num;  /* normalisation numerator: unitinmm (flux/fluence) or 1 (energy) */
+    double scale_den;  /* multiplies energytot in the denominator: Vvox*tstep (flux), Vvox (fluence), 1 (energy) */
+    int normalize;     /* 0: raw tally */
 };
 
 #endif

@@ -56,6 +56,12 @@ __device__ __forceinline__ uint32_t load_media<uint32_t>(const void* m, uint32_t
     return ((const uint32_t*)m)[i];
 }
 
+/* XCD executing this wave: s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4),
+ * masked so the copy index is always in range */
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (MCX_NUM_XCD - 1);
+}
+
 __device__ __forceinline__ float sgnf(float x) { return (float)((x > 0.f) - (x < 0.f)); }
 
 __device__ __forceinline__ float nextafter_mcx(float a, int dir, uint32_t maxvoidstep) {
@@ -82,6 +88,11 @@ struct Lane {
     uint32_t idx1d, mediaid;
     float w0;
     float esc, launched, winit;
+    /* FP32 tallies: the returning atomic of the previous deposit, consumed
+     * at the next deposit (or lane exit) so its latency overlaps a step */
+    size_t pend_i;
+    float pend_old, pend_w;
+    double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t nstep, ndep, nscat, nlaunch;
 
     __device__ __forceinline__ Lane(const KParam& k, const F4* tb, float* ppb, uint32_t s, uint32_t id)
@@ -452,25 +463,44 @@ struct Lane {
         return 0;
     }
 
+    /* Second half of an FP32 deposit.  Shadow mode (src/mcx_core.cu:
+     * 2246-2253): once the returned old value exceeds MAX_ACCUM, move
+     * MAX_ACCUM from the value half to the shadow half.  Hi/lo mode: the
+     * atomic stored s = fl(old + w); TwoSum gives the exact rounding error,
+     * which goes to the lo half.  Both moves preserve the merged sum, so
+     * doing them one deposit later changes only when, not what. */
+    __device__ __forceinline__ void flush_pending() {
+        if (TALLY == MCX_TALLY_F64 || TALLY == MCX_TALLY_F64_XCD || pend_i == ~(size_t)0) return;
+        const size_t i = pend_i;
+        const float a = pend_old, w = pend_w;
+        pend_i = ~(size_t)0;
+        if (TALLY == MCX_TALLY_F32_SHADOW) {
+            if (fabsf(a) > MAX_ACCUM) {
+                unsafeAtomicAdd(kp.field + i, (a > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
+                unsafeAtomicAdd(kp.field2 + i, (a > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
+            }
+        } else {
+            const float s = a + w;
+            const float bb = s - a;
+            const float e = (a - (s - bb)) + (w - bb);
+            if (e != 0.f) unsafeAtomicAdd(kp.field2 + i, e);
+        }
+    }
+
     __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight) {
         const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
         ndep++;
-        if (TALLY == MCX_TALLY_F32_SHADOW) {
-            float oldval = unsafeAtomicAdd(kp.field + i, weight);
-            if (fabsf(oldval) > MAX_ACCUM) {
-                unsafeAtomicAdd(kp.field + i, (oldval > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
-                unsafeAtomicAdd(kp.field2 + i, (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
-            }
-        } else if (TALLY == MCX_TALLY_F32_HILO) {
-            /* TwoSum of (old, weight): the atomic stored s = fl(old + weight);
-             * the rounding error e is exact and goes to the lo half. */
-            float a = unsafeAtomicAdd(kp.field + i, weight);
-            float s = a + weight;
-            float bb = s - a;
-            float e = (a - (s - bb)) + (weight - bb);
-            if (e != 0.f) unsafeAtomicAdd(kp.field2 + i, e);
+        if (TALLY == MCX_TALLY_F64) {
+            unsafeAtomicAdd(tally64 + i, (double)weight);  /* no return: fire and forget */
+        } else if (TALLY == MCX_TALLY_F64_XCD) {
+            /* only this XCD's workgroups touch this copy: the add can
+             * complete in the XCD's own L2 */
+            __hip_atomic_fetch_add(tally64 + i, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
-            unsafeAtomicAdd(kp.field64 + i, (double)weight);
+            flush_pending();
+            pend_old = unsafeAtomicAdd(kp.field + i, weight);
+            pend_w = weight;
+            pend_i = i;
         }
     }
 
@@ -488,6 +518,9 @@ struct Lane {
         esc = kp.genergy[2 * idx];
         launched = kp.genergy[2 * idx + 1];
         winit = 0.f;
+        pend_i = ~(size_t)0;
+        tally64 = kp.field64;
+        if (TALLY == MCX_TALLY_F64_XCD) tally64 += (size_t)xcc_id() * kp.fieldlen;
         nstep = ndep = nscat = nlaunch = 0;
         uint32_t idx1dold, mediaidold = 0, isdet = 0;
         float n1, len, slen;
@@ -694,9 +727,11 @@ struct Lane {
             }
             if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
                 atomicAdd(kp.detcount + 1, 1u);
+                flush_pending();
                 return;
             }
         }
+        flush_pending();
         kp.genergy[2 * idx] = esc;
         kp.genergy[2 * idx + 1] = launched;
     }
@@ -709,8 +744,11 @@ using namespace mcxk;
 /* The parameter block is read through a device pointer: every field is a
  * wave-uniform load served by the scalar cache.  (Binding a by-value kernel
  * argument to a reference would copy the whole block to scratch.) */
+#ifndef MCX_MIN_WAVES_PER_SIMD
+#define MCX_MIN_WAVES_PER_SIMD 1
+#endif
 template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
-__global__ void __launch_bounds__(256) mcx_transport(const KParam* __restrict__ kpp) {
+__global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(const KParam* __restrict__ kpp) {
     const KParam& kp = *kpp;
     extern __shared__ F4 lds[];
     for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
@@ -732,6 +770,99 @@ __global__ void __launch_bounds__(256) mcx_transport(const KParam* __restrict__ 
         atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)L.nscat);
         atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)L.nlaunch);
     }
+}
+
+/* Batched persistent launch: every wave repeatedly takes the next chunk of
+ * 64 virtual threads from a global queue; a chunk belongs to one instance,
+ * so the instance parameters stay wave-uniform (scalar loads) and each
+ * wave keeps that instance's property table in its own LDS slice.  Chunks
+ * are numbered in `order` (most expensive instances first) with
+ * chunk_start[k] the first chunk of the k-th instance in that order.  A
+ * virtual thread's work is identical to the single-instance kernel's, so a
+ * batched instance reproduces its plain run lane for lane. */
+template <int TALLY>
+__global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD)
+mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restrict__ chunk_start,
+                    const uint32_t* __restrict__ order, uint32_t ninst, uint32_t total_chunks, uint32_t maxtab,
+                    uint32_t* __restrict__ queue) {
+    extern __shared__ F4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    F4* tab = lds + wave * maxtab;
+    while (true) {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1u);
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+        if (c >= total_chunks) break;
+        uint32_t lo = 0, hi = ninst;          /* last k with chunk_start[k] <= c */
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (chunk_start[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint32_t inst = __builtin_amdgcn_readfirstlane(order[lo]);
+        const KParam& kp = params[inst];
+        for (uint32_t i = lane; i < kp.ntable; i += 64) {
+            const float* g = kp.gprop + 4 * i;
+            tab[i] = F4{g[0], g[1], g[2], g[3]};
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t idx = (c - chunk_start[lo]) * 64u + lane;
+        const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
+        if (idx < kp.nthread && (long long)idx < total) {
+            Lane<0, 1, TALLY, uint8_t> L(kp, tab, (float*)tab, 64, idx);
+            L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
+            L.run();
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+/* FP64 sums of the per-lane escaped / launched energies of each instance */
+__global__ void mcx_batch_energy(const KParam* __restrict__ params, BatchInst* __restrict__ bi) {
+    __shared__ double red[2][256];
+    const KParam& kp = params[blockIdx.x];
+    double e = 0.0, t = 0.0;
+    for (uint32_t i = threadIdx.x; i < kp.nthread; i += blockDim.x) {
+        e += (double)kp.genergy[2 * i];
+        t += (double)kp.genergy[2 * i + 1];
+    }
+    red[0][threadIdx.x] = e;
+    red[1][threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + s];
+            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bi[blockIdx.x].energy[0] = red[0][0];
+        bi[blockIdx.x].energy[1] = red[1][0];
+    }
+}
+
+/* merge the tally halves in FP64 and normalise in FP64, one row of blocks
+ * per instance (blockIdx.y) */
+template <int TALLY>
+__global__ void mcx_batch_merge(const KParam* __restrict__ params, const BatchInst* __restrict__ bi) {
+    const KParam& kp = params[blockIdx.y];
+    const BatchInst& b = bi[blockIdx.y];
+    const double scale = b.normalize ? b.scale_num / (b.energy[1] * b.scale_den) : 1.0;
+    const size_t n = kp.fieldlen;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v;
+        if (TALLY == MCX_TALLY_F64)
+            v = kp.field64[i];
+        else
+            v = (double)kp.field[i] + (double)kp.field2[i];
+        b.out[i] = (float)(v * scale);
+    }
+}
+
+__global__ void mcx_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 /* src/mcx_core.cu:1731-1742: n draws of one lane's stream */
@@ -765,6 +896,16 @@ __global__ void mcx_merge_hilo(float* __restrict__ out, const float* __restrict_
     for (; i < n; i += stride) out[i] = (float)((double)hi[i] + (double)lo[i]);
 }
 
+__global__ void mcx_merge_f64x(float* __restrict__ out, const double* __restrict__ d, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        double t = 0.0;
+        for (int c = 0; c < MCX_NUM_XCD; c++) t += d[(size_t)c * n + i];
+        out[i] = (float)t;
+    }
+}
+
 __global__ void mcx_merge_f64(float* __restrict__ out, const double* __restrict__ d, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -785,6 +926,7 @@ static mcx_kernel_fn pick_tally(int tally, int media32) {
     switch (tally) {
         case MCX_TALLY_F32_HILO: return pick_mt<P, R, MCX_TALLY_F32_HILO>(media32);
         case MCX_TALLY_F64: return pick_mt<P, R, MCX_TALLY_F64>(media32);
+        case MCX_TALLY_F64_XCD: return pick_mt<P, R, MCX_TALLY_F64_XCD>(media32);
         default: return pick_mt<P, R, MCX_TALLY_F32_SHADOW>(media32);
     }
 }
@@ -815,6 +957,44 @@ extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, 
     return e;
 }
 
+extern "C" hipError_t mcxk_launch_batch(int tally, const KParam* params, const uint32_t* chunk_start, const uint32_t* order,
+                                        uint32_t ninst, uint32_t total_chunks, uint32_t maxtab, uint32_t* queue,
+                                        unsigned int grid, unsigned int block, hipStream_t stream) {
+    const size_t shmem = (size_t)(block / 64) * maxtab * sizeof(F4);
+    if (tally == MCX_TALLY_F32_HILO)
+        hipLaunchKernelGGL(mcx_transport_batch<MCX_TALLY_F32_HILO>, dim3(grid), dim3(block), shmem, stream, params, chunk_start,
+                           order, ninst, total_chunks, maxtab, queue);
+    else if (tally == MCX_TALLY_F64)
+        hipLaunchKernelGGL(mcx_transport_batch<MCX_TALLY_F64>, dim3(grid), dim3(block), shmem, stream, params, chunk_start,
+                           order, ninst, total_chunks, maxtab, queue);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_batch_attrs(int tally, int* numregs) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, tally == MCX_TALLY_F64 ? (const void*)mcx_transport_batch<MCX_TALLY_F64>
+                                                                   : (const void*)mcx_transport_batch<MCX_TALLY_F32_HILO>);
+    if (e == hipSuccess) *numregs = a.numRegs;
+    return e;
+}
+
+extern "C" hipError_t mcxk_launch_batch_post(int tally, const KParam* params, BatchInst* bi, uint32_t ninst, hipStream_t stream) {
+    hipLaunchKernelGGL(mcx_batch_energy, dim3(ninst), dim3(256), 0, stream, params, bi);
+    if (tally == MCX_TALLY_F64)
+        hipLaunchKernelGGL(mcx_batch_merge<MCX_TALLY_F64>, dim3(512, ninst), dim3(256), 0, stream, params, (const BatchInst*)bi);
+    else
+        hipLaunchKernelGGL(mcx_batch_merge<MCX_TALLY_F32_HILO>, dim3(512, ninst), dim3(256), 0, stream, params, (const BatchInst*)bi);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t stream) {
+    size_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(mcx_fill_u32, dim3((unsigned int)(g > 4096 ? 4096 : (g ? g : 1))), dim3(256), 0, stream, p, v, n);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream) {
     hipLaunchKernelGGL(mcx_test_rng, dim3(1), dim3(64), 0, stream, out, seeds, n);
     return hipGetLastError();
@@ -828,6 +1008,8 @@ extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, co
         hipLaunchKernelGGL(mcx_merge_hilo, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b, n);
     else if (tally == MCX_TALLY_F64)
         hipLaunchKernelGGL(mcx_merge_f64, dim3(grid), dim3(block), 0, stream, out, (const double*)a, n);
+    else if (tally == MCX_TALLY_F64_XCD)
+        hipLaunchKernelGGL(mcx_merge_f64x, dim3(grid), dim3(block), 0, stream, out, (const double*)a, n);
     else
         hipLaunchKernelGGL(mcx_merge_f32, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b, n);
     return hipGetLastError();

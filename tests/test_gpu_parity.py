@@ -122,7 +122,7 @@ def test_source_types_bitexact(src):
     _check_flux(res, ora, rtol=2e-4, floor=1e-2)
 
 
-@pytest.mark.parametrize("tally,rtol", [(1, 2e-6), (2, 1e-6)])
+@pytest.mark.parametrize("tally,rtol", [(1, 2e-6), (2, 1e-6), (3, 1e-6)])
 def test_compensated_and_fp64_tallies(tally, rtol):
     cfg = workloads.cube60(100000)
     res = _run(_layout(cfg), tally=tally)
