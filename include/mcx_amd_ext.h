@@ -10,6 +10,8 @@
 
 #include <stddef.h>
 
+#include "mcx_abi.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -49,10 +51,45 @@ int  mcx_amd_last_stats(double* out, int n);
  * success. */
 int  mcx_amd_atomic_bench(int mode, size_t nelem, unsigned int nper, unsigned int nblocks, double* out);
 
+/* Options and results of mcx_run_batch.  Inputs: zero selects the default
+ * (tally 1; photons_per_lane and block_size autotuned; one execution).
+ * Outputs are written back. */
+typedef struct McxBatchOptions {
+    int   tally_mode;        /* 1: FP32 hi/lo compensated, merged in FP64 (default); 2: FP64 */
+    int   photons_per_lane;  /* virtual threads per instance = ceil(nphoton / this), rounded up to 64;
+                                0: each Config's own nthread when autopilot is off, else autotuned */
+    int   block_size;        /* workgroup size, multiple of 64; 0: autotuned */
+    int   use_graph;         /* capture clear + transport + energy sums + merge in a hipGraph */
+    int   repeat;            /* executions of the sequence (results are the last one's); <= 1: once */
+    int   download;          /* copy each normalised fluence to cfg->exportfield (allocated when NULL) */
+    float tune_fraction;     /* photon fraction used by the autotuner's trial runs (0: 1/16) */
+    int   chosen_photons_per_lane;  /* out */
+    int   chosen_block_size;        /* out */
+    int   vgprs;                    /* out: vector registers per lane of the batched kernel */
+    float kernel_ms;         /* out: transport kernel time of the last execution (graph: whole graph) */
+    float total_ms;          /* out: last execution, clear to merge */
+    double photons;          /* out: photons simulated per execution */
+    double lanes;            /* out: virtual threads over all instances */
+} McxBatchOptions;
+
+/* Runs `ncfg` independent simulations in one persistent kernel launch.
+ * Extends mcx_run_simulation (src/mcx_core.h:40-42; same preconditions on
+ * each Config, e.g. after mcx_validatecfg) to many instances: each gets its
+ * own volume, media, source, time gates, photon count and seed, and on
+ * return its energytot/energyesc/energyabs/normalizer/runtime and, with
+ * `download`, its normalised fluence in exportfield (overwritten, not
+ * accumulated).  An instance with autopilot off and the same nthread as a
+ * single-simulation run reproduces that run's per-thread trajectories.
+ * Not batched: detected photons, respin != 1, labels >= 128.
+ * `gpu` may be NULL (device from cfgs[0]->deviceid).  Returns 0, or a
+ * negative error code with the message in mcx_last_error(). */
+int  mcx_run_batch(Config** cfgs, int ncfg, GPUInfo* gpu, McxBatchOptions* opt);
+
 /* sizeof/offsetof values of the ABI structs as compiled into the library:
  * sizeof(Config), sizeof(GPUInfo), sizeof(History), then the offsets of
  * Config.srcpos, .session, .flog, .his, .exportfield, .srcparam1,
- * .deviceid, .energytot, .bc, .srcdata.  Returns the count available. */
+ * .deviceid, .energytot, .bc, .srcdata, then sizeof(McxBatchOptions) and
+ * offsetof(McxBatchOptions, photons).  Returns the count available. */
 int  mcx_amd_abi_layout(size_t* out, int n);
 
 #ifdef __cplusplus

@@ -5,11 +5,13 @@ The engine is libmcx_amd.so (HIP kernel + C++ host layer, C ABI declared in
 include/mcx_abi.h).  This package exposes the pmcx-style dictionary API.
 """
 from .pmcx import gpuinfo, run, version  # noqa: F401
+from .batch import run_batch  # noqa: F401
 from . import _lib  # noqa: F401
 
 
 def set_tally_mode(mode):
-    """0: FP32 + MAX_ACCUM shadow (reference), 1: FP32 hi/lo compensated, 2: FP64."""
+    """0: FP32 + MAX_ACCUM shadow (reference), 1: FP32 hi/lo compensated, 2: FP64,
+    3: FP64 with one private copy per XCD."""
     names = {"f32": 0, "shadow": 0, "hilo": 1, "f64": 2}
     m = names.get(mode, mode) if isinstance(mode, str) else int(mode)
     old = _lib.lib().mcx_amd_set_tally_mode(m)

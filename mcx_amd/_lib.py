@@ -114,7 +114,7 @@ ABI_SYMBOLS = [
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
-    "mcx_amd_abi_layout", "mcx_amd_atomic_bench",
+    "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
 ]
 
 _lib = None
@@ -151,6 +151,19 @@ def _setup(L):
     L.mcx_amd_atomic_bench.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint, ctypes.c_uint, P(ctypes.c_double)]
     L.mcx_amd_atomic_bench.restype = ctypes.c_int
     L.mcx_amd_abi_layout.restype = ctypes.c_int
+    L.mcx_run_batch.argtypes = [P(P(Config)), ctypes.c_int, P(GPUInfo), P(McxBatchOptions)]
+    L.mcx_run_batch.restype = ctypes.c_int
+
+
+class McxBatchOptions(ctypes.Structure):
+    """Mirror of McxBatchOptions in include/mcx_amd_ext.h."""
+    _fields_ = [
+        ("tally_mode", ctypes.c_int), ("photons_per_lane", ctypes.c_int), ("block_size", ctypes.c_int),
+        ("use_graph", ctypes.c_int), ("repeat", ctypes.c_int), ("download", ctypes.c_int),
+        ("tune_fraction", ctypes.c_float), ("chosen_photons_per_lane", ctypes.c_int),
+        ("chosen_block_size", ctypes.c_int), ("vgprs", ctypes.c_int), ("kernel_ms", ctypes.c_float),
+        ("total_ms", ctypes.c_float), ("photons", ctypes.c_double), ("lanes", ctypes.c_double),
+    ]
 
 
 def check_layout(L):
@@ -160,7 +173,7 @@ def check_layout(L):
     want = [ctypes.sizeof(Config), ctypes.sizeof(GPUInfo), ctypes.sizeof(History), Config.srcpos.offset,
             Config.session.offset, Config.flog.offset, Config.his.offset, Config.exportfield.offset,
             Config.srcparam1.offset, Config.deviceid.offset, Config.energytot.offset, Config.bc.offset,
-            Config.srcdata.offset]
+            Config.srcdata.offset, ctypes.sizeof(McxBatchOptions), McxBatchOptions.photons.offset]
     got = list(out[:n])
     if got != want[:n]:
         raise RuntimeError("Config layout mismatch between libmcx_amd.so %s and the ctypes mirror %s" % (got, want))

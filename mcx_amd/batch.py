@@ -1,0 +1,75 @@
+"""Batched execution: many independent simulations in one persistent launch.
+
+run_batch(cfgs) takes a list of pmcx dicts (as accepted by run()) and calls
+mcx_run_batch through the C ABI (include/mcx_amd_ext.h).  Each result is the
+same dict run() returns for that configuration ({"flux", "stat"}); the second
+return value describes the launch (photons, lanes, chosen shape, times).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, GPUInfo, McxBatchOptions
+from .pmcx import _check, _libc, _parse_config
+
+
+def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, repeat=1, download=True,
+              tune_fraction=0.0):
+    L = _lib.lib()
+    n = len(cfgs)
+    if n == 0:
+        raise ValueError("run_batch needs at least one configuration")
+    cs = [Config() for _ in range(n)]
+    gpu = ctypes.POINTER(GPUInfo)()
+    results = []
+    try:
+        for i, (user_cfg, c) in enumerate(zip(cfgs, cs)):
+            _parse_config(dict(user_cfg), c)
+            if i == 0:
+                ndev = L.mcx_list_gpu(ctypes.byref(c), ctypes.byref(gpu))
+                _check()
+                if not ndev:
+                    raise RuntimeError("No GPU device found")
+            else:
+                c.deviceid = cs[0].deviceid
+            L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+            _check()
+            if not c.vol or c.medianum == 0:
+                raise ValueError("You must define 'vol' and 'prop' field.")
+            if download:
+                ngates = int((c.tend - c.tstart) / c.tstep + 0.5)
+                nvox = c.dim.x * c.dim.y * c.dim.z
+                c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates, 4), ctypes.POINTER(ctypes.c_float))
+        arr = (ctypes.POINTER(Config) * n)(*[ctypes.pointer(c) for c in cs])
+        opt = McxBatchOptions()
+        opt.tally_mode = int(tally)
+        opt.photons_per_lane = int(photons_per_lane)
+        opt.block_size = int(block_size)
+        opt.use_graph = 1 if use_graph else 0
+        opt.repeat = int(repeat)
+        opt.download = 1 if download else 0
+        opt.tune_fraction = float(tune_fraction)
+        rc = L.mcx_run_batch(arr, n, gpu, ctypes.byref(opt))
+        if rc != 0:
+            _check()
+            raise RuntimeError("mcx_run_batch failed (%d)" % rc)
+        for c in cs:
+            ngates = int((c.tend - c.tstart) / c.tstep + 0.5)
+            out = {"stat": {"runtime": int(c.runtime), "nphoton": int(c.nphoton), "energytot": float(c.energytot),
+                            "energyabs": float(c.energyabs), "normalizer": float(c.normalizer),
+                            "unitinmm": float(c.unitinmm)}}
+            if download:
+                nvox = c.dim.x * c.dim.y * c.dim.z
+                f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates,)).copy()
+                out["flux"] = f.reshape((c.dim.x, c.dim.y, c.dim.z, ngates), order="F")
+            results.append(out)
+        info = {"photons": opt.photons, "lanes": int(opt.lanes), "kernel_ms": float(opt.kernel_ms),
+                "total_ms": float(opt.total_ms), "photons_per_lane": int(opt.chosen_photons_per_lane),
+                "block_size": int(opt.chosen_block_size), "vgprs": int(opt.vgprs)}
+    finally:
+        if gpu:
+            L.mcx_cleargpuinfo(ctypes.byref(gpu))
+        for c in cs:
+            L.mcx_clearcfg(ctypes.byref(c))
+    return results, info

@@ -23,6 +23,7 @@
 #include <mutex>
 #include <vector>
 
+#include "../../include/mcx_amd_ext.h"
 #include "mcx_internal.h"
 
 namespace mcx {
@@ -471,7 +472,8 @@ static_assert(offsetof(Config, srcid) == 5280 && offsetof(Config, srcdata) == 52
 extern "C" int mcx_amd_abi_layout(size_t* out, int n) {
     const size_t v[] = {sizeof(Config), sizeof(GPUInfo), sizeof(History), offsetof(Config, srcpos), offsetof(Config, session),
                         offsetof(Config, flog), offsetof(Config, his), offsetof(Config, exportfield), offsetof(Config, srcparam1),
-                        offsetof(Config, deviceid), offsetof(Config, energytot), offsetof(Config, bc), offsetof(Config, srcdata)};
+                        offsetof(Config, deviceid), offsetof(Config, energytot), offsetof(Config, bc), offsetof(Config, srcdata),
+                        sizeof(McxBatchOptions), offsetof(McxBatchOptions, photons)};
     int k = n < (int)(sizeof(v) / sizeof(v[0])) ? n : (int)(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < k; i++) out[i] = v[i];
     return (int)(sizeof(v) / sizeof(v[0]));

@@ -27,8 +27,7 @@
 #include <vector>
 
 #include "../../include/mcx_amd_ext.h"
-#include "mcx_internal.h"
-#include "mcx_kernel.h"
+#include "mcx_device.h"
 
 /* dkp: the launch parameters in device memory */
 extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int isreflect, int tally, int media32,
@@ -44,51 +43,6 @@ static double g_stats[11];
 static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
 
-#define HIP_CHECK(x)                                                                                      \
-    do {                                                                                                  \
-        hipError_t e_ = (x);                                                                              \
-        if (e_ != hipSuccess)                                                                             \
-            throw mcx::Error(-(int)e_, std::string("HIP error (") + hipGetErrorString(e_) + ") in " #x); \
-    } while (0)
-
-/* device allocation released on scope exit */
-struct DevMem {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevMem() = default;
-    DevMem(const DevMem&) = delete;
-    DevMem& operator=(const DevMem&) = delete;
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-    void alloc(size_t n) {
-        bytes = n;
-        if (n) HIP_CHECK(hipMalloc(&p, n));
-    }
-    template <typename T>
-    T* as() const { return (T*)p; }
-};
-
-struct Stream {
-    hipStream_t s = nullptr;
-    Stream() { HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
-    ~Stream() {
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
-
-struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    EventPair() {
-        HIP_CHECK(hipEventCreate(&a));
-        HIP_CHECK(hipEventCreate(&b));
-    }
-    ~EventPair() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-};
-
 static bool supported_source(int t) {
     switch (t) {
         case MCX_SRC_PENCIL: case MCX_SRC_ISOTROPIC: case MCX_SRC_CONE: case MCX_SRC_GAUSSIAN: case MCX_SRC_PLANAR:
@@ -99,7 +53,7 @@ static bool supported_source(int t) {
     }
 }
 
-static void check_supported(const Config* cfg) {
+void check_supported(const Config* cfg) {
     if (cfg->polmedianum > 0) MCX_THROW(-1, "polarized photon simulation is not supported by this engine");
     if (cfg->mediabyte > 4) MCX_THROW(-1, "only label-based media formats are supported by this engine");
     if (cfg->seed == MCX_SEED_FROM_FILE) MCX_THROW(-1, "photon replay is not supported by this engine");
@@ -116,10 +70,61 @@ static void check_supported(const Config* cfg) {
 
 /* waves per SIMD for a kernel using `vgprs` architectural VGPRs (gfx950:
  * 512 per lane per SIMD, granule 8, at most 8 waves) */
-static int waves_per_simd(int vgprs) {
+int waves_per_simd(int vgprs) {
     if (vgprs <= 0) return 8;
     int g = (vgprs + 7) / 8 * 8;
     return std::max(1, std::min(8, 512 / g));
+}
+
+
+/* launch constants of one simulation (buffers, work split and time
+ * window are set by the caller) */
+void fill_kparam(const Config* cfg, unsigned int maxgate, unsigned int is2d, KParam& kp) {
+    memset(&kp, 0, sizeof(kp));
+    const size_t nvox = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+    const unsigned int partialdata = det_partialdata(cfg);
+    const unsigned int hostdetreclen = det_reclen(cfg);
+    const unsigned int ntable = cfg->medianum + cfg->detnum;
+    kp.nx = cfg->dim.x;
+    kp.ny = cfg->dim.y;
+    kp.nz = cfg->dim.z;
+    kp.dimxy = cfg->dim.x * cfg->dim.y;
+    kp.nvox = (uint32_t)nvox;
+    kp.maxidx_x = (float)cfg->dim.x;
+    kp.maxidx_y = (float)cfg->dim.y;
+    kp.maxidx_z = (float)cfg->dim.z;
+    memcpy(kp.srcpos, &cfg->srcpos, 16);
+    memcpy(kp.srcdir, &cfg->srcdir, 16);
+    memcpy(kp.srcparam1, &cfg->srcparam1, 16);
+    memcpy(kp.srcparam2, &cfg->srcparam2, 16);
+    kp.srctype = cfg->srctype;
+    kp.Rtstep = 1.f / cfg->tstep;
+    kp.oneoverc0 = MCX_R_C0 * cfg->unitinmm;
+    kp.minaccumtime = 1.f * MCX_R_C0 * cfg->unitinmm;
+    kp.maxgate = maxgate;
+    kp.minenergy = cfg->minenergy;
+    kp.doreflect = (uint32_t)cfg->isreflect;
+    kp.isspecular = cfg->isspecular > 0;
+    kp.voidtime = (uint32_t)cfg->voidtime;
+    kp.maxvoidstep = (uint32_t)cfg->maxvoidstep;
+    kp.gscatter = cfg->gscatter;
+    kp.outputtype = (uint32_t)cfg->outputtype;
+    kp.save2pt = (uint32_t)cfg->issave2pt;
+    kp.is2d = is2d;
+    kp.bcface = kp.bcdet = 0;
+    for (int i = 0; i < 6; i++) {
+        kp.bcface |= ((uint32_t)(unsigned char)cfg->bc[i] & 0xFu) << (4 * i);
+        kp.bcdet |= (uint32_t)(cfg->bc[6 + i] != 0) << i;
+    }
+    kp.savedet = (uint32_t)cfg->issavedet;
+    kp.savedetflag = cfg->savedetflag;
+    kp.maxdetphoton = cfg->maxdetphoton;
+    kp.reclen = hostdetreclen;
+    kp.partialdata = partialdata;
+    kp.maxmedia = cfg->medianum - 1;
+    kp.detnum = cfg->detnum;
+    kp.medianum = cfg->medianum;
+    kp.ntable = ntable;
 }
 
 static void rng_test_path(Config* cfg, GPUInfo* gpu, int gpuid, size_t fieldlen) {
@@ -273,43 +278,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     HIP_CHECK(hipMemsetAsync(dcounters.p, 0, dcounters.bytes, st.s));
 
     KParam kp;
-    memset(&kp, 0, sizeof(kp));
-    kp.nx = cfg->dim.x;
-    kp.ny = cfg->dim.y;
-    kp.nz = cfg->dim.z;
-    kp.dimxy = cfg->dim.x * cfg->dim.y;
-    kp.nvox = (uint32_t)nvox;
-    kp.maxidx_x = (float)cfg->dim.x;
-    kp.maxidx_y = (float)cfg->dim.y;
-    kp.maxidx_z = (float)cfg->dim.z;
-    memcpy(kp.srcpos, &cfg->srcpos, 16);
-    memcpy(kp.srcdir, &cfg->srcdir, 16);
-    memcpy(kp.srcparam1, &cfg->srcparam1, 16);
-    memcpy(kp.srcparam2, &cfg->srcparam2, 16);
-    kp.srctype = cfg->srctype;
-    kp.Rtstep = 1.f / cfg->tstep;
-    kp.oneoverc0 = MCX_R_C0 * cfg->unitinmm;
-    kp.minaccumtime = 1.f * MCX_R_C0 * cfg->unitinmm;
-    kp.maxgate = maxgate;
-    kp.minenergy = cfg->minenergy;
-    kp.doreflect = (uint32_t)cfg->isreflect;
-    kp.isspecular = cfg->isspecular > 0;
-    kp.voidtime = (uint32_t)cfg->voidtime;
-    kp.maxvoidstep = (uint32_t)cfg->maxvoidstep;
-    kp.gscatter = cfg->gscatter;
-    kp.outputtype = (uint32_t)cfg->outputtype;
-    kp.save2pt = (uint32_t)cfg->issave2pt;
-    kp.is2d = is2d;
-    memcpy(kp.bc, cfg->bc, 12);
-    kp.savedet = (uint32_t)cfg->issavedet;
-    kp.savedetflag = cfg->savedetflag;
-    kp.maxdetphoton = cfg->maxdetphoton;
-    kp.reclen = hostdetreclen;
-    kp.partialdata = partialdata;
-    kp.maxmedia = cfg->medianum - 1;
-    kp.detnum = cfg->detnum;
-    kp.medianum = cfg->medianum;
-    kp.ntable = ntable;
+    fill_kparam(cfg, maxgate, is2d, kp);
     kp.threadphoton = threadphoton;
     kp.oddphotons = oddphotons;
     kp.nthread = nthread;

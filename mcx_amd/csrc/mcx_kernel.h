@@ -35,7 +35,8 @@ struct KParam {
     /* physics */
     float minenergy;
     uint32_t doreflect, isspecular, voidtime, maxvoidstep, gscatter, outputtype, save2pt, is2d;
-    uint8_t bc[12];
+    uint32_t bcface;  /* face f (x-,y-,z-,x+,y+,z+) boundary code in bits 4f..4f+3 */
+    uint32_t bcdet;   /* bit f: face f of the domain is a detector (bc chars 7-12) */
     /* detected photons */
     uint32_t savedet, savedetflag, maxdetphoton, reclen, partialdata, maxmedia, detnum, medianum;
     uint32_t ntable; /* medianum + detnum rows of float4 */
@@ -56,13 +57,16 @@ struct KParam {
     uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
 };
 
-/* One instance of a batched launch (device side; parameters in KParam). */
+/* One instance of a batched launch (its launch constants are in KParam) */
 struct BatchInst {
-    float* out;        /* merged, normalised FP32 fluence (fieldlen) */
-    double* energy;    /* [0] escaped, [1] launched: FP64 sums over lanes */
-    double scale_num;  /* normalisation numerator: unitinmm (flux/fluence) or 1 (energy) */
-    double scale_den;  /* multiplies energytot in the denominator: Vvox*tstep (flux), Vvox (fluence), 1 (energy) */
-    int normalize;     /* 0: raw tally */
+    float* out;          /* merged, normalised FP32 fluence (fieldlen) */
+    double* energy;      /* out: [0] escaped, [1] launched (FP64 sums over lanes), [2] normalizer */
+    float unitinmm;      /* normalisation inputs, as the single-simulation path uses them */
+    float vvox;
+    float tstep;
+    int32_t outputtype;
+    int32_t normalize;   /* 0: raw tally */
+    int32_t pad;
 };
 
 #endif

@@ -42,18 +42,40 @@ enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL };
 
 struct F4 { float x, y, z, w; };
 
+/* The buffers arrive through the KParam block in device memory, so the
+ * compiler cannot see that they are global memory and would emit FLAT
+ * instructions, whose completion waits on both the vector-memory and the
+ * LDS/scalar counters.  Every buffer access goes through these casts to
+ * the global address space (global_load / global_atomic instructions). */
+#define MCX_GLB __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ MCX_GLB T* glb(T* p) { return (MCX_GLB T*)p; }
+
+__device__ __forceinline__ void atom_add_nr(double* p, double v) {
+    __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atom_add_nr(float* p, float v) {
+    __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float atom_add_ret(float* p, float v) {
+    return __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atom_add_u32(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename MT>
 __device__ __forceinline__ uint32_t load_media(const void* m, uint32_t i);
 
 template <>
 __device__ __forceinline__ uint32_t load_media<uint8_t>(const void* m, uint32_t i) {
-    uint32_t b = ((const uint8_t*)m)[i];
+    uint32_t b = glb((const uint8_t*)m)[i];
     return (b & 0x7Fu) | ((b & 0x80u) << 24);
 }
 
 template <>
 __device__ __forceinline__ uint32_t load_media<uint32_t>(const void* m, uint32_t i) {
-    return ((const uint32_t*)m)[i];
+    return glb((const uint32_t*)m)[i];
 }
 
 /* XCD executing this wave: s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4),
@@ -227,12 +249,12 @@ struct Lane {
         const uint32_t lane = __lane_id();
         const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(kp.detcount, (uint32_t)__popcll(mask));
+        if (lane == leader) base = atom_add_u32(kp.detcount, (uint32_t)__popcll(mask));
         base = __shfl(base, leader);
         if (!hit) return;
         const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
         if (slot < kp.maxdetphoton) {
-            float* rec = kp.detdata + (size_t)slot * kp.reclen;
+            MCX_GLB float* rec = glb(kp.detdata) + (size_t)slot * kp.reclen;
             uint32_t k = 0;
             const uint32_t flag = kp.savedetflag;
             if (flag & 1) rec[k++] = (float)detid;
@@ -241,7 +263,7 @@ struct Lane {
             if ((flag >> 5) & 1) { rec[k++] = v.x; rec[k++] = v.y; rec[k++] = v.z; }
             if ((flag >> 6) & 1) rec[k++] = winit;
         } else if (kp.savedet == 3) {
-            atomicSub(kp.detcount, 1u);
+            atom_add_u32(kp.detcount, ~0u);
         }
     }
 
@@ -420,7 +442,7 @@ struct Lane {
      * next one.  Returns 0 launched, 1 lane finished, -1 launch failed. */
     __device__ __forceinline__ int launch(uint32_t isdet) {
         w0 = 1.f;
-        if (kp.savedet == 3 && __hip_atomic_load(kp.detcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= kp.maxdetphoton)
+        if (kp.savedet == 3 && __hip_atomic_load(glb(kp.detcount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= kp.maxdetphoton)
             return 1;
         const bool live = fabsf(p.w) >= 0.f;
         if (live) esc += p.w;
@@ -435,7 +457,8 @@ struct Lane {
             f = {0.f, 0.f, kp.minaccumtime, f.w};
             idx1d = mcx_f2bits(kp.srcparam2[2]);
             mediaid = mcx_f2bits(kp.srcparam2[3]);
-            if (!ISPENCIL) {
+            /* ISPENCIL 2: decided per launch parameter block (batched launch) */
+            if (!(ISPENCIL == 1 || (ISPENCIL == 2 && kp.srctype == 0))) {
                 bool skip;
                 launch_source(skip);
                 if (skip) continue;
@@ -476,14 +499,14 @@ struct Lane {
         pend_i = ~(size_t)0;
         if (TALLY == MCX_TALLY_F32_SHADOW) {
             if (fabsf(a) > MAX_ACCUM) {
-                unsafeAtomicAdd(kp.field + i, (a > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
-                unsafeAtomicAdd(kp.field2 + i, (a > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
+                atom_add_nr(kp.field + i, (a > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
+                atom_add_nr(kp.field2 + i, (a > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
             }
         } else {
             const float s = a + w;
             const float bb = s - a;
             const float e = (a - (s - bb)) + (w - bb);
-            if (e != 0.f) unsafeAtomicAdd(kp.field2 + i, e);
+            if (e != 0.f) atom_add_nr(kp.field2 + i, e);
         }
     }
 
@@ -491,14 +514,14 @@ struct Lane {
         const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
         ndep++;
         if (TALLY == MCX_TALLY_F64) {
-            unsafeAtomicAdd(tally64 + i, (double)weight);  /* no return: fire and forget */
+            atom_add_nr(tally64 + i, (double)weight);  /* no return: fire and forget */
         } else if (TALLY == MCX_TALLY_F64_XCD) {
             /* only this XCD's workgroups touch this copy: the add can
              * complete in the XCD's own L2 */
-            __hip_atomic_fetch_add(tally64 + i, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(glb(tally64) + i, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             flush_pending();
-            pend_old = unsafeAtomicAdd(kp.field + i, weight);
+            pend_old = atom_add_ret(kp.field + i, weight);
             pend_w = weight;
             pend_i = i;
         }
@@ -515,8 +538,8 @@ struct Lane {
         fd3 = -1;
         mediaid = mcx_f2bits(kp.srcparam2[3]);
         idx1d = 0;
-        esc = kp.genergy[2 * idx];
-        launched = kp.genergy[2 * idx + 1];
+        esc = glb(kp.genergy)[2 * idx];
+        launched = glb(kp.genergy)[2 * idx + 1];
         winit = 0.f;
         pend_i = ~(size_t)0;
         tally64 = kp.field64;
@@ -525,8 +548,9 @@ struct Lane {
         uint32_t idx1dold, mediaidold = 0, isdet = 0;
         float n1, len, slen;
         prop = tab[1];
-        t0 = ((uint64_t)kp.seeds[4 * idx] << 32) | kp.seeds[4 * idx + 1];
-        t1 = ((uint64_t)kp.seeds[4 * idx + 2] << 32) | kp.seeds[4 * idx + 3];
+        const MCX_GLB uint32_t* sd = glb(kp.seeds) + 4 * idx;
+        t0 = ((uint64_t)sd[0] << 32) | sd[1];
+        t1 = ((uint64_t)sd[2] << 32) | sd[3];
         /* Every termination sets `relaunch` and restarts the loop, so the
          * (large) launch code exists once; the operations and their order
          * are those of the reference, where launchnewphoton is called in
@@ -600,17 +624,16 @@ struct Lane {
             if (kp.savedet && PPATH) ppath(kp.maxmedia * NSCAT + (mediaid & MED_MASK) - 1) += len;
             mediaidold = mediaid | (isdet & DET_MASK);
             idx1dold = idx1d;
-            idx1d = (uint32_t)(fd2 * (int)kp.dimxy + fd1 * (int)kp.nx + fd0);
-            if ((uint32_t)fd0 >= kp.nx || (uint32_t)fd1 >= kp.ny || (uint32_t)fd2 >= kp.nz) {
-                mediaid = 0;
-                idx1d = (fd0 < 0 || fd1 < 0 || fd2 < 0) ? OUT_MIN : OUT_MAX;
-                isdet = kp.bc[(idx1d == OUT_MAX) * 3 + fd3];
-            } else {
-                mediaid = load_media<MT>(kp.media, idx1d);
-                isdet = mediaid & DET_MASK;
-                mediaid &= MED_MASK;
-            }
-            if (idx1d != idx1dold && mediaidold) {
+            const bool inside = (uint32_t)fd0 < kp.nx && (uint32_t)fd1 < kp.ny && (uint32_t)fd2 < kp.nz;
+            const uint32_t vox = (uint32_t)(fd2 * (int)kp.dimxy + fd1 * (int)kp.nx + fd0);
+            /* The label of the voxel entered is loaded (at index 0 when the
+             * photon left the grid) before the deposit for the voxel left,
+             * which does not need it: the wait for this load then leaves
+             * the tally atomic in flight until the next step's load. */
+            const uint32_t label = load_media<MT>(kp.media, inside ? vox : 0u);
+            idx1d = inside ? vox : ((fd0 < 0 || fd1 < 0 || fd2 < 0) ? OUT_MIN : OUT_MAX);
+            const bool moved = (idx1d != idx1dold && mediaidold);
+            if (moved) {
                 if (kp.save2pt && f.y >= kp.twin0 && f.y < kp.twin1) {
                     float weight = 0.f;
                     int tshift = (int)floorf((f.y - kp.twin0) * kp.Rtstep);
@@ -625,9 +648,15 @@ struct Lane {
                 }
                 w0 = p.w;
                 f.z = 0.f;
-            } else {
-                mediaid = mediaidold;
             }
+            if (inside) {
+                isdet = label & DET_MASK;
+                mediaid = label & MED_MASK;
+            } else {
+                mediaid = 0;
+                isdet = (kp.bcface >> (4 * ((idx1d == OUT_MAX) * 3 + fd3))) & 0xFu;
+            }
+            if (!moved) mediaid = mediaidold;
             const uint32_t bcf = isdet & 0xF;
             if ((mediaid == 0 &&
                  ((!ISREFLECT || (ISREFLECT && n1 == tab[0].w)) || (bcf == bcUnknown && !kp.doreflect) ||
@@ -655,7 +684,7 @@ struct Lane {
                         continue;
                     }
                 }
-                detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                detarg = (((idx1d == OUT_MAX || idx1d == OUT_MIN) && ((kp.bcdet >> ((idx1d == OUT_MAX) * 3 + fd3)) & 1u))
                               ? OUT_MIN : (mediaidold & DET_MASK));
                 relaunch = true;
                 continue;
@@ -691,7 +720,7 @@ struct Lane {
                     if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && rnd() > Rtotal) {
                         transmit(n1, prop.w, fd3);
                         if (mediaid == 0) {
-                            detarg = (((idx1d == OUT_MAX && kp.bc[9 + fd3]) || (idx1d == OUT_MIN && kp.bc[6 + fd3]))
+                            detarg = (((idx1d == OUT_MAX || idx1d == OUT_MIN) && ((kp.bcdet >> ((idx1d == OUT_MAX) * 3 + fd3)) & 1u))
                                           ? OUT_MIN : (mediaidold & DET_MASK));
                             relaunch = true;
                             continue;
@@ -726,14 +755,14 @@ struct Lane {
                 }
             }
             if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
-                atomicAdd(kp.detcount + 1, 1u);
+                atom_add_u32(kp.detcount + 1, 1u);
                 flush_pending();
                 return;
             }
         }
         flush_pending();
-        kp.genergy[2 * idx] = esc;
-        kp.genergy[2 * idx + 1] = launched;
+        glb(kp.genergy)[2 * idx] = esc;
+        glb(kp.genergy)[2 * idx + 1] = launched;
     }
 };
 
@@ -752,7 +781,7 @@ __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(con
     const KParam& kp = *kpp;
     extern __shared__ F4 lds[];
     for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
-        const float* g = kp.gprop + 4 * i;
+        const MCX_GLB float* g = glb(kp.gprop) + 4 * i;
         lds[i] = F4{g[0], g[1], g[2], g[3]};
     }
     __syncthreads();
@@ -801,7 +830,7 @@ mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restric
         const uint32_t inst = __builtin_amdgcn_readfirstlane(order[lo]);
         const KParam& kp = params[inst];
         for (uint32_t i = lane; i < kp.ntable; i += 64) {
-            const float* g = kp.gprop + 4 * i;
+            const MCX_GLB float* g = glb(kp.gprop) + 4 * i;
             tab[i] = F4{g[0], g[1], g[2], g[3]};
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -810,7 +839,7 @@ mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restric
         const uint32_t idx = (c - chunk_start[lo]) * 64u + lane;
         const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
         if (idx < kp.nthread && (long long)idx < total) {
-            Lane<0, 1, TALLY, uint8_t> L(kp, tab, (float*)tab, 64, idx);
+            Lane<2, 1, TALLY, uint8_t> L(kp, tab, (float*)tab, 64, idx);
             L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
             L.run();
         }
@@ -843,22 +872,41 @@ __global__ void mcx_batch_energy(const KParam* __restrict__ params, BatchInst* _
     }
 }
 
-/* merge the tally halves in FP64 and normalise in FP64, one row of blocks
- * per instance (blockIdx.y) */
+/* merge the tally halves in FP64 and normalise, one row of blocks per
+ * instance (blockIdx.y).  The scale is the single-simulation path's:
+ * unitinmm / (energytot * Vvox * tstep) in FP64, rounded to FP32 (times
+ * tstep for fluence), applied to the FP32-rounded merged value. */
 template <int TALLY>
 __global__ void mcx_batch_merge(const KParam* __restrict__ params, const BatchInst* __restrict__ bi) {
     const KParam& kp = params[blockIdx.y];
     const BatchInst& b = bi[blockIdx.y];
-    const double scale = b.normalize ? b.scale_num / (b.energy[1] * b.scale_den) : 1.0;
-    const size_t n = kp.fieldlen;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        double v;
-        if (TALLY == MCX_TALLY_F64)
-            v = kp.field64[i];
-        else
-            v = (double)kp.field[i] + (double)kp.field2[i];
-        b.out[i] = (float)(v * scale);
+    const double et = b.energy[1];
+    float scale = 1.f;
+    if (b.normalize) {
+        if (b.outputtype == otFlux || b.outputtype == otFluence) {
+            scale = (float)((double)b.unitinmm / (et * (double)b.vvox * (double)b.tstep));
+            if (b.outputtype == otFluence) scale *= b.tstep;
+        } else if (b.outputtype == otEnergy || b.outputtype == otL) {
+            scale = (float)(1.0 / et);
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.energy[2] = (double)scale;
+    const size_t n = kp.fieldlen;
+    MCX_GLB float* out = glb(b.out);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v;
+        if (TALLY == MCX_TALLY_F64)
+            v = (float)glb(kp.field64)[i];
+        else
+            v = (float)((double)glb(kp.field)[i] + (double)glb(kp.field2)[i]);
+        out[i] = v * scale;
+    }
+}
+
+/* zero fill of 16-byte words (arena clears inside a captured graph) */
+__global__ void mcx_fill_zero16(uint4* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ void mcx_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -986,6 +1034,13 @@ extern "C" hipError_t mcxk_launch_batch_post(int tally, const KParam* params, Ba
         hipLaunchKernelGGL(mcx_batch_merge<MCX_TALLY_F64>, dim3(512, ninst), dim3(256), 0, stream, params, (const BatchInst*)bi);
     else
         hipLaunchKernelGGL(mcx_batch_merge<MCX_TALLY_F32_HILO>, dim3(512, ninst), dim3(256), 0, stream, params, (const BatchInst*)bi);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_fill_zero16(void* p, size_t bytes, hipStream_t stream) {
+    const size_t n = bytes / 16;
+    size_t g = (n + 255) / 256;
+    hipLaunchKernelGGL(mcx_fill_zero16, dim3((unsigned int)(g > 8192 ? 8192 : (g ? g : 1))), dim3(256), 0, stream, (uint4*)p, n);
     return hipGetLastError();
 }
 

@@ -65,3 +65,61 @@ def sphere100(nphoton=100_000_000, seed=12345, **kw):
     }
     cfg.update(kw)
     return cfg
+
+
+def batch_instance(master_seed, index, edge_range=(40, 160), photon_range=(1e5, 1e7), ngates=10, tstep=5e-10):
+    """One instance of the heterogeneous batch workload.
+
+    A counter-based generator (Philox keyed by the master seed and the
+    instance index) draws, in this order: the cube edge (uniform integer in
+    edge_range), 2-6 planar layers along z (cut planes uniform, distinct),
+    0-3 ellipsoidal inclusions (semi-axes 3-20 voxels, centres uniform in
+    the cube), per-label optical properties (mua U(0.001, 0.1)/mm, mus
+    U(0.5, 20)/mm, g U(0.7, 0.95), n U(1.33, 1.45)), the photon count
+    (log-uniform in photon_range) and the RNG seed.  Pencil source at the
+    centre of the z=0 face along +z; 10 gates of 0.5 ns; reflection on.
+    """
+    g = np.random.Generator(np.random.Philox(key=[int(master_seed), int(index)]))
+    edge = int(g.integers(edge_range[0], edge_range[1] + 1))
+    nlayer = int(g.integers(2, 7))
+    cuts = np.sort(g.choice(np.arange(1, edge), size=nlayer - 1, replace=False))
+    z = np.arange(edge)
+    zlab = (1 + np.searchsorted(cuts, z, side="right")).astype(np.uint8)
+    vol = np.broadcast_to(zlab[None, None, :], (edge, edge, edge)).copy()
+    nell = int(g.integers(0, 4))
+    c = np.arange(edge, dtype=np.float32) + np.float32(0.5)
+    for k in range(nell):
+        ctr = g.uniform(0, edge, size=3).astype(np.float32)
+        ax = g.uniform(3, 20, size=3).astype(np.float32)
+        r = (((c - ctr[0]) / ax[0])[:, None, None] ** 2 + ((c - ctr[1]) / ax[1])[None, :, None] ** 2
+             + ((c - ctr[2]) / ax[2])[None, None, :] ** 2)
+        vol[r <= 1.0] = nlayer + 1 + k
+    nlab = nlayer + nell
+    prop = np.zeros((nlab + 1, 4), dtype=np.float32)
+    prop[0] = [0, 0, 1, 1]
+    prop[1:, 0] = g.uniform(0.001, 0.1, size=nlab)
+    prop[1:, 1] = g.uniform(0.5, 20.0, size=nlab)
+    prop[1:, 2] = g.uniform(0.7, 0.95, size=nlab)
+    prop[1:, 3] = g.uniform(1.33, 1.45, size=nlab)
+    lo, hi = np.log(photon_range[0]), np.log(photon_range[1])
+    nphoton = int(round(float(np.exp(g.uniform(lo, hi)))))
+    seed = int(g.integers(1, 2**31 - 1))
+    return {
+        "nphoton": nphoton,
+        "vol": vol,
+        "prop": prop,
+        "srcpos": [edge / 2.0, edge / 2.0, 0],
+        "srcdir": [0, 0, 1],
+        "issrcfrom0": 1,
+        "tstart": 0.0,
+        "tend": tstep * ngates,
+        "tstep": tstep,
+        "seed": seed,
+        "isreflect": 1,
+        "issavedet": 0,
+    }
+
+
+def batch256(master_seed=7, n=256, **kw):
+    """The 256-instance heterogeneous batch (edges 40-160, 1e5-1e7 photons)."""
+    return [batch_instance(master_seed, i, **kw) for i in range(n)]

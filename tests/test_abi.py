@@ -263,3 +263,36 @@ def test_run_without_gpu_raises_cleanly():
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_batch_options_layout_and_empty_batch():
+    """McxBatchOptions mirror matches the compiled struct; an empty batch is
+    rejected before any device call."""
+    assert ctypes.sizeof(_lib.McxBatchOptions) == 64
+    opt = _lib.McxBatchOptions()
+    rc = L.mcx_run_batch(None, 0, None, ctypes.byref(opt))
+    assert rc < 0
+    code, msg = _lib.last_error()
+    assert "no instances" in msg
+    L.mcx_clear_error()
+
+
+def test_batch_generator_is_deterministic_and_in_range():
+    from mcx_amd import workloads
+    a = workloads.batch_instance(7, 3)
+    b = workloads.batch_instance(7, 3)
+    assert np.array_equal(a["vol"], b["vol"]) and np.array_equal(a["prop"], b["prop"])
+    assert a["seed"] == b["seed"] and a["nphoton"] == b["nphoton"]
+    for i in range(16):
+        c = workloads.batch_instance(7, i)
+        e = c["vol"].shape[0]
+        assert c["vol"].shape == (e, e, e) and 40 <= e <= 160
+        p = c["prop"][1:]
+        nlab = int(c["vol"].max())
+        assert p.shape[0] == nlab and 2 <= nlab <= 9
+        assert np.all((p[:, 0] >= 0.001) & (p[:, 0] <= 0.1)) and np.all((p[:, 1] >= 0.5) & (p[:, 1] <= 20))
+        assert np.all((p[:, 2] >= 0.7) & (p[:, 2] <= 0.95)) and np.all((p[:, 3] >= 1.33) & (p[:, 3] <= 1.45))
+        assert 1e5 <= c["nphoton"] <= 1e7
+        assert c["tend"] == pytest.approx(5e-9) and c["tstep"] == pytest.approx(5e-10)
+        # layers are planar along z: every z-slice outside inclusions has one label
+        assert len(np.unique(c["vol"][0, 0, :])) >= 1

@@ -1,0 +1,95 @@
+"""GPU tests of the batched launch (mcx_run_batch through the C ABI).
+
+A batched instance whose thread count matches a single-simulation run is
+the same computation lane for lane: same seeds, same trajectories, so the
+per-instance launched/escaped energies agree with that run's to FP64
+summation-order error, and the fluence to FP64-tally order error.  Against
+the CPU oracle (FP64 tally), the FP32 hi/lo compensated tally agrees to a
+few FP32 ulps.  Graph replays reproduce the eager execution.
+"""
+import numpy as np
+import pytest
+
+import mcx_amd
+from mcx_amd import workloads
+from parity_util import normalize_like_engine, oracle_run, seq_sum
+
+pytestmark = pytest.mark.gpu
+
+NT = 1024
+
+
+def _small_batch(n=5, master_seed=11):
+    cfgs = [workloads.batch_instance(master_seed, i, edge_range=(16, 40), photon_range=(5e3, 3e4)) for i in range(n)]
+    for c in cfgs:
+        c.update(nthread=NT, nblocksize=64, autopilot=0, quiet=True)
+    return cfgs
+
+
+def _plain(cfg, tally):
+    old = mcx_amd.set_tally_mode(tally)
+    try:
+        return mcx_amd.run(cfg)
+    finally:
+        mcx_amd.set_tally_mode(old)
+
+
+def _close(a, b, rtol, floor):
+    sig = np.abs(b) > floor * np.abs(b).max()
+    rel = np.abs(a[sig] - b[sig]) / np.abs(b[sig])
+    assert rel.max() < rtol, rel.max()
+    assert np.all(np.abs(a[~sig]) <= floor * np.abs(b).max() * 1.01 + 1e-30)
+
+
+@pytest.mark.parametrize("tally", [1, 2])
+def test_batch_reproduces_single_runs(tally):
+    cfgs = _small_batch()
+    res, info = mcx_amd.run_batch(cfgs, tally=tally, block_size=64)
+    assert info["lanes"] == NT * len(cfgs)
+    for cfg, r in zip(cfgs, res):
+        p = _plain(cfg, tally)
+        assert r["stat"]["energytot"] == pytest.approx(p["stat"]["energytot"], rel=1e-12)
+        assert r["stat"]["energyabs"] == pytest.approx(p["stat"]["energyabs"], rel=1e-9)
+        assert r["stat"]["energytot"] == cfg["nphoton"]
+        assert r["flux"].shape == p["flux"].shape
+        _close(r["flux"].astype(np.float64), p["flux"].astype(np.float64), 2e-6, 1e-4)
+
+
+def test_batch_hilo_against_fp64_oracle():
+    cfgs = _small_batch(n=3, master_seed=5)
+    res, _ = mcx_amd.run_batch(cfgs, tally=1, block_size=256)
+    for cfg, r in zip(cfgs, res):
+        ora = oracle_run(cfg, NT)
+        assert ora["errors"] == 0
+        tot = seq_sum(ora["genergy"][1::2])
+        esc = seq_sum(ora["genergy"][0::2])
+        assert r["stat"]["energytot"] == pytest.approx(tot, rel=1e-12)
+        assert r["stat"]["energyabs"] == pytest.approx(tot - esc, rel=1e-9)
+        ref, _ = normalize_like_engine(ora["field"], r["stat"]["energytot"], cfg["tstep"])
+        _close(r["flux"].astype(np.float64), ref, 2e-6, 1e-4)
+
+
+def test_batch_graph_replays_match_eager():
+    cfgs = _small_batch(n=4, master_seed=3)
+    eager, _ = mcx_amd.run_batch(cfgs, tally=1, block_size=128)
+    graph, info = mcx_amd.run_batch(cfgs, tally=1, block_size=128, use_graph=True, repeat=3)
+    assert info["total_ms"] > 0
+    for a, b in zip(eager, graph):
+        assert a["stat"]["energytot"] == b["stat"]["energytot"]
+        assert a["stat"]["energyabs"] == b["stat"]["energyabs"]
+        _close(b["flux"].astype(np.float64), a["flux"].astype(np.float64), 2e-6, 1e-4)
+
+
+def test_batch_autotuned_shape_and_energy():
+    cfgs = [workloads.batch_instance(7, i, edge_range=(20, 48), photon_range=(1e4, 1e5)) for i in range(12)]
+    for c in cfgs:
+        c["quiet"] = True
+    res, info = mcx_amd.run_batch(cfgs, tally=1, tune_fraction=0.25)
+    assert info["photons_per_lane"] in (16, 32, 64, 128)
+    assert info["block_size"] in (64, 256)
+    assert info["photons"] == sum(c["nphoton"] for c in cfgs)
+    for cfg, r in zip(cfgs, res):
+        assert r["stat"]["energytot"] == cfg["nphoton"]
+        a = r["stat"]["energyabs"] / r["stat"]["energytot"]
+        assert 0.0 < a < 1.0
+        assert np.all(np.isfinite(r["flux"])) and r["flux"].min() >= 0.0
