@@ -115,6 +115,9 @@ ABI_SYMBOLS = [
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
+    "mcx_dn_conv3_fwd", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
+    "mcx_dn_maxpool_fwd", "mcx_dn_maxpool_bwd", "mcx_dn_upsample_fwd", "mcx_dn_upsample_bwd", "mcx_dn_add",
+    "mcx_dn_head_fwd", "mcx_dn_head_bwd", "mcx_dn_adam", "mcx_dn_f32_to_bf16",
 ]
 
 _lib = None
@@ -153,6 +156,27 @@ def _setup(L):
     L.mcx_amd_abi_layout.restype = ctypes.c_int
     L.mcx_run_batch.argtypes = [P(P(Config)), ctypes.c_int, P(GPUInfo), P(McxBatchOptions)]
     L.mcx_run_batch.restype = ctypes.c_int
+    V, I, LL, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+    dn = {
+        "mcx_dn_conv3_fwd": [V, I, V, I, V, I, V, V, I, I, I, I, I, I, I, I, V],
+        "mcx_dn_weight_flip": [V, I, V, I, I, I, V],
+        "mcx_dn_conv3_wgrad": [V, I, V, I, V, I, V, I, I, I, I, I, I, I, V],
+        "mcx_dn_wgrad_reduce": [V, I, V, I, I, I, V],
+        "mcx_dn_bias_grad": [V, I, V, I, V, I, V, LL, I, V],
+        "mcx_dn_maxpool_fwd": [V, I, V, I, V, I, I, I, I, I, V],
+        "mcx_dn_maxpool_bwd": [V, I, V, V, I, I, I, I, I, I, V],
+        "mcx_dn_upsample_fwd": [V, I, V, I, I, I, I, I, I, V],
+        "mcx_dn_upsample_bwd": [V, I, V, I, I, I, I, I, I, V],
+        "mcx_dn_add": [V, I, V, I, V, I, LL, I, V],
+        "mcx_dn_head_fwd": [V, I, V, V, V, I, V, V, V, V, I, LL, I, V],
+        "mcx_dn_head_bwd": [V, I, V, V, V, I, V, I, V, LL, I, V],
+        "mcx_dn_adam": [V, V, V, V, V, LL, F, F, F, F, I, V],
+        "mcx_dn_f32_to_bf16": [V, V, LL, V],
+    }
+    for name, args in dn.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
 
 
 class McxBatchOptions(ctypes.Structure):

@@ -1,0 +1,646 @@
+/*
+ * dn_kernels.hip -- kernels of the learned fluence denoiser (3D U-Net),
+ * forward and backward, for gfx950.
+ *
+ * Tensors are BF16 in NDHWC order (channels innermost) with an explicit
+ * channel stride, so a tensor may be a channel slice of a wider buffer (the
+ * U-Net's skip concatenations are written in place).  All sums accumulate in
+ * FP32; master weights, gradients of weights and the optimiser state are
+ * FP32.
+ *
+ * 3x3x3 convolutions (stride 1, zero padding 1) are implicit GEMMs on the
+ * matrix cores with v_mfma_f32_16x16x32_bf16:
+ *  - forward / input gradient: M = output voxels, N = output channels,
+ *    K = 27 taps x input channels (tap-major).  A wave owns 16 voxels and
+ *    every output channel; a lane's A fragment is 8 consecutive channels of
+ *    one tap of one voxel (one 16-byte load), its B fragments 8 consecutive
+ *    K entries of one output channel's weight row.  The input gradient is
+ *    the same kernel over the output gradient with the weights transposed
+ *    and the taps mirrored; the ReLU derivative is applied as a mask on the
+ *    A operand.
+ *  - weight gradient: per tap, M = output channels, N = input channels,
+ *    K = voxels.  A workgroup stages 32 voxels of the output gradient and of
+ *    the tap-shifted input in LDS, transposed so that every fragment is one
+ *    16-byte LDS read, and accumulates its voxel range; partial sums per
+ *    range are reduced by a second kernel (no atomics).
+ *
+ * Operand layout of the 16x16x32 BF16 MFMA (wave64): lane l supplies
+ * A[row = l%16][k = 8*(l/16) .. +7] and B[k = 8*(l/16) .. +7][col = l%16];
+ * it receives D[row = 4*(l/16) + i][col = l%16], i = 0..3.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dnk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+/* FP32 -> BF16, round to nearest even (NaN kept quiet) */
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+union Frag {
+    uint4 u;
+    uint16_t h[8];
+    bf16x8 v;
+};
+
+__device__ __forceinline__ Frag load8(const uint16_t* p) {
+    Frag f;
+    f.u = *(const uint4*)p;
+    return f;
+}
+
+__device__ __forceinline__ Frag zero8() {
+    Frag f;
+    f.u = uint4{0u, 0u, 0u, 0u};
+    return f;
+}
+
+/* dz, dy, dx of tap t = 9*(dz+1) + 3*(dy+1) + (dx+1) */
+__device__ __forceinline__ void tap_offsets(int t, int& dz, int& dy, int& dx) {
+    dz = t / 9 - 1;
+    dy = (t / 3) % 3 - 1;
+    dx = t % 3 - 1;
+}
+
+/* ---- 3x3x3 convolution, forward (and input gradient) --------------------
+ * y[m][co] = act(bias[co] + sum_{t,ci} x[m + off_t][ci] * w[co][t*cin + ci])
+ * where x is multiplied by (mask > 0) channel-wise when mask is given.
+ * w rows are kpad long (27*cin rounded up to 32, zero padded). */
+template <int NT>
+__global__ void __launch_bounds__(256) conv3_fwd(const uint16_t* __restrict__ x, int xcs, const uint16_t* __restrict__ mask,
+                                                 int mcs, const uint16_t* __restrict__ w, int kpad,
+                                                 const float* __restrict__ bias, uint16_t* __restrict__ y, int ycs, int nb,
+                                                 int D, int H, int W, int cin, int cout, int relu) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const long long mtot = (long long)nb * D * H * W;
+    const long long m = (long long)blockIdx.x * 64 + wave * 16 + r;
+    int vx = 0, vy = 0, vz = 0;
+    long long vn = 0;
+    const bool mvalid = m < mtot;
+    if (mvalid) {
+        long long q = m;
+        vx = (int)(q % W); q /= W;
+        vy = (int)(q % H); q /= H;
+        vz = (int)(q % D); vn = q / D;
+    }
+    f32x4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int ktot = 27 * cin;
+    const int nch = kpad / 32;
+    for (int kc = 0; kc < nch; kc++) {
+        const int k0 = kc * 32 + 8 * g;
+        Frag a = zero8();
+        if (mvalid && k0 < ktot) {
+            const int t = k0 / cin, ci0 = k0 - t * cin;
+            int dz, dy, dx;
+            tap_offsets(t, dz, dy, dx);
+            const int zz = vz + dz, yy = vy + dy, xx = vx + dx;
+            if ((unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+                const long long v = ((vn * D + zz) * H + yy) * W + xx;
+                a = load8(x + v * xcs + ci0);
+                if (mask) {
+                    const Frag mk = load8(mask + v * mcs + ci0);
+#pragma unroll
+                    for (int i = 0; i < 8; i++)
+                        if (!(bf2f(mk.h[i]) > 0.f)) a.h[i] = 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            const Frag b = load8(w + (long long)(j * 16 + r) * kpad + k0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j], 0, 0, 0);
+        }
+    }
+    const long long mbase = (long long)blockIdx.x * 64 + wave * 16;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int co = j * 16 + r;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const long long mr = mbase + 4 * g + i;
+            if (mr < mtot && co < cout) {
+                float v = acc[j][i] + bv;
+                if (relu) v = v > 0.f ? v : 0.f;
+                y[mr * ycs + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+/* wt[ci][t'*cout + co] = w[co][t*cin + ci], t' = 26 - t (mirrored taps), rows
+ * of wt padded with zeros to kpadt */
+__global__ void weight_flip(const uint16_t* __restrict__ w, int kpad, uint16_t* __restrict__ wt, int kpadt, int cin, int cout) {
+    const long long n = (long long)cin * kpadt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ci = (int)(i / kpadt), k = (int)(i % kpadt);
+        uint16_t v = 0;
+        if (k < 27 * cout) {
+            const int tp = k / cout, co = k - tp * cout;
+            v = w[(long long)co * kpad + (26 - tp) * cin + ci];
+        }
+        wt[i] = v;
+    }
+}
+
+/* ---- weight gradient ----------------------------------------------------
+ * partial[s][t][co][ci] = sum over voxels v of split s of
+ *   dy[v][co] * (mask[v][co] > 0) * x[v + off_t][ci]
+ * grid = (27, nsplit), 256 threads; TPW output tiles (16x16) per wave. */
+template <int TPW>
+__global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ x, int xcs, const uint16_t* __restrict__ dy,
+                                                   int dycs, const uint16_t* __restrict__ mask, int mcs,
+                                                   float* __restrict__ partial, int vps, int nb, int D, int H, int W, int cin,
+                                                   int cout) {
+    extern __shared__ uint4 smraw[];
+    uint16_t* sm = (uint16_t*)smraw;
+    const int cinp = (cin + 15) & ~15;
+    uint16_t* sdy = sm;               /* [cout][32] */
+    uint16_t* sx = sm + cout * 32;    /* [cinp][32] */
+    const int t = blockIdx.x, s = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    int dz, dyo, dxo;
+    tap_offsets(t, dz, dyo, dxo);
+    const long long mtot = (long long)nb * D * H * W;
+    const long long v0 = (long long)s * vps;
+    const long long v1 = (v0 + vps < mtot) ? v0 + vps : mtot;
+    const int tco = cout / 16, tci = cinp / 16, ntile = tco * tci;
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long long vb = v0; vb < v1; vb += 32) {
+        /* stage dy (masked) transposed: sdy[co][vi] */
+        for (int e = threadIdx.x; e < 32 * (cout / 8); e += 256) {
+            const int vi = e / (cout / 8), c8 = (e % (cout / 8)) * 8;
+            const long long v = vb + vi;
+            Frag f = zero8();
+            if (v < v1) {
+                f = load8(dy + v * dycs + c8);
+                if (mask) {
+                    const Frag mk = load8(mask + v * mcs + c8);
+#pragma unroll
+                    for (int i = 0; i < 8; i++)
+                        if (!(bf2f(mk.h[i]) > 0.f)) f.h[i] = 0;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) sdy[(c8 + i) * 32 + vi] = f.h[i];
+        }
+        /* stage tap-shifted x transposed: sx[ci][vi], zero outside */
+        for (int e = threadIdx.x; e < 32 * (cinp / 8); e += 256) {
+            const int vi = e / (cinp / 8), c8 = (e % (cinp / 8)) * 8;
+            const long long v = vb + vi;
+            Frag f = zero8();
+            if (v < v1 && c8 < cin) {
+                long long q = v;
+                const int vx = (int)(q % W); q /= W;
+                const int vy = (int)(q % H); q /= H;
+                const int vz = (int)(q % D);
+                const long long vn = q / D;
+                const int zz = vz + dz, yy = vy + dyo, xx = vx + dxo;
+                if ((unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W)
+                    f = load8(x + (((vn * D + zz) * H + yy) * W + xx) * xcs + c8);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) sx[(c8 + i) * 32 + vi] = f.h[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < TPW; j++) {
+            const int tile = wave + 4 * j;
+            if (tile < ntile) {
+                const int ico = tile / tci, ici = tile % tci;
+                Frag a, b;
+                a.u = *(const uint4*)(sdy + (ico * 16 + r) * 32 + 8 * g);
+                b.u = *(const uint4*)(sx + (ici * 16 + r) * 32 + 8 * g);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = partial + ((long long)s * 27 + t) * cout * cin;
+#pragma unroll
+    for (int j = 0; j < TPW; j++) {
+        const int tile = wave + 4 * j;
+        if (tile < ntile) {
+            const int ico = tile / tci, ici = tile % tci;
+            const int ci = ici * 16 + r;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int co = ico * 16 + 4 * g + i;
+                if (ci < cin) out[(long long)co * cin + ci] = acc[j][i];
+            }
+        }
+    }
+}
+
+/* dw[co][t*cin + ci] = sum_s partial[s][t][co][ci]; padding entries of dw
+ * (k >= 27*cin) are set to zero */
+__global__ void wgrad_reduce(const float* __restrict__ partial, int nsplit, float* __restrict__ dw, int kpad, int cin, int cout) {
+    const long long n = (long long)cout * kpad;
+    const long long per = 27LL * cout * cin;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int co = (int)(i / kpad), k = (int)(i % kpad);
+        float sum = 0.f;
+        if (k < 27 * cin) {
+            const int t = k / cin, ci = k - t * cin;
+            const long long off = ((long long)t * cout + co) * cin + ci;
+            for (int s = 0; s < nsplit; s++) sum += partial[s * per + off];
+        }
+        dw[i] = sum;
+    }
+}
+
+/* db[co] partial sums over voxel ranges: part[s][co] */
+__global__ void bias_grad(const uint16_t* __restrict__ dy, int dycs, const uint16_t* __restrict__ mask, int mcs,
+                          float* __restrict__ part, long long nvox, int vps, int cout) {
+    __shared__ float red[256];
+    const int s = blockIdx.x;
+    const long long v0 = (long long)s * vps;
+    const long long v1 = (v0 + vps < nvox) ? v0 + vps : nvox;
+    /* threads: cout channels x (256 / cout) voxel lanes */
+    const int lanes = 256 / cout;
+    const int co = threadIdx.x % cout, vl = threadIdx.x / cout;
+    float sum = 0.f;
+    if (vl < lanes) {
+        for (long long v = v0 + vl; v < v1; v += lanes) {
+            float d = bf2f(dy[v * dycs + co]);
+            if (mask && !(bf2f(mask[v * mcs + co]) > 0.f)) d = 0.f;
+            sum += d;
+        }
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int k = 0; k < lanes; k++) t += red[k * cout + threadIdx.x];
+        part[(long long)s * cout + threadIdx.x] = t;
+    }
+}
+
+__global__ void sum_rows(const float* __restrict__ part, int nrow, int ncol, float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    float t = 0.f;
+    for (int s = 0; s < nrow; s++) t += part[(long long)s * ncol + c];
+    out[c] = t;
+}
+
+/* ---- 2x2x2 max pooling --------------------------------------------------- */
+__global__ void maxpool_fwd(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs,
+                            uint8_t* __restrict__ arg, int nb, int D, int H, int W, int c) {
+    const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
+    const long long n = (long long)nb * d2 * h2 * w2 * c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        long long q = i / c;
+        const int ox = (int)(q % w2); q /= w2;
+        const int oy = (int)(q % h2); q /= h2;
+        const int oz = (int)(q % d2);
+        const long long on = q / d2;
+        float best = -3.0e38f;
+        int bi = 0;
+        for (int k = 0; k < 8; k++) {
+            const int zz = 2 * oz + (k >> 2), yy = 2 * oy + ((k >> 1) & 1), xx = 2 * ox + (k & 1);
+            const float v = bf2f(x[(((on * D + zz) * H + yy) * W + xx) * xcs + ch]);
+            if (v > best) { best = v; bi = k; }
+        }
+        y[(i / c) * ycs + ch] = f2bf(best);
+        arg[i] = (uint8_t)bi;
+    }
+}
+
+/* dx = dy routed to the window's maximum, zero elsewhere (every input
+ * voxel is written) */
+__global__ void maxpool_bwd(const uint16_t* __restrict__ dy, int dycs, const uint8_t* __restrict__ arg,
+                            uint16_t* __restrict__ dx, int dxcs, int nb, int D, int H, int W, int c) {
+    const int d2 = D / 2, h2 = H / 2, w2 = W / 2;
+    const long long n = (long long)nb * D * H * W * c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        long long q = i / c;
+        const int xx = (int)(q % W); q /= W;
+        const int yy = (int)(q % H); q /= H;
+        const int zz = (int)(q % D);
+        const long long nn = q / D;
+        const long long o = ((nn * d2 + zz / 2) * h2 + yy / 2) * w2 + xx / 2;
+        const int k = ((zz & 1) << 2) | ((yy & 1) << 1) | (xx & 1);
+        const uint16_t v = (arg[o * c + ch] == k) ? dy[o * dycs + ch] : (uint16_t)0;
+        dx[(i / c) * dxcs + ch] = v;
+    }
+}
+
+/* ---- nearest-neighbour x2 upsampling -------------------------------------
+ * forward: y (2D x 2H x 2W) from x (D x H x W); backward: dx = sum of the
+ * 8 children's gradients */
+__global__ void upsample_fwd(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs, int nb, int D,
+                             int H, int W, int c) {
+    const int D2 = 2 * D, H2 = 2 * H, W2 = 2 * W;
+    const long long n = (long long)nb * D2 * H2 * W2 * c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        long long q = i / c;
+        const int xx = (int)(q % W2); q /= W2;
+        const int yy = (int)(q % H2); q /= H2;
+        const int zz = (int)(q % D2);
+        const long long nn = q / D2;
+        y[(i / c) * ycs + ch] = x[(((nn * D + zz / 2) * H + yy / 2) * W + xx / 2) * xcs + ch];
+    }
+}
+
+__global__ void upsample_bwd(const uint16_t* __restrict__ dy, int dycs, uint16_t* __restrict__ dx, int dxcs, int nb, int D,
+                             int H, int W, int c) {
+    const long long n = (long long)nb * D * H * W * c;
+    const int H2 = 2 * H, W2 = 2 * W;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c);
+        long long q = i / c;
+        const int xx = (int)(q % W); q /= W;
+        const int yy = (int)(q % H); q /= H;
+        const int zz = (int)(q % D);
+        const long long nn = q / D;
+        float s = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const int z2 = 2 * zz + (k >> 2), y2 = 2 * yy + ((k >> 1) & 1), x2 = 2 * xx + (k & 1);
+            s += bf2f(dy[(((nn * 2 * D + z2) * H2 + y2) * W2 + x2) * dycs + ch]);
+        }
+        dx[(i / c) * dxcs + ch] = f2bf(s);
+    }
+}
+
+/* out = a + b (BF16, FP32 add) */
+__global__ void add_bf16(const uint16_t* __restrict__ a, int acs, const uint16_t* __restrict__ b, int bcs,
+                         uint16_t* __restrict__ out, int ocs, long long nvox, int c) {
+    const long long n = nvox * c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long v = i / c;
+        const int ch = (int)(i % c);
+        out[v * ocs + ch] = f2bf(bf2f(a[v * acs + ch]) + bf2f(b[v * bcs + ch]));
+    }
+}
+
+/* ---- output head: 1x1x1 conv to one channel + residual + MSE -------------
+ * pred[v] = b + sum_c f[v][c] w[c] + res[v*rcs]; with a target, also
+ * dpred[v] = 2 (pred - target) / nvox and per-block partial sums of the
+ * squared error. */
+__global__ void head_fwd(const uint16_t* __restrict__ f, int fcs, const float* __restrict__ w, const float* __restrict__ b,
+                         const uint16_t* __restrict__ res, int rcs, const float* __restrict__ target, float* __restrict__ pred,
+                         float* __restrict__ dpred, float* __restrict__ lpart, long long nvox, int c) {
+    __shared__ float red[256];
+    float l = 0.f;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long long)gridDim.x * blockDim.x) {
+        float s = b[0];
+        for (int k = 0; k < c; k++) s += bf2f(f[v * fcs + k]) * w[k];
+        s += bf2f(res[v * rcs]);
+        pred[v] = s;
+        if (target) {
+            const float e = s - target[v];
+            dpred[v] = 2.f * e / (float)nvox;
+            l += e * e;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && lpart) lpart[blockIdx.x] = red[0];
+}
+
+/* df[v][c] = dpred[v] * w[c] (BF16); per-block partial sums of
+ * dpred[v] * f[v][c] (dw) and dpred[v] (db): part[blk][c], part[blk][c] */
+__global__ void head_bwd(const uint16_t* __restrict__ f, int fcs, const float* __restrict__ w, const float* __restrict__ dpred,
+                         uint16_t* __restrict__ df, int dfcs, float* __restrict__ part, long long nvox, int c) {
+    __shared__ float red[256][17];
+    float acc[17];
+    for (int k = 0; k <= c; k++) acc[k] = 0.f;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long long)gridDim.x * blockDim.x) {
+        const float d = dpred[v];
+        for (int k = 0; k < c; k++) {
+            acc[k] += d * bf2f(f[v * fcs + k]);
+            df[v * dfcs + k] = f2bf(d * w[k]);
+        }
+        acc[c] += d;
+    }
+    for (int k = 0; k <= c; k++) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x <= c) {
+        float t = 0.f;
+        for (int i = 0; i < 256; i++) t += red[i][threadIdx.x];
+        part[(long long)blockIdx.x * (c + 1) + threadIdx.x] = t;
+    }
+}
+
+/* Adam (Kingma & Ba) on FP32 master weights; optional BF16 copy */
+__global__ void adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+                     uint16_t* __restrict__ pbf, long long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        if (pbf) pbf[i] = f2bf(pi);
+    }
+}
+
+__global__ void f32_to_bf16(const float* __restrict__ a, uint16_t* __restrict__ b, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = f2bf(a[i]);
+}
+
+}  // namespace dnk
+
+/* ---- C ABI ---------------------------------------------------------------
+ * Every entry point validates its shapes and alignment on the host and
+ * returns 0, or a negative code (-22 bad argument, otherwise -hipError). */
+using namespace dnk;
+
+static int dn_grid(long long n, int cap = 65536) {
+    long long g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+static int dn_ret() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : -(int)e;
+}
+
+static bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mcs, const void* w, int kpad, const float* bias,
+                                void* y, int ycs, int nb, int d, int h, int wd, int cin, int cout, int relu, void* stream) {
+    if (!x || !w || !y || nb <= 0 || d <= 0 || h <= 0 || wd <= 0) return -22;
+    if (cin <= 0 || cin % 8 || cout <= 0 || cout % 16 || xcs < cin || xcs % 8 || ycs < cout) return -22;
+    if (kpad < 27 * cin || kpad % 32 || !al16(x) || !al16(w) || (mask && (!al16(mask) || mcs % 8 || mcs < cin))) return -22;
+    const long long m = (long long)nb * d * h * wd;
+    const unsigned int grid = (unsigned int)((m + 63) / 64);
+    hipStream_t st = (hipStream_t)stream;
+    const uint16_t *xp = (const uint16_t*)x, *mp = (const uint16_t*)mask, *wp = (const uint16_t*)w;
+    uint16_t* yp = (uint16_t*)y;
+#define DN_FWD(NT) hipLaunchKernelGGL(conv3_fwd<NT>, dim3(grid), dim3(256), 0, st, xp, xcs, mp, mcs, wp, kpad, bias, yp, ycs, nb, d, h, wd, cin, cout, relu)
+    switch (cout / 16) {
+        case 1: DN_FWD(1); break;
+        case 2: DN_FWD(2); break;
+        case 3: DN_FWD(3); break;
+        case 4: DN_FWD(4); break;
+        case 6: DN_FWD(6); break;
+        case 8: DN_FWD(8); break;
+        case 12: DN_FWD(12); break;
+        default: return -22;
+    }
+#undef DN_FWD
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_weight_flip(const void* w, int kpad, void* wt, int kpadt, int cin, int cout, void* stream) {
+    if (!w || !wt || kpad < 27 * cin || kpadt < 27 * cout || cin <= 0 || cout <= 0) return -22;
+    const long long n = (long long)cin * kpadt;
+    hipLaunchKernelGGL(weight_flip, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)w, kpad, (uint16_t*)wt,
+                       kpadt, cin, cout);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dycs, const void* mask, int mcs, float* partial,
+                                  int nsplit, int nb, int d, int h, int wd, int cin, int cout, void* stream) {
+    if (!x || !dy || !partial || nsplit <= 0 || nsplit > 65535 || nb <= 0 || d <= 0 || h <= 0 || wd <= 0) return -22;
+    if (cin <= 0 || cin % 8 || cout <= 0 || cout % 16 || xcs % 8 || dycs % 8 || xcs < cin || dycs < cout) return -22;
+    if (!al16(x) || !al16(dy) || (mask && (!al16(mask) || mcs % 8 || mcs < cout))) return -22;
+    const long long m = (long long)nb * d * h * wd;
+    long long vps = (m + nsplit - 1) / nsplit;
+    vps = (vps + 31) / 32 * 32;
+    const int cinp = (cin + 15) & ~15;
+    const int ntile = (cout / 16) * (cinp / 16);
+    const int tpw = (ntile + 3) / 4;
+    const size_t shmem = (size_t)(cout + cinp) * 32 * 2;
+    if (shmem > 65536) return -22;
+    hipStream_t st = (hipStream_t)stream;
+    const uint16_t *xp = (const uint16_t*)x, *dp = (const uint16_t*)dy, *mp = (const uint16_t*)mask;
+    const dim3 grid(27, (unsigned int)nsplit);
+#define DN_WG(T) hipLaunchKernelGGL(conv3_wgrad<T>, grid, dim3(256), shmem, st, xp, xcs, dp, dycs, mp, mcs, partial, (int)vps, nb, d, h, wd, cin, cout)
+    if (tpw <= 1) DN_WG(1);
+    else if (tpw <= 2) DN_WG(2);
+    else if (tpw <= 4) DN_WG(4);
+    else if (tpw <= 8) DN_WG(8);
+    else if (tpw <= 16) DN_WG(16);
+    else if (tpw <= 24) DN_WG(24);
+    else return -22;
+#undef DN_WG
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_wgrad_reduce(const float* partial, int nsplit, float* dw, int kpad, int cin, int cout, void* stream) {
+    if (!partial || !dw || nsplit <= 0 || kpad < 27 * cin || cin <= 0 || cout <= 0) return -22;
+    hipLaunchKernelGGL(wgrad_reduce, dim3(dn_grid((long long)cout * kpad)), dim3(256), 0, (hipStream_t)stream, partial, nsplit, dw,
+                       kpad, cin, cout);
+    return dn_ret();
+}
+
+/* part must hold nsplit*cout floats; db receives cout sums */
+extern "C" int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int mcs, float* part, int nsplit, float* db,
+                                long long nvox, int cout, void* stream) {
+    if (!dy || !part || !db || nsplit <= 0 || nsplit > 65535 || cout <= 0 || cout > 256 || 256 % cout || nvox <= 0) return -22;
+    const int vps = (int)((nvox + nsplit - 1) / nsplit);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(bias_grad, dim3(nsplit), dim3(256), 0, st, (const uint16_t*)dy, dycs, (const uint16_t*)mask, mcs, part, nvox,
+                       vps, cout);
+    hipLaunchKernelGGL(sum_rows, dim3((cout + 255) / 256), dim3(256), 0, st, (const float*)part, nsplit, cout, db);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_maxpool_fwd(const void* x, int xcs, void* y, int ycs, unsigned char* arg, int nb, int d, int h, int wd,
+                                  int c, void* stream) {
+    if (!x || !y || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || xcs < c || ycs < c) return -22;
+    const long long n = (long long)nb * (d / 2) * (h / 2) * (wd / 2) * c;
+    hipLaunchKernelGGL(maxpool_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
+                       ycs, (uint8_t*)arg, nb, d, h, wd, c);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char* arg, void* dx, int dxcs, int nb, int d, int h,
+                                  int wd, int c, void* stream) {
+    if (!dy || !dx || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || dycs < c || dxcs < c) return -22;
+    const long long n = (long long)nb * d * h * wd * c;
+    hipLaunchKernelGGL(maxpool_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                       (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    return dn_ret();
+}
+
+/* d, h, wd: the low-resolution dimensions */
+extern "C" int mcx_dn_upsample_fwd(const void* x, int xcs, void* y, int ycs, int nb, int d, int h, int wd, int c, void* stream) {
+    if (!x || !y || c <= 0 || xcs < c || ycs < c) return -22;
+    const long long n = (long long)nb * 8 * d * h * wd * c;
+    hipLaunchKernelGGL(upsample_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
+                       ycs, nb, d, h, wd, c);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs, int nb, int d, int h, int wd, int c, void* stream) {
+    if (!dy || !dx || c <= 0 || dycs < c || dxcs < c) return -22;
+    const long long n = (long long)nb * d * h * wd * c;
+    hipLaunchKernelGGL(upsample_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                       (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* out, int ocs, long long nvox, int c, void* stream) {
+    if (!a || !b || !out || c <= 0 || nvox <= 0) return -22;
+    hipLaunchKernelGGL(add_bf16, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
+                       (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
+    return dn_ret();
+}
+
+/* lpart: gridsize floats (gridsize returned through *nblocks) */
+extern "C" int mcx_dn_head_fwd(const void* f, int fcs, const float* w, const float* b, const void* res, int rcs,
+                               const float* target, float* pred, float* dpred, float* lpart, int nblocks, long long nvox, int c,
+                               void* stream) {
+    if (!f || !w || !b || !res || !pred || c <= 0 || c > 16 || nvox <= 0 || nblocks <= 0) return -22;
+    if (target && (!dpred || !lpart)) return -22;
+    hipLaunchKernelGGL(head_fwd, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)f, fcs, w, b,
+                       (const uint16_t*)res, rcs, target, pred, dpred, lpart, nvox, c);
+    return dn_ret();
+}
+
+/* part: nblocks*(c+1) floats; dwb receives c weight sums then the bias sum */
+extern "C" int mcx_dn_head_bwd(const void* f, int fcs, const float* w, const float* dpred, void* df, int dfcs, float* part,
+                               int nblocks, float* dwb, long long nvox, int c, void* stream) {
+    if (!f || !w || !dpred || !df || !part || !dwb || c <= 0 || c > 16 || nblocks <= 0 || nvox <= 0) return -22;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(head_bwd, dim3(nblocks), dim3(256), 0, st, (const uint16_t*)f, fcs, w, dpred, (uint16_t*)df, dfcs, part,
+                       nvox, c);
+    hipLaunchKernelGGL(sum_rows, dim3(1), dim3(256), 0, st, (const float*)part, nblocks, c + 1, dwb);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_adam(float* p, float* m, float* v, const float* g, void* pbf16, long long n, float lr, float beta1,
+                           float beta2, float eps, int step, void* stream) {
+    if (!p || !m || !v || !g || n <= 0 || step <= 0) return -22;
+    const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(adam, dim3(dn_grid(n, 4096)), dim3(256), 0, (hipStream_t)stream, p, m, v, g, (uint16_t*)pbf16, n, lr, beta1,
+                       beta2, eps, c1, c2);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_f32_to_bf16(const float* a, void* b, long long n, void* stream) {
+    if (!a || !b || n <= 0) return -22;
+    hipLaunchKernelGGL(f32_to_bf16, dim3(dn_grid(n, 4096)), dim3(256), 0, (hipStream_t)stream, a, (uint16_t*)b, n);
+    return dn_ret();
+}

@@ -24,7 +24,7 @@ L = _lib.lib()
 
 def _declared_symbols():
     names = set()
-    for h in ("include/mcx_abi.h", "include/mcx_amd_ext.h"):
+    for h in ("include/mcx_abi.h", "include/mcx_amd_ext.h", "include/mcx_amd_denoise.h"):
         text = open(os.path.join(ROOT, h)).read()
         names |= set(re.findall(r"^\s*(?:void|int|float|double)\s+\**(mcx_\w+)\s*\(", text, re.M))
     return names
