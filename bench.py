@@ -24,6 +24,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per photon of the transport kernel on the sphere100 workload with
+# the FP64 tally: (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000) per launch / 1e7
+# photons, scale factors from the FP64->FP32 merge kernel as a copy of known
+# size (profiles/transport_sphere100_1e7_counters.md)
+TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 9201.0}
+ATOMIC_SIZES_MB = (4, 64, 400, 2048)
+
+
+def _atomic_roof(mode, nelem):
+    """Scattered-atomic rate (atomics/s) over an array of nelem elements."""
+    from mcx_amd import _lib
+    r = _lib.atomic_bench(mode, int(nelem), 256, 4096)
+    if abs(r["sum_ratio"] - 1.0) > 1e-9:
+        raise RuntimeError("atomic microbenchmark lost updates: %r" % r)
+    return r["atomics_per_s"]
+
+
+def atomic_roof_table():
+    """Microbenchmark of the device's scattered-atomic rate: every one of
+    4096 x 256 lanes adds 1 at 256 uniformly random indices.  Modes: 0 FP64
+    agent scope no return, 1 FP32 no return, 2 FP32 returning, 3 FP64
+    workgroup scope into a private copy per XCD."""
+    from mcx_amd import _lib
+    for mode in (0, 1, 2, 3):
+        for mb in ATOMIC_SIZES_MB:
+            es = 8 if mode in (0, 3) else 4
+            r = _lib.atomic_bench(mode, (mb << 20) // es, 256, 4096)
+            r.update(mode=mode, mbytes=mb)
+            print(json.dumps(r), flush=True)
 
 
 def _cpu_baseline(cfg_builder, budget_s):
@@ -50,21 +79,95 @@ def _cpu_baseline(cfg_builder, budget_s):
     }
 
 
+def bench_batch(args):
+    """The 256-instance heterogeneous batch in one persistent launch (FP32
+    hi/lo tally), replayed from a captured hipGraph.  value = photons over
+    the graph execution time (clear, transport, energy sums, merge and
+    normalisation), inputs resident in HBM."""
+    import mcx_amd
+    from mcx_amd import workloads
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    cfgs = workloads.batch256()
+    for c in cfgs:
+        c["nphoton"] = max(1000, int(c["nphoton"] * args.batch_scale))
+        c["quiet"] = True
+    tally = 1 if args.tally == 2 else args.tally
+    mcx_amd.set_counters(True)
+    res, info_w = mcx_amd.run_batch(cfgs, tally=tally, tune_fraction=1.0 / 32, download=False)
+    mcx_amd.set_counters(False)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, info = mcx_amd.run_batch(cfgs, tally=tally, photons_per_lane=info_w["photons_per_lane"],
+                                    block_size=info_w["block_size"], use_graph=True, repeat=1, download=False)
+        times.append(((time.perf_counter() - t0) * 1e3, info["total_ms"]))
+    nph = info_w["photons"]
+    g_ms = sum(t[1] for t in times) / len(times)
+    ctr = info_w["counters"]
+    steps_pp, dep_pp = ctr["nstep"] / nph, ctr["ndeposit"] / nph
+    bytes_pp = steps_pp * 1.0 + dep_pp * 16.0
+    achieved = bytes_pp * nph / (g_ms * 1e-3) / 1e9
+    total_cells = sum(int(c["vol"].size) * 10 for c in cfgs)
+    aroof = _atomic_roof(2 if tally == 1 else 0, min(total_cells, 1 << 29))
+    arate = ctr["ndeposit"] / (g_ms * 1e-3)
+    line = {
+        "metric": "photons/ms at 1 GPU (batched)", "value": round(nph / g_ms, 1), "unit": "photons/ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(sum(t[0] for t in times) / len(times), 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch256: 256 heterogeneous instances (master seed 7, edges 40-160, 2-6 layers, "
+                               "0-3 ellipsoids, 10 gates x 0.5 ns), %.4g photons, %s tally, one persistent launch "
+                               "in a hipGraph" % (nph, ["FP32+shadow", "FP32 hi/lo", "FP64"][tally]),
+                   "graph_ms": round(g_ms, 3), "lanes": info_w["lanes"], "photons_per_lane": info_w["photons_per_lane"],
+                   "block_size": info_w["block_size"], "vgprs": info_w["vgprs"],
+                   "steps_per_photon": round(steps_pp, 2), "deposits_per_photon": round(dep_pp, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3),
+                                "unit": "G atomics/s", "frac": round(arate / aroof, 4)}},
+    }
+    if not args.no_cpu:
+        from parity_util import oracle_run
+        import oracle_py
+        oracle_py.lib()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        n, secs = 0, 0.0
+        for c in cfgs[:16]:
+            cc = dict(c)
+            cc["nphoton"] = min(c["nphoton"], 20000)
+            r = oracle_run(cc, 4096, ompthreads=threads)
+            n += cc["nphoton"]
+            secs += r["seconds"]
+            if secs > args.cpu_budget:
+                break
+        line["cpu_baseline"] = {"value": n / (secs * 1e3), "unit": "photons/ms", "cores": threads, "kind": "port",
+                                "sample": "%d photons over the first batch instances (<= 2e4 each), %.1f s" % (n, secs)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nphoton", type=float, default=1e8)
-    ap.add_argument("--workload", default="sphere100", choices=["sphere100", "cube60"])
+    ap.add_argument("--workload", default="sphere100", choices=["sphere100", "cube60", "batch256"])
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="photon scale of the batch256 workload")
     ap.add_argument("--tally", type=int, default=2, help="0 FP32+shadow, 1 FP32 hi/lo, 2 FP64, 3 FP64 per-XCD copies")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--atomic-roof", action="store_true", help="print the scattered-atomic microbenchmark and exit")
     args = ap.parse_args()
+    if args.atomic_roof:
+        atomic_roof_table()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch256":
+        bench_batch(args)
+        return
 
     import mcx_amd
     from mcx_amd import workloads
@@ -111,6 +214,14 @@ def main():
     bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
+    tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally))
+    # the tally's atomic instruction: FP64 no-return (modes 2, 3), FP32
+    # returning (1), FP32 no-return + rare returning shadow moves (0)
+    nvox = int(cfg["vol"].size)
+    ngates = int(round((cfg["tend"] - cfg["tstart"]) / cfg["tstep"]))
+    amode = {0: 1, 1: 2, 2: 0, 3: 0}[args.tally]
+    aroof = _atomic_roof(amode, nvox * ngates)
+    arate = dep_pp * nph / (k_ms * 1e-3)
     line = {
         "metric": "photons/ms at 1 GPU",
         "value": round(value, 1),
@@ -144,7 +255,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": round(tpp * nph) if tpp else None,
+            "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3), "unit": "G atomics/s",
+                       "frac": round(arate / aroof, 4),
+                       "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},
         },
     }
     if not args.no_cpu:

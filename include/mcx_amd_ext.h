@@ -70,6 +70,8 @@ typedef struct McxBatchOptions {
     float total_ms;          /* out: last execution, clear to merge */
     double photons;          /* out: photons simulated per execution */
     double lanes;            /* out: virtual threads over all instances */
+    double counters[4];      /* out, with mcx_amd_set_counters(1): steps, deposits, scatters, launches of the
+                                last execution; zeros otherwise */
 } McxBatchOptions;
 
 /* Runs `ncfg` independent simulations in one persistent kernel launch.

@@ -66,7 +66,8 @@ def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, 
             results.append(out)
         info = {"photons": opt.photons, "lanes": int(opt.lanes), "kernel_ms": float(opt.kernel_ms),
                 "total_ms": float(opt.total_ms), "photons_per_lane": int(opt.chosen_photons_per_lane),
-                "block_size": int(opt.chosen_block_size), "vgprs": int(opt.vgprs)}
+                "block_size": int(opt.chosen_block_size), "vgprs": int(opt.vgprs),
+                "counters": dict(zip(("nstep", "ndeposit", "nscatter", "nlaunch"), list(opt.counters)))}
     finally:
         if gpu:
             L.mcx_cleargpuinfo(ctypes.byref(gpu))

@@ -71,6 +71,7 @@ class BatchRun {
     void enqueue(hipStream_t s);
     float run(hipStream_t s, bool graph, int repeat, float* kernel_ms);
     void download(hipStream_t s, bool fields);
+    void read_counters(hipStream_t s, double* out);
 
     double photons = 0.0;
     unsigned long long lanes = 0;
@@ -87,7 +88,7 @@ class BatchRun {
     uint32_t total_chunks_ = 0, maxtab_ = 0;
     unsigned int grid_ = 0;
     DevMem dmedia_, dprop_, dseed_, dzero_, dout_, dkp_, dchunk_, dorder_, dbi_;
-    size_t queue_off_ = 0;
+    size_t queue_off_ = 0, counters_off_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
 };
@@ -175,6 +176,8 @@ void BatchRun::setup() {
     }
     queue_off_ = zero_b;
     zero_b += 256;
+    counters_off_ = zero_b;
+    zero_b += 256;
 
     /* work queue: instances by decreasing chunk cost */
     order_.resize(n_);
@@ -255,6 +258,7 @@ void BatchRun::setup() {
         kp.genergy = (float*)(zb + P.off_genergy);
         kp.seeds = (const uint32_t*)(dseed_.as<char>() + P.off_seed);
         kp.detcount = (uint32_t*)(zb + P.off_energy + 64); /* error-exit counter ([1]) */
+        kp.counters = counters_enabled() ? (uint64_t*)(zb + counters_off_) : nullptr;
         BatchInst& b = bis[i];
         memset(&b, 0, sizeof(b));
         b.out = (float*)(dout_.as<char>() + P.off_out);
@@ -357,6 +361,15 @@ void BatchRun::download(hipStream_t s, bool fields) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
+void BatchRun::read_counters(hipStream_t s, double* out) {
+    uint64_t c[4] = {0, 0, 0, 0};
+    if (counters_enabled()) {
+        HIP_CHECK(hipMemcpyAsync(c, dzero_.as<char>() + counters_off_, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    for (int i = 0; i < 4; i++) out[i] = (double)c[i];
+}
+
 /* Autotuner: measured photons/ms of the transport kernel over a reduced
  * copy of the batch (every instance, photon counts scaled down) for each
  * candidate shape; the fastest wins.  Results are cached per batch
@@ -432,6 +445,7 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     float kms = 0.f;
     const float tms = r.run(st.s, opt->use_graph != 0, opt->repeat, &kms);
     r.download(st.s, opt->download != 0);
+    r.read_counters(st.s, opt->counters);
     for (int i = 0; i < n; i++) cfgs[i]->runtime = (unsigned int)llround(tms);
     opt->kernel_ms = kms;
     opt->total_ms = tms;

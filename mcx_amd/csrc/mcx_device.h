@@ -67,6 +67,8 @@ int waves_per_simd(int vgprs);
 /* launch constants of one simulation: grid, source, physics, boundary and
  * detector fields; buffers, work split and time window are left zero */
 void fill_kparam(const Config* cfg, unsigned int maxgate, unsigned int is2d, KParam& kp);
+/* whether work counters are requested (mcx_amd_set_counters) */
+int counters_enabled();
 
 }  // namespace mcx
 

@@ -70,6 +70,8 @@ void check_supported(const Config* cfg) {
 
 /* waves per SIMD for a kernel using `vgprs` architectural VGPRs (gfx950:
  * 512 per lane per SIMD, granule 8, at most 8 waves) */
+int counters_enabled() { return g_counters; }
+
 int waves_per_simd(int vgprs) {
     if (vgprs <= 0) return 8;
     int g = (vgprs + 7) / 8 * 8;

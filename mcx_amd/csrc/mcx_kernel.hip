@@ -527,6 +527,16 @@ struct Lane {
         }
     }
 
+    /* work counters of this lane into kp.counters (when requested) */
+    __device__ __forceinline__ void add_counters() const {
+        if (kp.counters) {
+            atomicAdd((unsigned long long*)kp.counters + 0, (unsigned long long)nstep);
+            atomicAdd((unsigned long long*)kp.counters + 1, (unsigned long long)ndep);
+            atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)nscat);
+            atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)nlaunch);
+        }
+    }
+
     /* src/mcx_core.cu:1767-2528, one lane */
     __device__ __forceinline__ void run() {
         const uint32_t flag = kp.savedetflag;
@@ -793,12 +803,7 @@ __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(con
     L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
     for (uint32_t i = 0; i < kp.partialdata; i++) L.ppath(i) = 0.f;
     L.run();
-    if (kp.counters) {
-        atomicAdd((unsigned long long*)kp.counters + 0, (unsigned long long)L.nstep);
-        atomicAdd((unsigned long long*)kp.counters + 1, (unsigned long long)L.ndep);
-        atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)L.nscat);
-        atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)L.nlaunch);
-    }
+    L.add_counters();
 }
 
 /* Batched persistent launch: every wave repeatedly takes the next chunk of
@@ -842,6 +847,7 @@ mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restric
             Lane<2, 1, TALLY, uint8_t> L(kp, tab, (float*)tab, 64, idx);
             L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
             L.run();
+            L.add_counters();
         }
         __builtin_amdgcn_wave_barrier();
     }

@@ -1,0 +1,248 @@
+"""Denoised low-photon runs against plain high-photon runs.
+
+Trains the U-Net of mcx_amd.denoise on pairs of low- and high-photon CW
+fluence maps that the transport engine generates (batched launches), then
+measures, against high-photon plain references:
+
+ * held-out heterogeneous volumes like the training ones: median voxel
+   error of the raw low-photon map, of the denoised map, and of plain runs
+   with 10x and 100x the photons;
+ * the large head-like volume (workloads.head256): median voxel error of
+   plain runs of increasing photon count and of the denoised 1e6-photon
+   run, and the wall time each needs to reach 5% median error.
+
+Voxel error is |estimate - reference| / reference over the voxels where
+the reference is reliable: its coefficient of variation, estimated from K
+independent runs, is below 1%.
+
+    python -m mcx_amd.dn_experiment --out results.json [--ntrain 256 ...]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import denoise, workloads
+
+
+def training_volume(master_seed, index, edge=64, photons=1e5):
+    """A heterogeneous volume like the batch workload's (layers and
+    ellipsoids, same property ranges), edge^3 voxels of 1 mm, one 0-5 ns
+    gate (CW), with a pencil or a disk source (radius 1-6 voxels) entering
+    the z=0 face along +z."""
+    cfg = workloads.batch_instance(master_seed, index, edge_range=(edge, edge), photon_range=(photons, photons),
+                                   ngates=1, tstep=5e-9)
+    g = np.random.default_rng([int(master_seed), int(index), 17])
+    if g.uniform() < 0.5:
+        cfg["srctype"] = "disk"
+        cfg["srcparam1"] = [float(g.uniform(1, 6)), 0, 0, 0]
+    cfg["nphoton"] = int(photons)
+    return cfg
+
+
+def cw_fluence(results, cfgs):
+    """CW fluence (sum over gates of flux x gate width) per result, in grid
+    units (per voxel area)."""
+    out = []
+    for r, c in zip(results, cfgs):
+        u = float(c.get("unitinmm", 1.0))
+        out.append(r["flux"].sum(axis=3) * float(c["tstep"]) * u * u)
+    return out
+
+
+def simulate(cfgs, nphoton, seed_salt, tally=2):
+    """Run every configuration with nphoton photons and a fresh seed; returns
+    (CW fluence maps, kernel ms, photons)."""
+    import mcx_amd
+    runs = []
+    for i, c in enumerate(cfgs):
+        d = dict(c)
+        d["nphoton"] = int(nphoton)
+        d["seed"] = int((c["seed"] * 1000003 + seed_salt * 7919 + i) % (2 ** 31 - 2)) + 1
+        d["quiet"] = True
+        runs.append(d)
+    res, info = mcx_amd.run_batch(runs, tally=tally, photons_per_lane=32, block_size=256)
+    return cw_fluence(res, runs), info["kernel_ms"], info["photons"]
+
+
+def reliable_mask(runs, cv_max=0.01):
+    """Mean of K independent maps and the voxels whose mean has a
+    coefficient of variation below cv_max."""
+    a = np.stack(runs).astype(np.float64)
+    mean = a.mean(axis=0)
+    sd = a.std(axis=0, ddof=1) / np.sqrt(a.shape[0])
+    mask = (mean > 0) & (sd < cv_max * mean)
+    return mean, mask
+
+
+def median_error(est, ref, mask):
+    return float(np.median(np.abs(est[mask] - ref[mask]) / ref[mask]))
+
+
+def make_features(phis, cfgs, flip_z=False):
+    f = []
+    for phi, c in zip(phis, cfgs):
+        x = denoise.features(phi, c["vol"], c["prop"], float(c.get("unitinmm", 1.0)))
+        f.append(x[:, :, ::-1] if flip_z else x)
+    return np.stack(f)
+
+
+def to_device_bf16(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to("cuda").to(torch.bfloat16)
+
+
+def denoise_maps(net, feats, flip_z=False):
+    """Denoised CW fluence maps for a stack of feature volumes."""
+    import torch
+    out = []
+    for k in range(feats.shape[0]):
+        pred = net.predict(to_device_bf16(feats[k:k + 1]))
+        torch.cuda.synchronize()
+        S = feats.shape[1]
+        m = denoise.inv_log_feature(pred.cpu().numpy().reshape(S, S, S))
+        out.append(m[:, :, ::-1] if flip_z else m)
+    return out
+
+
+def train(net, X, T, steps, batch=8, lr=1e-3, seed=2024, log=None):
+    """X: bf16 device tensor [N, S, S, S, 8]; T: fp32 device [N, S^3]."""
+    import torch
+    g = np.random.default_rng(seed)
+    n = X.shape[0]
+    losses = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        idx = torch.from_numpy(g.choice(n, size=batch, replace=False)).to(X.device)
+        xb = X.index_select(0, idx)
+        tb = T.index_select(0, idx).reshape(-1)
+        loss = net.train_step(xb, tb, lr=lr)
+        if s % 50 == 0 or s == steps - 1:
+            losses.append((s, float(loss)))
+            if log:
+                log("step %d loss %.5f (%.1f s)" % (s, float(loss), time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    return losses, time.perf_counter() - t0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--ntrain", type=int, default=256)
+    ap.add_argument("--neval", type=int, default=16)
+    ap.add_argument("--edge", type=int, default=64)
+    ap.add_argument("--low", type=float, default=1e5)
+    ap.add_argument("--high", type=float, default=1e7)
+    ap.add_argument("--ref", type=float, default=1e7, help="photons per reference run (x --refruns)")
+    ap.add_argument("--refruns", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--head", type=int, default=256, help="edge of the head volume (0: skip)")
+    ap.add_argument("--head-low", type=float, default=1e6)
+    ap.add_argument("--head-ref", type=float, default=1e8, help="photons per head reference run (x --refruns)")
+    ap.add_argument("--weights", default=None, help="save the trained parameters (npz)")
+    args = ap.parse_args(argv)
+    import torch
+
+    def log(msg):
+        print(msg, flush=True)
+
+    R = {"args": vars(args)}
+    t_all = time.perf_counter()
+    # ---- training pairs --------------------------------------------------
+    tr = [training_volume(2024, i, args.edge) for i in range(args.ntrain)]
+    lo, lo_ms, _ = simulate(tr, args.low, 1)
+    log("train inputs: %d x %.0e photons, %.0f ms" % (len(tr), args.low, lo_ms))
+    hi, hi_ms, hi_ph = simulate(tr, args.high, 2)
+    log("train targets: %d x %.0e photons, %.0f ms (%.0f photons/ms)" % (len(tr), args.high, hi_ms, hi_ph / hi_ms))
+    X = to_device_bf16(make_features(lo, tr))
+    T = torch.from_numpy(np.stack([denoise.log_feature(h).astype(np.float32).reshape(-1) for h in hi])).to("cuda")
+    del lo, hi
+    net = denoise.UNet(seed=2024)
+    R["params"] = net.count
+    losses, train_s = train(net, X, T, args.steps, args.batch, args.lr, log=log)
+    R["train"] = {"losses": losses, "seconds": train_s, "steps": args.steps, "batch": args.batch,
+                  "step_ms": train_s / max(1, args.steps) * 1e3}
+    del X, T
+    if args.weights:
+        np.savez_compressed(args.weights, p=net.state())
+    # ---- held-out volumes ----------------------------------------------------
+    ev = [training_volume(4048, i, args.edge) for i in range(args.neval)]
+    refs = [simulate(ev, args.ref, 100 + k)[0] for k in range(args.refruns)]
+    low, low_ms, _ = simulate(ev, args.low, 3)
+    p10, p10_ms, _ = simulate(ev, args.low * 10, 4)
+    p100, p100_ms, _ = simulate(ev, args.low * 100, 5)
+    t0 = time.perf_counter()
+    den = denoise_maps(net, make_features(low, ev))
+    den_s = time.perf_counter() - t0
+    rows = []
+    for i in range(len(ev)):
+        ref, mask = reliable_mask([r[i] for r in refs])
+        rows.append({"reliable_voxels": int(mask.sum()), "low": median_error(low[i], ref, mask),
+                     "denoised": median_error(den[i], ref, mask), "plain_x10": median_error(p10[i], ref, mask),
+                     "plain_x100": median_error(p100[i], ref, mask)})
+    R["heldout"] = {"volumes": rows, "denoise_seconds": den_s,
+                    "median": {k: float(np.median([r[k] for r in rows])) for k in ("low", "denoised", "plain_x10",
+                                                                                   "plain_x100")},
+                    "low_photons": args.low, "ref_photons": args.ref * args.refruns}
+    log("held-out: %s" % json.dumps(R["heldout"]["median"]))
+    # ---- head volume -------------------------------------------------------
+    if args.head:
+        import mcx_amd
+        hc = workloads.head256(nphoton=int(args.head_low), n=args.head, quiet=True)
+        hc["autopilot"] = 1
+
+        def plain(nph, seed):
+            c = dict(hc)
+            c["nphoton"] = int(nph)
+            c["seed"] = seed
+            r = mcx_amd.run(c)
+            st = mcx_amd.last_stats()
+            return cw_fluence([r], [c])[0], st["kernel_ms"]
+
+        hrefs, href_ms = [], 0.0
+        for k in range(args.refruns):
+            m, ms = plain(args.head_ref, 1000 + k)
+            hrefs.append(m)
+            href_ms += ms
+            log("head ref run %d: %.0f ms" % (k, ms))
+        ref, mask = reliable_mask(hrefs)
+        # leave-one-out reference for the error of one reference-sized run
+        ref_loo, mask_loo = reliable_mask(hrefs[1:])
+        e_ref1 = median_error(hrefs[0], ref_loo, mask_loo & mask)
+        plain_rows = []
+        for nph, seed in ((args.head_low, 7), (args.head_low * 10, 8)):
+            m, ms = plain(nph, seed)
+            plain_rows.append({"photons": nph, "kernel_ms": ms, "error": median_error(m, ref, mask)})
+        plain_rows.append({"photons": args.head_ref, "kernel_ms": href_ms / args.refruns, "error": e_ref1})
+        low_map, low_ms_h = plain(args.head_low, 11)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dmap = denoise_maps(net, make_features([low_map], [hc], flip_z=True), flip_z=True)[0]
+        dn_s = time.perf_counter() - t0
+        e_low = median_error(low_map, ref, mask)
+        e_den = median_error(dmap, ref, mask)
+        # plain: error ~ c / sqrt(N) fitted over the plain runs
+        ph = np.array([r["photons"] for r in plain_rows])
+        er = np.array([r["error"] for r in plain_rows])
+        cfit = float(np.exp(np.mean(np.log(er) + 0.5 * np.log(ph))))
+        n5 = (cfit / 0.05) ** 2
+        rate = float(np.sum(ph) / np.sum([r["kernel_ms"] for r in plain_rows]))
+        R["head"] = {"edge": args.head, "reliable_voxels": int(mask.sum()), "ref_photons": args.head_ref * args.refruns,
+                     "plain": plain_rows, "plain_photons_per_ms": rate, "plain_photons_for_5pct": n5,
+                     "plain_ms_to_5pct": n5 / rate, "low_error": e_low, "denoised_error": e_den,
+                     "denoised_ms": low_ms_h + dn_s * 1e3, "denoise_inference_ms": dn_s * 1e3,
+                     "low_kernel_ms": low_ms_h}
+        log("head: %s" % json.dumps(R["head"]))
+    R["seconds"] = time.perf_counter() - t_all
+    with open(args.out, "w") as f:
+        json.dump(R, f, indent=1)
+    log("wrote %s" % args.out)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -123,3 +123,66 @@ def batch_instance(master_seed, index, edge_range=(40, 160), photon_range=(1e5, 
 def batch256(master_seed=7, n=256, **kw):
     """The 256-instance heterogeneous batch (edges 40-160, 1e5-1e7 photons)."""
     return [batch_instance(master_seed, i, **kw) for i in range(n)]
+
+
+def _perturbed_radius(g, theta, phi, amp, lmax=4):
+    """1 + a smooth random radial perturbation of relative amplitude ~amp:
+    a sum of low-order products cos(l theta + a) cos(m phi + b)."""
+    r = np.ones_like(theta)
+    for l in range(1, lmax + 1):
+        for m in range(0, lmax + 1):
+            c = g.normal(0, amp / (l + m + 1))
+            r = r + c * np.cos(l * theta + g.uniform(0, 2 * np.pi)) * np.cos(m * phi + g.uniform(0, 2 * np.pi))
+    return r
+
+
+def head256(nphoton=1_000_000_000, seed=99, n=256, unitinmm=0.5, **kw):
+    """Large-scale five-label head-like volume: nested randomly perturbed
+    ellipsoidal shells on a 256^3 grid of 0.5 mm voxels.  Shell thicknesses
+    (mm): outer 3-5, next 5-8, next 1-3; the core is split into two random
+    regions by a smooth random surface.  Media (per mm): 1 outer (0.019, 7.8,
+    0.89, 1.37), 2 next (0.019, 7.8, 0.89, 1.37), 3 thin layer (0.004,
+    0.009, 0.89, 1.37), 4 core A (0.02, 9.0, 0.89, 1.37), 5 core B (0.08,
+    40.9, 0.89, 1.37).  Pencil source entering the outer surface at the top
+    of the head along -z; 20 gates of 0.25 ns."""
+    g = np.random.default_rng(seed)
+    c = (np.arange(n, dtype=np.float32) + 0.5) - n / 2.0
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    semi = np.array([0.40, 0.46, 0.44]) * n * g.uniform(0.95, 1.05, size=3)
+    rr = np.sqrt((X / semi[0]) ** 2 + (Y / semi[1]) ** 2 + (Z / semi[2]) ** 2)
+    rxy = np.sqrt(X * X + Y * Y)
+    theta = np.arctan2(rxy, Z)
+    phi = np.arctan2(Y, X)
+    pert = _perturbed_radius(g, theta, phi, 0.04)
+    rho = rr / pert  # < 1 inside the outer surface
+    mean_semi = float(np.mean(semi))
+    t1 = g.uniform(3, 5) / unitinmm / mean_semi
+    t2 = g.uniform(5, 8) / unitinmm / mean_semi
+    t3 = g.uniform(1, 3) / unitinmm / mean_semi
+    vol = np.zeros((n, n, n), dtype=np.uint8)
+    vol[rho < 1.0] = 1
+    vol[rho < 1.0 - t1] = 2
+    vol[rho < 1.0 - t1 - t2] = 3
+    core = rho < 1.0 - t1 - t2 - t3
+    split = (X * g.normal() + Y * g.normal() + Z * g.normal()) / n + 0.15 * _perturbed_radius(g, theta, phi, 1.0, 3) - 0.15
+    vol[core & (split >= 0)] = 4
+    vol[core & (split < 0)] = 5
+    prop = np.array([[0, 0, 1, 1], [0.019, 7.8, 0.89, 1.37], [0.019, 7.8, 0.89, 1.37], [0.004, 0.009, 0.89, 1.37],
+                     [0.02, 9.0, 0.89, 1.37], [0.08, 40.9, 0.89, 1.37]], dtype=np.float32)
+    cfg = {
+        "nphoton": int(nphoton),
+        "vol": vol,
+        "prop": prop,
+        "unitinmm": unitinmm,
+        "srcpos": [n / 2.0, n / 2.0, n - 0.5],
+        "srcdir": [0, 0, -1],
+        "issrcfrom0": 1,
+        "tstart": 0.0,
+        "tend": 5e-9,
+        "tstep": 2.5e-10,
+        "seed": seed,
+        "isreflect": 1,
+        "issavedet": 0,
+    }
+    cfg.update(kw)
+    return cfg

@@ -268,7 +268,7 @@ def test_bench_refuses_multi_gpu():
 def test_batch_options_layout_and_empty_batch():
     """McxBatchOptions mirror matches the compiled struct; an empty batch is
     rejected before any device call."""
-    assert ctypes.sizeof(_lib.McxBatchOptions) == 64
+    assert ctypes.sizeof(_lib.McxBatchOptions) == 96
     opt = _lib.McxBatchOptions()
     rc = L.mcx_run_batch(None, 0, None, ctypes.byref(opt))
     assert rc < 0
@@ -296,3 +296,27 @@ def test_batch_generator_is_deterministic_and_in_range():
         assert c["tend"] == pytest.approx(5e-9) and c["tstep"] == pytest.approx(5e-10)
         # layers are planar along z: every z-slice outside inclusions has one label
         assert len(np.unique(c["vol"][0, 0, :])) >= 1
+
+
+def test_denoise_experiment_helpers():
+    """error metric and reliability mask of the denoising experiment, and
+    the feature/target transform (host logic only)"""
+    from mcx_amd import denoise, dn_experiment
+    rng = np.random.default_rng(0)
+    truth = np.exp(rng.uniform(-8, 0, size=(8, 8, 8)))
+    runs = [truth * (1 + 0.005 * rng.standard_normal(truth.shape)) for _ in range(10)]
+    runs[3][0, 0, 0] = truth[0, 0, 0] * 3.0  # one noisy voxel
+    mean, mask = dn_experiment.reliable_mask(runs)
+    assert not mask[0, 0, 0] and mask.sum() > 0.99 * (truth.size - 1)
+    assert dn_experiment.median_error(truth * 1.05, truth, mask) == pytest.approx(0.05)
+    phi = np.array([0.0, 1e-9, 1e-3, 1.0])
+    assert np.allclose(denoise.inv_log_feature(denoise.log_feature(phi)), phi, rtol=1e-9, atol=1e-15)
+    c1 = dn_experiment.training_volume(2024, 5)
+    c2 = dn_experiment.training_volume(2024, 5)
+    assert np.array_equal(c1["vol"], c2["vol"]) and c1["vol"].shape == (64, 64, 64)
+    assert c1["tend"] == pytest.approx(5e-9) and c1["tstep"] == pytest.approx(5e-9)
+    f = denoise.features(np.ones((64, 64, 64)), c1["vol"], c1["prop"])
+    assert f.shape == (64, 64, 64, denoise.NFEAT) and np.all(f[..., 5] == 1.0)
+    from mcx_amd import workloads
+    h = workloads.head256(n=64)
+    assert set(np.unique(h["vol"])) <= {0, 1, 2, 3, 4, 5} and h["unitinmm"] == 0.5
