@@ -53,7 +53,7 @@ int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs, int nb, in
 int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* out, int ocs, long long nvox, int c, void* stream);
 /* output head: pred[v] = b + sum_c f[v][c] w[c] + res[v*rcs] (FP32); with a
  * target also dpred = 2 (pred - target) / nvox and lpart[blk] = partial sums
- * of the squared error over nblocks blocks */
+ * of the squared error over nblocks blocks; NaN targets are left out */
 int mcx_dn_head_fwd(const void* f, int fcs, const float* w, const float* b, const void* res, int rcs, const float* target,
                     float* pred, float* dpred, float* lpart, int nblocks, long long nvox, int c, void* stream);
 /* df[v][c] = dpred[v] w[c]; dwb[0..c-1] = sum_v dpred f, dwb[c] = sum_v dpred;
@@ -65,6 +65,17 @@ int mcx_dn_head_bwd(const void* f, int fcs, const float* w, const float* dpred, 
 int mcx_dn_adam(float* p, float* m, float* v, const float* g, void* pbf16, long long n, float lr, float beta1, float beta2,
                 float eps, int step, void* stream);
 int mcx_dn_f32_to_bf16(const float* a, void* b, long long n, void* stream);
+/* network input features of a CW fluence map: phi (FP32) and vol (uint8
+ * labels) in the engine's x-fastest order, tab = ntab x {mua, mus, g, n}
+ * per voxel; f = BF16 [nx][ny][nz][8]: (log(phi + eps) + shift) / scale,
+ * log mua + 4, log mus, 10 (g - 0.8), 10 (n - 1.4), tissue flag, 0, 0.
+ * flipz mirrors the z axis. */
+int mcx_dn_features(const float* phi, const unsigned char* vol, const float* tab, int ntab, void* f, int nx, int ny, int nz,
+                    int flipz, float eps, float shift, float scale, void* stream);
+/* phi (x-fastest) = max(exp(pred * scale - shift) - eps, 0) from a
+ * prediction in the [nx][ny][nz] order of the features */
+int mcx_dn_inv_features(const float* pred, float* phi, int nx, int ny, int nz, int flipz, float eps, float shift, float scale,
+                        void* stream);
 
 #ifdef __cplusplus
 }

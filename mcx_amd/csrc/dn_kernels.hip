@@ -407,7 +407,9 @@ __global__ void head_fwd(const uint16_t* __restrict__ f, int fcs, const float* _
         s += bf2f(res[v * rcs]);
         pred[v] = s;
         if (target) {
-            const float e = s - target[v];
+            /* a NaN target marks a voxel outside the loss */
+            const float t = target[v];
+            const float e = (t == t) ? s - t : 0.f;
             dpred[v] = 2.f * e / (float)nvox;
             l += e * e;
         }
@@ -442,6 +444,51 @@ __global__ void head_bwd(const uint16_t* __restrict__ f, int fcs, const float* _
         float t = 0.f;
         for (int i = 0; i < 256; i++) t += red[i][threadIdx.x];
         part[(long long)blockIdx.x * (c + 1) + threadIdx.x] = t;
+    }
+}
+
+/* Network input features of a CW fluence map (grid units) on a label
+ * volume, BF16 [v][8]: scaled log fluence, then (tissue voxels only) log mua
+ * + 4, log mus, 10 (g - 0.8), 10 (n - 1.4) from the per-voxel media table
+ * tab[label] = {mua, mus, g, n}, the tissue flag, two zeros.  flipz mirrors
+ * the z axis (the output voxel (x, y, z) takes input (x, y, D-1-z)). */
+__global__ void features(const float* __restrict__ phi, const uint8_t* __restrict__ vol, const float* __restrict__ tab,
+                         int ntab, uint16_t* __restrict__ f, int nx, int ny, int nz, int flipz, float eps, float shift,
+                         float scale) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+        /* output index o is x-slowest ([x][y][z]); the engine's arrays are
+         * x-fastest ([z][y][x]) */
+        const int z = (int)(o % nz), y = (int)((o / nz) % ny), x = (int)(o / ((long long)nz * ny));
+        const int zs = flipz ? nz - 1 - z : z;
+        const long long i = ((long long)zs * ny + y) * nx + x;
+        const int lab = vol[i];
+        float v[8];
+        v[0] = (logf(phi[i] + eps) + shift) / scale;
+        const bool t = lab > 0 && lab < ntab;
+        const float* m = tab + 4 * (t ? lab : 0);
+        v[1] = t ? logf(fmaxf(m[0], 1e-6f)) + 4.f : 0.f;
+        v[2] = t ? logf(fmaxf(m[1], 1e-6f)) : 0.f;
+        v[3] = t ? (m[2] - 0.8f) * 10.f : 0.f;
+        v[4] = t ? (m[3] - 1.4f) * 10.f : 0.f;
+        v[5] = t ? 1.f : 0.f;
+        v[6] = 0.f;
+        v[7] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) f[o * 8 + k] = f2bf(v[k]);
+    }
+}
+
+/* fluence from a predicted scaled log fluence, back in the engine's
+ * x-fastest order (with the same optional z mirror) */
+__global__ void inv_features(const float* __restrict__ pred, float* __restrict__ phi, int nx, int ny, int nz, int flipz,
+                             float eps, float shift, float scale) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(o % nz), y = (int)((o / nz) % ny), x = (int)(o / ((long long)nz * ny));
+        const int zs = flipz ? nz - 1 - z : z;
+        const long long i = ((long long)zs * ny + y) * nx + x;
+        phi[i] = fmaxf(expf(pred[o] * scale - shift) - eps, 0.f);
     }
 }
 
@@ -642,5 +689,26 @@ extern "C" int mcx_dn_adam(float* p, float* m, float* v, const float* g, void* p
 extern "C" int mcx_dn_f32_to_bf16(const float* a, void* b, long long n, void* stream) {
     if (!a || !b || n <= 0) return -22;
     hipLaunchKernelGGL(f32_to_bf16, dim3(dn_grid(n, 4096)), dim3(256), 0, (hipStream_t)stream, a, (uint16_t*)b, n);
+    return dn_ret();
+}
+
+/* phi: FP32 CW fluence in the engine's x-fastest order; vol: uint8 labels
+ * (same order); tab: ntab x {mua, mus, g, n} per voxel; f: BF16 features
+ * [nx][ny][nz][8] (the network's NDHWC order with D = x) */
+extern "C" int mcx_dn_features(const float* phi, const unsigned char* vol, const float* tab, int ntab, void* f, int nx, int ny,
+                               int nz, int flipz, float eps, float shift, float scale, void* stream) {
+    if (!phi || !vol || !tab || !f || ntab <= 0 || nx <= 0 || ny <= 0 || nz <= 0 || scale == 0.f) return -22;
+    const long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(features, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, phi, (const uint8_t*)vol, tab, ntab,
+                       (uint16_t*)f, nx, ny, nz, flipz, eps, shift, scale);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_inv_features(const float* pred, float* phi, int nx, int ny, int nz, int flipz, float eps, float shift,
+                                   float scale, void* stream) {
+    if (!pred || !phi || nx <= 0 || ny <= 0 || nz <= 0) return -22;
+    const long long n = (long long)nx * ny * nz;
+    hipLaunchKernelGGL(inv_features, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, pred, phi, nx, ny, nz, flipz, eps,
+                       shift, scale);
     return dn_ret();
 }

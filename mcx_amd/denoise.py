@@ -80,6 +80,42 @@ def features(phi, vol, prop, unitinmm=1.0):
     return f
 
 
+def _media_table(prop, unitinmm):
+    prop = np.asarray(prop, dtype=np.float32)
+    tab = prop.copy()
+    tab[:, 0] *= unitinmm
+    tab[:, 1] *= unitinmm
+    return np.ascontiguousarray(tab)
+
+
+def features_gpu(phi, vol, prop, unitinmm=1.0, flip_z=False):
+    """features() computed on the GPU by mcx_dn_features: a BF16 device
+    tensor [1, X, Y, Z, 8].  phi and vol are [X, Y, Z] arrays."""
+    torch = _torch()
+    L = _lib.lib()
+    X, Y, Z = vol.shape
+    dphi = torch.from_numpy(np.asfortranarray(phi, dtype=np.float32).ravel(order="F")).to("cuda")
+    dvol = torch.from_numpy(np.asfortranarray(vol, dtype=np.uint8).ravel(order="F")).to("cuda")
+    dtab = torch.from_numpy(_media_table(prop, unitinmm)).to("cuda")
+    f = torch.empty((1, X, Y, Z, NFEAT), dtype=torch.bfloat16, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _chk(L.mcx_dn_features(_ptr(dphi), _ptr(dvol), _ptr(dtab), int(dtab.shape[0]), _ptr(f), X, Y, Z, int(flip_z), LOG_EPS,
+                           LOG_SHIFT, LOG_SCALE, st), "features")
+    return f
+
+
+def inverse_gpu(pred, shape, flip_z=False):
+    """Fluence [X, Y, Z] (float32, F-order) from a prediction tensor."""
+    torch = _torch()
+    L = _lib.lib()
+    X, Y, Z = shape
+    phi = torch.empty(X * Y * Z, dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _chk(L.mcx_dn_inv_features(_ptr(pred), _ptr(phi), X, Y, Z, int(flip_z), LOG_EPS, LOG_SHIFT, LOG_SCALE, st),
+         "inv_features")
+    return phi.cpu().numpy().reshape((X, Y, Z), order="F")
+
+
 # (name, cin, cout) of the 3x3x3 convolutions in forward order
 CONVS = [("e1a", NFEAT, 16), ("e1", 16, 16), ("e2a", 16, 32), ("e2", 32, 32), ("e3a", 32, 64), ("e3", 64, 64),
          ("b1", 64, 128), ("b2", 128, 128), ("d3a", 192, 64), ("d3", 64, 64), ("d2a", 96, 32), ("d2", 32, 32),

@@ -261,3 +261,47 @@ def test_unet_training_reduces_loss():
     losses = [float(net.train_step(xd, td, lr=1e-3)) for _ in range(60)]
     assert np.all(np.isfinite(losses))
     assert min(losses[-5:]) < 0.5 * losses[0], (losses[0], losses[-5:])
+
+
+def test_feature_kernels_match_host_features():
+    from mcx_amd import denoise
+    rng = np.random.default_rng(12)
+    X, Y, Z = 6, 10, 12
+    vol = rng.integers(0, 4, size=(X, Y, Z)).astype(np.uint8)
+    prop = np.array([[0, 0, 1, 1], [0.01, 5.0, 0.9, 1.37], [0.05, 12.0, 0.8, 1.4], [0.002, 0.8, 0.95, 1.33]],
+                    dtype=np.float32)
+    phi = np.exp(rng.uniform(-20, 0, size=(X, Y, Z))) * (rng.uniform(size=(X, Y, Z)) > 0.1)
+    for flip in (False, True):
+        got = denoise.features_gpu(phi, vol, prop, unitinmm=0.5, flip_z=flip)
+        torch.cuda.synchronize()
+        g = got.float().cpu().numpy()[0]
+        ref = denoise.features(phi, vol, prop, unitinmm=0.5)
+        if flip:
+            ref = ref[:, :, ::-1]
+        assert np.all(np.abs(g - ref) <= np.abs(ref) * 2.0 ** -8 + 1e-5)
+        pred = torch.from_numpy(np.ascontiguousarray(ref[..., 0]).reshape(-1).astype(np.float32)).cuda()
+        back = denoise.inverse_gpu(pred, (X, Y, Z), flip_z=flip)
+        assert np.allclose(back, phi, rtol=1e-4, atol=1e-12)
+
+
+def test_head_loss_skips_nan_targets():
+    rng = np.random.default_rng(13)
+    nv, c = 2000, 16
+    f = dn_ref.bf16_round(rng.standard_normal((nv, c)))
+    res = dn_ref.bf16_round(rng.standard_normal((nv, 8)))
+    w = rng.standard_normal(c).astype(np.float32)
+    b = np.zeros(1, dtype=np.float32)
+    tgt = rng.standard_normal(nv).astype(np.float32)
+    tgt[::3] = np.nan
+    pred = torch.zeros(nv, dtype=torch.float32, device="cuda")
+    dpred = torch.zeros(nv, dtype=torch.float32, device="cuda")
+    lpart = torch.zeros(32, dtype=torch.float32, device="cuda")
+    dw, dbb, dt = (torch.from_numpy(a).cuda() for a in (w, b, tgt))
+    assert L.mcx_dn_head_fwd(_p(_dev_bf16(f)), c, _p(dw), _p(dbb), _p(_dev_bf16(res)), 8, _p(dt), _p(pred), _p(dpred),
+                             _p(lpart), 32, nv, c, None) == 0
+    rp = f.astype(np.float64) @ w + res[:, 0]
+    ok = ~np.isnan(tgt)
+    e = np.where(ok, rp - tgt, 0.0)
+    assert abs(float(lpart.sum()) - float((e ** 2).sum())) < 1e-3 * float((e ** 2).sum())
+    d = _host_f32(dpred, (nv,))
+    assert np.all(d[~ok] == 0) and np.allclose(d[ok], 2 * e[ok] / nv, rtol=1e-4, atol=1e-9)
