@@ -140,3 +140,33 @@ def test_autopilot_cube60b_known_answer():
     assert st["energytot"] == 1000000.0
     assert 0.27 <= st["energyabs"] / st["energytot"] < 0.28
     assert mcx_amd.last_stats()["errlanes"] == 0
+
+
+@pytest.mark.parametrize("bc,nphoton,band", [("aarraa", 100000, (0.27, 0.28)), ("cccccc", 1000, (0.99, 1.0)),
+                                             ("rrrrrr", 20000, (0.0, 1.0)), ("mmaamm", 20000, (0.0, 1.0))])
+def test_boundary_conditions_bitexact(bc, nphoton, band):
+    """Per-face boundary strings (absorb / reflect / mirror / cyclic):
+    bit-exact against the oracle, and the reference's known answers
+    (test/testmcx.sh:73, :77)."""
+    cfg = workloads.cube60(nphoton, reflect=0, bc=bc)
+    res = _run(_layout(cfg, nthread=2048))
+    res["_tstep"] = cfg["tstep"]
+    ora = oracle_run(cfg, 2048)
+    assert ora["errors"] == 0
+    _check_energies(res, ora)
+    _check_flux(res, ora, rtol=2e-4, floor=1e-2)
+    a = res["stat"]["energyabs"] / res["stat"]["energytot"]
+    assert band[0] <= a < band[1], a
+
+
+def test_boundary_detector_faces_bitexact():
+    """All faces detectors ('______111111', test/testmcx.sh:112): 97-99 %
+    detected, records identical to the oracle's"""
+    cfg = workloads.cube60(10000, reflect=0, bc="______111111", issavedet=1, savedetflag="DP")
+    res = _run(_layout(cfg, nthread=2048))
+    ora = oracle_run(cfg, 2048)
+    _check_energies(res, ora)
+    detp = res["detp"].T
+    assert 9700 <= detp.shape[0] <= 9999
+    assert detp.shape[0] == ora["detected"]
+    assert np.array_equal(sort_rows(detp), sort_rows(ora["detp"]))

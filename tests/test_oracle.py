@@ -194,3 +194,23 @@ def test_detection_count_known_answer():
     assert 4000 <= per_million < 5000, per_million
     detid = r["detp"][:, 0]
     assert set(np.unique(detid)) <= {1, 2, 3, 4}
+
+
+@pytest.mark.parametrize("bc,nphoton,lo,hi", [
+    ("aarraa", 100000, 0.27, 0.28),   # test/testmcx.sh:73 (-b 0 -B aarraa: absorbed 27.x%)
+    ("cccccc", 1000, 0.99, 1.0),      # test/testmcx.sh:77 (cyclic: absorbed 99.x%)
+])
+def test_boundary_condition_known_answers(bc, nphoton, lo, hi):
+    cfg = workloads.cube60(nphoton, reflect=0, bc=bc)
+    r = oracle_run(cfg, 2048)
+    assert r["errors"] == 0
+    esc, tot = seq_sum(r["genergy"][0::2]), seq_sum(r["genergy"][1::2])
+    assert lo <= (tot - esc) / tot < hi
+
+
+def test_boundary_detector_faces_known_answer():
+    """test/testmcx.sh:112: with every face a detector ('______111111'),
+    97-99 % of 1e4 photons are detected (the rest are cut by the 5 ns gate)"""
+    cfg = workloads.cube60(10000, reflect=0, bc="______111111", issavedet=1, savedetflag="DP")
+    r = oracle_run(cfg, 2048)
+    assert 9700 <= r["detected"] <= 9999
