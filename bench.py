@@ -156,9 +156,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atomic-roof", action="store_true", help="print the scattered-atomic microbenchmark and exit")
+    ap.add_argument("--dn-timing", action="store_true", help="time the denoiser U-Net (train step, 256^3 inference)")
     args = ap.parse_args()
     if args.atomic_roof:
         atomic_roof_table()
+        return
+    if args.dn_timing:
+        from mcx_amd import dn_experiment
+        dn_experiment.timing()
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,

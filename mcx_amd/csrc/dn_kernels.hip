@@ -71,6 +71,15 @@ __device__ __forceinline__ void tap_offsets(int t, int& dz, int& dy, int& dx) {
     dx = t % 3 - 1;
 }
 
+/* Workgroups are dispatched to the 8 XCDs round-robin by linear id, and
+ * each XCD has its own L2.  This bijection gives XCD x the contiguous
+ * logical range [x*q + min(x, r), ...) of the n ids (q = n / 8, r = n % 8),
+ * so neighbouring tiles, which read the same input rows, share one L2. */
+__device__ __forceinline__ unsigned int xcd_logical(unsigned int h, unsigned int n) {
+    const unsigned int x = h & 7u, j = h >> 3, q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + j;
+}
+
 /* ---- 3x3x3 convolution, forward (and input gradient) --------------------
  * y[m][co] = act(bias[co] + sum_{t,ci} x[m + off_t][ci] * w[co][t*cin + ci])
  * where x is multiplied by (mask > 0) channel-wise when mask is given.
@@ -83,7 +92,8 @@ __global__ void __launch_bounds__(256) conv3_fwd(const uint16_t* __restrict__ x,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const long long mtot = (long long)nb * D * H * W;
-    const long long m = (long long)blockIdx.x * 64 + wave * 16 + r;
+    const unsigned int blk = xcd_logical(blockIdx.x, gridDim.x);
+    const long long m = (long long)blk * 64 + wave * 16 + r;
     int vx = 0, vy = 0, vz = 0;
     long long vn = 0;
     const bool mvalid = m < mtot;
@@ -123,7 +133,7 @@ __global__ void __launch_bounds__(256) conv3_fwd(const uint16_t* __restrict__ x,
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j], 0, 0, 0);
         }
     }
-    const long long mbase = (long long)blockIdx.x * 64 + wave * 16;
+    const long long mbase = (long long)blk * 64 + wave * 16;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
         const int co = j * 16 + r;
@@ -169,7 +179,9 @@ __global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ 
     const int cinp = (cin + 15) & ~15;
     uint16_t* sdy = sm;               /* [cout][32] */
     uint16_t* sx = sm + cout * 32;    /* [cinp][32] */
-    const int t = blockIdx.x, s = blockIdx.y;
+    /* the 27 taps of one voxel range read the same data: keep them on one XCD */
+    const unsigned int lid = xcd_logical(blockIdx.x + 27u * blockIdx.y, 27u * gridDim.y);
+    const int t = (int)(lid % 27u), s = (int)(lid / 27u);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     int dz, dyo, dxo;

@@ -16,6 +16,7 @@ the reference is reliable: its coefficient of variation, estimated from K
 independent runs, is below 1%.
 
     python -m mcx_amd.dn_experiment --out results.json [--ntrain 256 ...]
+    python -m mcx_amd.dn_experiment --timing     (U-Net step / inference times)
 """
 import argparse
 import json
@@ -169,7 +170,47 @@ def train(net, X, T, steps, batch=8, lr=1e-3, seed=2024, log=None):
     return losses, time.perf_counter() - t0
 
 
+def timing(nb=8, S=64, steps=20, infer_S=256):
+    """Training-step and inference times of the U-Net on random inputs."""
+    import torch
+    g = np.random.default_rng(0)
+    net = denoise.UNet(seed=1)
+    X = to_device_bf16(g.standard_normal((nb, S, S, S, denoise.NFEAT)).astype(np.float32))
+    T = torch.from_numpy(g.standard_normal(nb * S ** 3).astype(np.float32)).cuda()
+    net.train_step(X, T)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(X, T)
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    out = {"train_batch": nb, "train_edge": S, "train_step_ms": step_ms}
+    if infer_S:
+        Xi = to_device_bf16(g.standard_normal((1, infer_S, infer_S, infer_S, denoise.NFEAT)).astype(np.float32))
+        net.predict(Xi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            net.predict(Xi)
+        torch.cuda.synchronize()
+        out.update(infer_edge=infer_S, infer_ms=(time.perf_counter() - t0) / 3 * 1e3)
+    # FLOPs of the convolutions (2 per multiply-add) per sample and edge
+    fl = 0.0
+    for name, cin, cout in denoise.CONVS:
+        lvl = {"e1": 0, "e2": 1, "e3": 2, "b1": 3, "b2": 3, "d3": 2, "d2": 1, "d1": 0}[name.rstrip("a")]
+        fl += 2.0 * 27 * cin * cout * (1.0 / 8 ** lvl)
+    out["fwd_gflop_per_voxel_ratio"] = fl
+    out["train_tflops"] = 3 * fl * nb * S ** 3 / (step_ms * 1e-3) / 1e12
+    if infer_S:
+        out["infer_tflops"] = fl * infer_S ** 3 / (out["infer_ms"] * 1e-3) / 1e12
+    print(json.dumps(out), flush=True)
+    return out
+
+
 def main(argv=None):
+    if argv is None and len(sys.argv) > 1 and sys.argv[1] == "--timing":
+        timing()
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--ntrain", type=int, default=256)
