@@ -276,29 +276,38 @@ __global__ void wgrad_reduce(const float* __restrict__ partial, int nsplit, floa
     }
 }
 
-/* db[co] partial sums over voxel ranges: part[s][co] */
+/* db[co] partial sums over voxel ranges: part[s][co].  Each thread reads
+ * 8 channels (one 16-byte load) of one voxel per iteration; cout/8 channel
+ * groups x 256/(cout/8) voxel lanes per workgroup. */
 __global__ void bias_grad(const uint16_t* __restrict__ dy, int dycs, const uint16_t* __restrict__ mask, int mcs,
                           float* __restrict__ part, long long nvox, int vps, int cout) {
-    __shared__ float red[256];
+    __shared__ float red[256][9];
     const int s = blockIdx.x;
     const long long v0 = (long long)s * vps;
     const long long v1 = (v0 + vps < nvox) ? v0 + vps : nvox;
-    /* threads: cout channels x (256 / cout) voxel lanes */
-    const int lanes = 256 / cout;
-    const int co = threadIdx.x % cout, vl = threadIdx.x / cout;
-    float sum = 0.f;
-    if (vl < lanes) {
-        for (long long v = v0 + vl; v < v1; v += lanes) {
-            float d = bf2f(dy[v * dycs + co]);
-            if (mask && !(bf2f(mask[v * mcs + co]) > 0.f)) d = 0.f;
-            sum += d;
+    const int ng = cout / 8, lanes = 256 / ng;
+    const int g = threadIdx.x % ng, vl = threadIdx.x / ng;
+    float acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] = 0.f;
+    for (long long v = v0 + vl; v < v1; v += lanes) {
+        Frag f = load8(dy + v * dycs + 8 * g);
+        if (mask) {
+            const Frag mk = load8(mask + v * mcs + 8 * g);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (!(bf2f(mk.h[k]) > 0.f)) f.h[k] = 0;
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] += bf2f(f.h[k]);
     }
-    red[threadIdx.x] = sum;
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[threadIdx.x][k] = acc[k];
     __syncthreads();
     if (threadIdx.x < cout) {
+        const int gg = threadIdx.x / 8, k = threadIdx.x % 8;
         float t = 0.f;
-        for (int k = 0; k < lanes; k++) t += red[k * cout + threadIdx.x];
+        for (int l = 0; l < lanes; l++) t += red[l * ng + gg][k];
         part[(long long)s * cout + threadIdx.x] = t;
     }
 }
@@ -390,6 +399,131 @@ __global__ void upsample_bwd(const uint16_t* __restrict__ dy, int dycs, uint16_t
             s += bf2f(dy[(((nn * 2 * D + z2) * H2 + y2) * W2 + x2) * dycs + ch]);
         }
         dx[(i / c) * dxcs + ch] = f2bf(s);
+    }
+}
+
+/* ---- 8-channel (16-byte) forms of the elementwise kernels, used when
+ * channel counts, strides and pointers allow ------------------------------ */
+__device__ __forceinline__ void store8(uint16_t* p, const Frag& f) { *(uint4*)p = f.u; }
+
+__global__ void maxpool_fwd8(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs,
+                             uint8_t* __restrict__ arg, int nb, int D, int H, int W, int c) {
+    const int d2 = D / 2, h2 = H / 2, w2 = W / 2, c8 = c / 8;
+    const long long n = (long long)nb * d2 * h2 * w2 * c8;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c8) * 8;
+        const long long ov = i / c8;
+        long long q = ov;
+        const int ox = (int)(q % w2); q /= w2;
+        const int oy = (int)(q % h2); q /= h2;
+        const int oz = (int)(q % d2);
+        const long long on = q / d2;
+        float best[8];
+        uint8_t bi[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) { best[j] = -3.0e38f; bi[j] = 0; }
+        for (int k = 0; k < 8; k++) {
+            const int zz = 2 * oz + (k >> 2), yy = 2 * oy + ((k >> 1) & 1), xx = 2 * ox + (k & 1);
+            const Frag f = load8(x + (((on * D + zz) * H + yy) * W + xx) * xcs + ch);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const float v = bf2f(f.h[j]);
+                if (v > best[j]) { best[j] = v; bi[j] = (uint8_t)k; }
+            }
+        }
+        Frag o;
+#pragma unroll
+        for (int j = 0; j < 8; j++) o.h[j] = f2bf(best[j]);
+        store8(y + ov * ycs + ch, o);
+        uint2 a;
+        a.x = bi[0] | (bi[1] << 8) | (bi[2] << 16) | ((uint32_t)bi[3] << 24);
+        a.y = bi[4] | (bi[5] << 8) | (bi[6] << 16) | ((uint32_t)bi[7] << 24);
+        *(uint2*)(arg + ov * c + ch) = a;
+    }
+}
+
+__global__ void maxpool_bwd8(const uint16_t* __restrict__ dy, int dycs, const uint8_t* __restrict__ arg,
+                             uint16_t* __restrict__ dx, int dxcs, int nb, int D, int H, int W, int c) {
+    const int d2 = D / 2, h2 = H / 2, w2 = W / 2, c8 = c / 8;
+    const long long n = (long long)nb * D * H * W * c8;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c8) * 8;
+        const long long v = i / c8;
+        long long q = v;
+        const int xx = (int)(q % W); q /= W;
+        const int yy = (int)(q % H); q /= H;
+        const int zz = (int)(q % D);
+        const long long nn = q / D;
+        const long long o = ((nn * d2 + zz / 2) * h2 + yy / 2) * w2 + xx / 2;
+        const int k = ((zz & 1) << 2) | ((yy & 1) << 1) | (xx & 1);
+        const uint2 a = *(const uint2*)(arg + o * c + ch);
+        const Frag g = load8(dy + o * dycs + ch);
+        Frag r;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t w = (j < 4) ? a.x : a.y;
+            r.h[j] = (((w >> (8 * (j & 3))) & 0xFFu) == (uint32_t)k) ? g.h[j] : (uint16_t)0;
+        }
+        store8(dx + v * dxcs + ch, r);
+    }
+}
+
+__global__ void upsample_fwd8(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs, int nb, int D,
+                              int H, int W, int c) {
+    const int D2 = 2 * D, H2 = 2 * H, W2 = 2 * W, c8 = c / 8;
+    const long long n = (long long)nb * D2 * H2 * W2 * c8;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c8) * 8;
+        const long long v = i / c8;
+        long long q = v;
+        const int xx = (int)(q % W2); q /= W2;
+        const int yy = (int)(q % H2); q /= H2;
+        const int zz = (int)(q % D2);
+        const long long nn = q / D2;
+        store8(y + v * ycs + ch, load8(x + (((nn * D + zz / 2) * H + yy / 2) * W + xx / 2) * xcs + ch));
+    }
+}
+
+__global__ void upsample_bwd8(const uint16_t* __restrict__ dy, int dycs, uint16_t* __restrict__ dx, int dxcs, int nb, int D,
+                              int H, int W, int c) {
+    const int H2 = 2 * H, W2 = 2 * W, c8 = c / 8;
+    const long long n = (long long)nb * D * H * W * c8;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(i % c8) * 8;
+        const long long v = i / c8;
+        long long q = v;
+        const int xx = (int)(q % W); q /= W;
+        const int yy = (int)(q % H); q /= H;
+        const int zz = (int)(q % D);
+        const long long nn = q / D;
+        float s8[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s8[j] = 0.f;
+        for (int k = 0; k < 8; k++) {
+            const int z2 = 2 * zz + (k >> 2), y2 = 2 * yy + ((k >> 1) & 1), x2 = 2 * xx + (k & 1);
+            const Frag f = load8(dy + (((nn * 2 * D + z2) * H2 + y2) * W2 + x2) * dycs + ch);
+#pragma unroll
+            for (int j = 0; j < 8; j++) s8[j] += bf2f(f.h[j]);
+        }
+        Frag r;
+#pragma unroll
+        for (int j = 0; j < 8; j++) r.h[j] = f2bf(s8[j]);
+        store8(dx + v * dxcs + ch, r);
+    }
+}
+
+__global__ void add_bf16_8(const uint16_t* __restrict__ a, int acs, const uint16_t* __restrict__ b, int bcs,
+                           uint16_t* __restrict__ out, int ocs, long long nvox, int c) {
+    const int c8 = c / 8;
+    const long long n = nvox * c8;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long v = i / c8;
+        const int ch = (int)(i % c8) * 8;
+        const Frag fa = load8(a + v * acs + ch), fb = load8(b + v * bcs + ch);
+        Frag r;
+#pragma unroll
+        for (int j = 0; j < 8; j++) r.h[j] = f2bf(bf2f(fa.h[j]) + bf2f(fb.h[j]));
+        store8(out + v * ocs + ch, r);
     }
 }
 
@@ -545,6 +679,12 @@ static int dn_ret() {
 
 static bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+/* 8 channels per thread: channel count and strides multiples of 8, both
+ * pointers 16-byte aligned */
+static bool vec8(int c, int s1, int s2, const void* p1, const void* p2) {
+    return c % 8 == 0 && s1 % 8 == 0 && s2 % 8 == 0 && al16(p1) && al16(p2);
+}
+
 extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mcs, const void* w, int kpad, const float* bias,
                                 void* y, int ycs, int nb, int d, int h, int wd, int cin, int cout, int relu, void* stream) {
     if (!x || !w || !y || nb <= 0 || d <= 0 || h <= 0 || wd <= 0) return -22;
@@ -616,7 +756,9 @@ extern "C" int mcx_dn_wgrad_reduce(const float* partial, int nsplit, float* dw, 
 /* part must hold nsplit*cout floats; db receives cout sums */
 extern "C" int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int mcs, float* part, int nsplit, float* db,
                                 long long nvox, int cout, void* stream) {
-    if (!dy || !part || !db || nsplit <= 0 || nsplit > 65535 || cout <= 0 || cout > 256 || 256 % cout || nvox <= 0) return -22;
+    if (!dy || !part || !db || nsplit <= 0 || nsplit > 65535 || cout <= 0 || cout > 256 || cout % 8 || 256 % (cout / 8) ||
+        nvox <= 0 || dycs % 8 || !al16(dy) || (mask && (mcs % 8 || !al16(mask))))
+        return -22;
     const int vps = (int)((nvox + nsplit - 1) / nsplit);
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(bias_grad, dim3(nsplit), dim3(256), 0, st, (const uint16_t*)dy, dycs, (const uint16_t*)mask, mcs, part, nvox,
@@ -629,8 +771,12 @@ extern "C" int mcx_dn_maxpool_fwd(const void* x, int xcs, void* y, int ycs, unsi
                                   int c, void* stream) {
     if (!x || !y || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || xcs < c || ycs < c) return -22;
     const long long n = (long long)nb * (d / 2) * (h / 2) * (wd / 2) * c;
-    hipLaunchKernelGGL(maxpool_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
-                       ycs, (uint8_t*)arg, nb, d, h, wd, c);
+    if (vec8(c, xcs, ycs, x, y) && ((uintptr_t)arg & 7u) == 0)
+        hipLaunchKernelGGL(maxpool_fwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs,
+                           (uint16_t*)y, ycs, (uint8_t*)arg, nb, d, h, wd, c);
+    else
+        hipLaunchKernelGGL(maxpool_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
+                           ycs, (uint8_t*)arg, nb, d, h, wd, c);
     return dn_ret();
 }
 
@@ -638,8 +784,12 @@ extern "C" int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char*
                                   int wd, int c, void* stream) {
     if (!dy || !dx || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || dycs < c || dxcs < c) return -22;
     const long long n = (long long)nb * d * h * wd * c;
-    hipLaunchKernelGGL(maxpool_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
-                       (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    if (vec8(c, dycs, dxcs, dy, dx) && ((uintptr_t)arg & 7u) == 0)
+        hipLaunchKernelGGL(maxpool_bwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                           (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    else
+        hipLaunchKernelGGL(maxpool_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                           (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
     return dn_ret();
 }
 
@@ -647,23 +797,35 @@ extern "C" int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char*
 extern "C" int mcx_dn_upsample_fwd(const void* x, int xcs, void* y, int ycs, int nb, int d, int h, int wd, int c, void* stream) {
     if (!x || !y || c <= 0 || xcs < c || ycs < c) return -22;
     const long long n = (long long)nb * 8 * d * h * wd * c;
-    hipLaunchKernelGGL(upsample_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
-                       ycs, nb, d, h, wd, c);
+    if (vec8(c, xcs, ycs, x, y))
+        hipLaunchKernelGGL(upsample_fwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs,
+                           (uint16_t*)y, ycs, nb, d, h, wd, c);
+    else
+        hipLaunchKernelGGL(upsample_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
+                           ycs, nb, d, h, wd, c);
     return dn_ret();
 }
 
 extern "C" int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs, int nb, int d, int h, int wd, int c, void* stream) {
     if (!dy || !dx || c <= 0 || dycs < c || dxcs < c) return -22;
     const long long n = (long long)nb * d * h * wd * c;
-    hipLaunchKernelGGL(upsample_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
-                       (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    if (vec8(c, dycs, dxcs, dy, dx))
+        hipLaunchKernelGGL(upsample_bwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                           (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+    else
+        hipLaunchKernelGGL(upsample_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
+                           (uint16_t*)dx, dxcs, nb, d, h, wd, c);
     return dn_ret();
 }
 
 extern "C" int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* out, int ocs, long long nvox, int c, void* stream) {
     if (!a || !b || !out || c <= 0 || nvox <= 0) return -22;
-    hipLaunchKernelGGL(add_bf16, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
-                       (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
+    if (vec8(c, acs, bcs, a, b) && ocs % 8 == 0 && al16(out))
+        hipLaunchKernelGGL(add_bf16_8, dim3(dn_grid(nvox * c / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
+                           (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
+    else
+        hipLaunchKernelGGL(add_bf16, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
+                           (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
     return dn_ret();
 }
 

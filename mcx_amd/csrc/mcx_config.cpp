@@ -405,10 +405,58 @@ void mcx_preprocess(Config* cfg) {
     }
 }
 
+/* src/mcx_utils.c:3624-3699: select the detected photons to replay (all, or
+ * those of detector replaydet), compacting their seeds, and derive each
+ * one's detected weight exp(-sum mua_m L_m) and time of flight
+ * sum n_m L_m / c0 from its partial paths; photons whose time of flight lies
+ * outside [tstart, tend] are dropped.  detps holds one record of
+ * dimdetps[0] floats per photon (dimdetps[1] photons). */
+static void replay_init(Config* cfg, const float* detps, const int* dimdetps, int seedbyte) {
+    if (cfg->seed == MCX_SEED_FROM_FILE && detps == NULL)
+        MCX_THROW(-6, "you give cfg.seed for replay, but did not specify cfg.detphotons.\nPlease define it as the detphoton output from the baseline simulation\n");
+    if (detps == NULL || cfg->seed != MCX_SEED_FROM_FILE) return;
+    if (!dimdetps || cfg->nphoton != (size_t)dimdetps[1]) MCX_THROW(-6, "the column numbers of detphotons and seed do not match\n");
+    if (seedbyte == 0 || !cfg->replay.seed) MCX_THROW(-6, "the seed input is empty");
+    if (seedbyte != 16) MCX_THROW(-6, "the seed byte-length does not match this engine's RNG (16 bytes)");
+    const int hasdetid = cfg->savedetflag & 1;
+    const int offset = ((cfg->savedetflag >> 1) & 1) * (int)(cfg->medianum - 1);
+    if ((!hasdetid && cfg->detnum > 1) || !((cfg->savedetflag >> 2) & 1))
+        MCX_THROW(-6, "please rerun the baseline simulation and save detector ID (D) and partial-path (P) using cfg.savedetflag='dp' ");
+    const int n = dimdetps[1], rl = dimdetps[0];
+    if (rl < hasdetid + offset + (int)cfg->medianum - 1) MCX_THROW(-6, "detphotons records are shorter than savedetflag implies");
+    if (cfg->replay.weight) free(cfg->replay.weight);
+    if (cfg->replay.tof) free(cfg->replay.tof);
+    if (cfg->replay.detid) free(cfg->replay.detid);
+    cfg->replay.weight = (float*)malloc(std::max(n, 1) * sizeof(float));
+    cfg->replay.tof = (float*)calloc(std::max(n, 1), sizeof(float));
+    cfg->replay.detid = (int*)calloc(std::max(n, 1), sizeof(int));
+    if (!cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid) MCX_THROW(-1, "out of memory for replay data");
+    char* seeds = (char*)cfg->replay.seed;
+    size_t kept = 0;
+    for (int i = 0; i < n; i++) {
+        const float* rec = detps + (size_t)i * rl;
+        const int det = hasdetid ? (int)rec[0] : 1;
+        if (cfg->replaydet > 0 && cfg->replaydet != (det & 0xFFFF)) continue;
+        if (cfg->srcid > 0 && !((det & 0xFFFF0000) && cfg->srcid == ((det & 0xFFFF0000) >> 16))) continue;
+        float w = 1.f, tof = 0.f;
+        for (unsigned int m = 0; m + 1 < cfg->medianum; m++) {
+            float plen = rec[hasdetid + offset + m];
+            w *= expf(-cfg->prop[m + 1].mua * plen);
+            plen *= cfg->unitinmm;
+            tof += plen * MCX_R_C0 * cfg->prop[m + 1].n;
+        }
+        if (tof < cfg->tstart || tof > cfg->tend) continue;
+        if ((size_t)i != kept) memmove(seeds + kept * seedbyte, seeds + (size_t)i * seedbyte, seedbyte);
+        cfg->replay.weight[kept] = w;
+        cfg->replay.tof[kept] = tof;
+        cfg->replay.detid[kept] = det;
+        kept++;
+    }
+    cfg->nphoton = kept;
+    cfg->his.seedbyte = seedbyte;
+}
+
 void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
-    (void)detps;
-    (void)dimdetps;
-    (void)seedbyte;
     EntryGuard g;
     try {
         unsigned int hostdetreclen = det_reclen(cfg);
@@ -427,7 +475,6 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         unsigned int gates = (unsigned int)((cfg->tend - cfg->tstart) / cfg->tstep + 0.5);
         if (cfg->maxgate > gates) cfg->maxgate = gates;
         if (cfg->seed < 0 && cfg->seed != MCX_SEED_FROM_FILE) cfg->seed = (int)time(NULL);
-        if (cfg->seed == MCX_SEED_FROM_FILE) MCX_THROW(-6, "photon replay is not supported by this engine");
         for (unsigned int i = 0; i < cfg->detnum; i++) {
             if (!cfg->issrcfrom0) {
                 cfg->detpos[i].x--;
@@ -443,6 +490,8 @@ void mcx_validatecfg(Config* cfg, float* detps, int dimdetps[2], int seedbyte) {
         cfg->his.colcount = det_reclen(cfg);
         (void)hostdetreclen;
         cfg->his.savedetflag = cfg->savedetflag;
+        if (cfg->issaveseed) cfg->his.seedbyte = 16;
+        replay_init(cfg, detps, dimdetps, seedbyte);
     } catch (const Error& e) {
         handle_error(e, cfg);
     }

@@ -56,15 +56,20 @@ static bool supported_source(int t) {
 void check_supported(const Config* cfg) {
     if (cfg->polmedianum > 0) MCX_THROW(-1, "polarized photon simulation is not supported by this engine");
     if (cfg->mediabyte > 4) MCX_THROW(-1, "only label-based media formats are supported by this engine");
-    if (cfg->seed == MCX_SEED_FROM_FILE) MCX_THROW(-1, "photon replay is not supported by this engine");
+    const bool replay = cfg->seed == MCX_SEED_FROM_FILE;
+    if (replay && (!cfg->replay.seed || !cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid))
+        MCX_THROW(-6, "photon replay needs the saved seeds and the detected-photon data (run mcx_validatecfg with detphotons)");
     if (cfg->nphase || cfg->nangle) MCX_THROW(-1, "user-defined phase/angle inverse CDFs are not supported by this engine");
     if (cfg->issaveref) MCX_THROW(-1, "diffuse reflectance output (issaveref) is not supported by this engine");
     if (cfg->extrasrclen) MCX_THROW(-1, "multiple sources are not supported by this engine");
-    if (cfg->issaveseed) MCX_THROW(-1, "saving photon seeds is not supported by this engine");
     if (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))
         MCX_THROW(-1, "trajectory output (-D M) is not supported by this engine");
     if (!supported_source(cfg->srctype)) MCX_THROW(-1, "the specified source type is not supported by this engine");
-    if (cfg->outputtype != otFlux && cfg->outputtype != otFluence && cfg->outputtype != otEnergy && cfg->outputtype != otL)
+    const bool replaytype = cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS ||
+                            cfg->outputtype == otWLTOF || cfg->outputtype == otWPTOF;
+    if (replaytype && !replay) MCX_THROW(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
+    if (cfg->outputtype != otFlux && cfg->outputtype != otFluence && cfg->outputtype != otEnergy && cfg->outputtype != otL &&
+        !replaytype)
         MCX_THROW(-1, "the specified output type is not supported by this engine");
 }
 
@@ -218,7 +223,11 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         cfg->isnormalized = 0;
     }
 
-    const size_t fieldlen = nvox * maxgate;
+    /* per-detector replay tallies: one block of gates per detector */
+    const bool replay = cfg->seed == MCX_SEED_FROM_FILE;
+    const unsigned int blocks = (replay && cfg->replaydet == -1) ? std::max(1u, cfg->detnum) : 1u;
+    if (replay && totalgates > maxgate) MCX_THROW(-1, "photon replay needs all time gates in one pass (reduce the gate count)");
+    const size_t fieldlen = nvox * maxgate * blocks;
     if (cfg->exportfield == NULL && cfg->issave2pt) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
     if (cfg->exportdetected == NULL) cfg->exportdetected = (float*)malloc(std::max<size_t>(1, (size_t)hostdetreclen * cfg->maxdetphoton) * sizeof(float));
 
@@ -242,6 +251,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
 
     Stream st;
     DevMem dkp, dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
+    DevMem dsaved, drseed, drweight, drtof, drdetid;
     std::vector<unsigned char> m8;
     if (media32) {
         dmedia.alloc(nvox * 4);
@@ -276,6 +286,19 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
     dcount.alloc(4 * sizeof(uint32_t));
     dcounters.alloc(4 * sizeof(uint64_t));
+    const bool saveseed = savedet && cfg->issaveseed;
+    if (saveseed) dsaved.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton) * 16);
+    if (replay) {
+        /* one saved state, weight, time of flight and detector id per replayed photon */
+        drseed.alloc(std::max<size_t>(1, gpuphoton) * 16);
+        drweight.alloc(std::max<size_t>(1, gpuphoton) * sizeof(float));
+        drtof.alloc(std::max<size_t>(1, gpuphoton) * sizeof(float));
+        drdetid.alloc(std::max<size_t>(1, gpuphoton) * sizeof(int));
+        HIP_CHECK(hipMemcpyAsync(drseed.p, cfg->replay.seed, gpuphoton * 16, hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(drweight.p, cfg->replay.weight, gpuphoton * sizeof(float), hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(drtof.p, cfg->replay.tof, gpuphoton * sizeof(float), hipMemcpyHostToDevice, st.s));
+        HIP_CHECK(hipMemcpyAsync(drdetid.p, cfg->replay.detid, gpuphoton * sizeof(int), hipMemcpyHostToDevice, st.s));
+    }
     dkp.alloc(sizeof(KParam));
     HIP_CHECK(hipMemsetAsync(dcounters.p, 0, dcounters.bytes, st.s));
 
@@ -295,10 +318,19 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.detdata = ddet.as<float>();
     kp.detcount = dcount.as<uint32_t>();
     kp.counters = g_counters ? dcounters.as<uint64_t>() : nullptr;
+    kp.saveseed = saveseed ? 1u : 0u;
+    kp.seeddata = dsaved.as<uint32_t>();
+    kp.replay = replay ? 1 : 0;
+    kp.replaydet = cfg->replaydet;
+    kp.replayseed = drseed.as<uint32_t>();
+    kp.replayweight = drweight.as<float>();
+    kp.replaytof = drtof.as<float>();
+    kp.replaydetid = drdetid.as<int32_t>();
 
     const unsigned int block = (unsigned int)G.autoblock;
     const unsigned int grid = nthread / block;
-    const size_t shmem = (size_t)ntable * 16 + (size_t)block * partialdata * sizeof(float);
+    /* per lane in LDS: the partial paths, then (seed saving) the launch RNG state */
+    const size_t shmem = (size_t)ntable * 16 + (size_t)block * (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
     if (shmem > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
 
     if (cfg->flog && cfg->printnum >= 0) {
@@ -370,6 +402,13 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
                     if (detected)
                         HIP_CHECK(hipMemcpyAsync(cfg->exportdetected + cfg->detectedcount * hostdetreclen, ddet.p,
                                                  (size_t)detected * hostdetreclen * sizeof(float), hipMemcpyDeviceToHost, st.s));
+                    if (saveseed) {
+                        cfg->seeddata = realloc(cfg->seeddata, std::max<size_t>(1, (cfg->detectedcount + detected) * (size_t)16));
+                        if (!cfg->seeddata) MCX_THROW(-1, "out of host memory for photon seeds");
+                        if (detected)
+                            HIP_CHECK(hipMemcpyAsync((char*)cfg->seeddata + cfg->detectedcount * 16, dsaved.p, (size_t)detected * 16,
+                                                     hipMemcpyDeviceToHost, st.s));
+                    }
                     cfg->detectedcount += detected;
                 }
             }
@@ -433,10 +472,42 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         } else if (cfg->outputtype == otEnergy || cfg->outputtype == otL) {
             scale = (float)(1.f / cfg->energytot);
         }
+        bool done = false;
+        if (replay) {
+            /* src/mcx_core.cu:3842-3889: replay tallies are divided by the
+             * summed detected weight (per detector with replaydet -1); path
+             * tallies (Jacobian, WL x TOF) are in grid units and also scaled
+             * by unitinmm */
+            const bool pathtype = cfg->outputtype == otJacobian || cfg->outputtype == otWLTOF;
+            if (cfg->replaydet == -1) {
+                for (unsigned int det = 1; det <= cfg->detnum; det++) {
+                    if (cfg->isnormalized != 2) {
+                        scale = 0.f;
+                        for (size_t i = 0; i < gpuphoton; i++)
+                            if ((unsigned int)(cfg->replay.detid[i] & 0xFFFF) == det) scale += cfg->replay.weight[i];
+                        if (scale > 0.f) {
+                            scale = 1.0f / scale;
+                            if (pathtype) scale = cfg->unitinmm * scale;
+                        }
+                    } else if (pathtype) {
+                        scale = cfg->unitinmm;
+                    }
+                    if (cfg->flog) fprintf(cfg->flog, "normalization factor for detector %u alpha=%f\n", det, scale);
+                    mcx_normalize(cfg->exportfield + (size_t)(det - 1) * nvox * maxgate, scale, nvox * maxgate, cfg->isnormalized, 0, 1);
+                }
+                done = true;
+            } else {
+                scale = 0.f;
+                for (size_t i = 0; i < gpuphoton; i++) scale += cfg->replay.weight[i];
+                if (scale > 0.f) scale = cfg->unitinmm / scale;
+            }
+        }
         cfg->normalizer = scale;
         cfg->his.normalizer = scale;
-        if (cfg->flog) fprintf(cfg->flog, "source 1, normalization factor alpha=%f\n", scale);
-        mcx_normalize(cfg->exportfield, scale, fieldlen, cfg->isnormalized, 0, 1);
+        if (!done) {
+            if (cfg->flog) fprintf(cfg->flog, "source 1, normalization factor alpha=%f\n", scale);
+            mcx_normalize(cfg->exportfield, scale, fieldlen, cfg->isnormalized, 0, 1);
+        }
     }
     if (cfg->parentid == mpStandalone) {
         if (cfg->issave2pt) mcx_savedata(cfg->exportfield, fieldlen, cfg);
@@ -445,7 +516,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             cfg->his.savedphoton = (unsigned int)cfg->detectedcount;
             cfg->his.totalphoton = (unsigned int)cfg->nphoton;
             cfg->his.detected = (unsigned int)cfg->detectedcount;
-            mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, 0, cfg);
+            mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, saveseed ? 16 : 0, cfg);
         }
     }
     if (cfg->flog) {

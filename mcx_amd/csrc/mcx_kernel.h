@@ -55,6 +55,15 @@ struct KParam {
     float* detdata;               /* maxdetphoton*reclen */
     uint32_t* detcount;           /* [0]=detected, [1]=error exits, [2]=lanes done */
     uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
+    /* seed saving and replay */
+    uint32_t saveseed;            /* store each detected photon's launch RNG state */
+    uint32_t* seeddata;           /* 4 words per detected-photon slot */
+    int32_t replay;               /* photons relaunched from saved RNG states */
+    int32_t replaydet;            /* 0: all detectors, -1: one tally block per detector, k: detector k */
+    const uint32_t* replayseed;   /* 4 words per replayed photon (u64 s0, s1, little endian) */
+    const float* replayweight;    /* detected weight of each replayed photon */
+    const float* replaytof;       /* time of flight of each replayed photon */
+    const int32_t* replaydetid;   /* detector id of each replayed photon */
 };
 
 /* One instance of a batched launch (its launch constants are in KParam) */

@@ -38,7 +38,7 @@ constexpr uint32_t OUT_MAX = 0x7FFFFFFFu;
 constexpr float MAX_ACCUM = 1000.f;
 constexpr float ROULETTE = 10.f;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
-enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL };
+enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
 struct F4 { float x, y, z, w; };
 
@@ -115,6 +115,7 @@ struct Lane {
     size_t pend_i;
     float pend_old, pend_w;
     double* tally64;      /* FP64 modes: this lane's tally copy */
+    uint32_t gid;         /* global index of the current photon (replay) */
     uint32_t nstep, ndep, nscat, nlaunch;
 
     __device__ __forceinline__ Lane(const KParam& k, const F4* tb, float* ppb, uint32_t s, uint32_t id)
@@ -262,6 +263,10 @@ struct Lane {
             if ((flag >> 4) & 1) { rec[k++] = p.x; rec[k++] = p.y; rec[k++] = p.z; }
             if ((flag >> 5) & 1) { rec[k++] = v.x; rec[k++] = v.y; rec[k++] = v.z; }
             if ((flag >> 6) & 1) rec[k++] = winit;
+            if (kp.saveseed) {
+                MCX_GLB uint32_t* sd = glb(kp.seeddata) + (size_t)slot * 4;
+                for (uint32_t i = 0; i < 4; i++) sd[i] = mcx_f2bits(ppath(kp.partialdata + i));
+            }
         } else if (kp.savedet == 3) {
             atom_add_u32(kp.detcount, ~0u);
         }
@@ -451,6 +456,21 @@ struct Lane {
             for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;
         }
         if ((int)f.w >= nphoton - 1) return 1;
+        /* src/mcx_core.cu:1205-1215: a replayed photon restarts from its
+         * saved RNG state; with seed saving the launch state is kept (in
+         * this lane's LDS column after the partial paths) until detection */
+        gid = idx * (uint32_t)kp.threadphoton + min(idx, (uint32_t)kp.oddphotons) + (uint32_t)((int)f.w + 1);
+        if (kp.replay) {
+            const MCX_GLB uint32_t* rs = glb(kp.replayseed) + (size_t)gid * 4;
+            t0 = ((uint64_t)rs[1] << 32) | rs[0];
+            t1 = ((uint64_t)rs[3] << 32) | rs[2];
+        }
+        if (kp.saveseed) {
+            ppath(kp.partialdata + 0) = mcx_bits2f((uint32_t)t0);
+            ppath(kp.partialdata + 1) = mcx_bits2f((uint32_t)(t0 >> 32));
+            ppath(kp.partialdata + 2) = mcx_bits2f((uint32_t)t1);
+            ppath(kp.partialdata + 3) = mcx_bits2f((uint32_t)(t1 >> 32));
+        }
         do {
             p = {kp.srcpos[0], kp.srcpos[1], kp.srcpos[2], kp.srcpos[3]};
             v = {kp.srcdir[0], kp.srcdir[1], kp.srcdir[2], kp.srcdir[3]};
@@ -612,6 +632,18 @@ struct Lane {
                     v.w += 1.f;
                     calc_rv();
                     nscat++;
+                    /* src/mcx_core.cu:2019-2047: replayed photons tally their
+                     * detected weight at every scattering event */
+                    if (kp.replay && (kp.outputtype == otWP || kp.outputtype == otDCS || kp.outputtype == otWPTOF)) {
+                        const float tof = glb(kp.replaytof)[gid];
+                        int ts = (int)floorf((tof - kp.twin0) * kp.Rtstep);
+                        ts = max(0, min((int)kp.maxgate - 1, ts));
+                        if (kp.replaydet == -1) ts += ((glb(kp.replaydetid)[gid] & 0xFFFF) - 1) * (int)kp.maxgate;
+                        float w = (kp.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
+                        w = (kp.outputtype == otWPTOF) ? tof : w;
+                        w *= glb(kp.replayweight)[gid];
+                        deposit(idx1d, ts, w);
+                    }
                 }
                 v.w = (float)(int)v.w;
             }
@@ -652,7 +684,16 @@ struct Lane {
                         weight = w0 - p.w;
                     else if (kp.outputtype == otFluence || kp.outputtype == otFlux)
                         weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
-                    else if (kp.outputtype == otL)
+                    else if (kp.replay && (kp.outputtype == otJacobian || kp.outputtype == otWLTOF)) {
+                        /* src/mcx_core.cu:2208-2220 (the gate index is clamped
+                         * to the gate range, as the scattering tally clamps it
+                         * to the last gate) */
+                        const float tof = glb(kp.replaytof)[gid];
+                        weight = glb(kp.replayweight)[gid] * f.z;
+                        if (kp.outputtype == otWLTOF) weight = weight * tof;
+                        tshift = max(0, min((int)kp.maxgate - 1, (int)floorf((tof - kp.twin0) * kp.Rtstep)));
+                        if (kp.replaydet == -1) tshift += ((glb(kp.replaydetid)[gid] & 0xFFFF) - 1) * (int)kp.maxgate;
+                    } else if (kp.outputtype == otL)
                         weight = w0 * f.z;
                     if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
                 }

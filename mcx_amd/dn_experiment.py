@@ -141,13 +141,40 @@ def denoise_maps(net, feats, flip_z=False):
     return out
 
 
-def denoise_once(net, phi, cfg):
-    """Denoise one CW map of the head volume entirely on the GPU (features,
-    U-Net, inverse transform); the source enters from +z, so the volume is
-    mirrored in z as in training."""
-    f = denoise.features_gpu(phi, cfg["vol"], cfg["prop"], float(cfg.get("unitinmm", 1.0)), flip_z=True)
-    pred = net.predict(f)
-    return denoise.inverse_gpu(pred, cfg["vol"].shape, flip_z=True)
+class HeadDenoiser:
+    """The denoised pipeline on one volume with everything resident on the
+    GPU: feature kernel, U-Net, inverse kernel (the volume is mirrored in z
+    because its source enters from +z)."""
+
+    def __init__(self, net, cfg):
+        import torch
+        self.net = net
+        self.shape = cfg["vol"].shape
+        self.vol = torch.from_numpy(np.asfortranarray(cfg["vol"], dtype=np.uint8).ravel(order="F")).cuda()
+        self.tab = torch.from_numpy(denoise._media_table(cfg["prop"], float(cfg.get("unitinmm", 1.0)))).cuda()
+
+    def upload(self, phi):
+        import torch
+        return torch.from_numpy(np.asfortranarray(phi, dtype=np.float32).ravel(order="F")).cuda()
+
+    def run(self, dphi):
+        """device fluence (x-fastest) -> device denoised fluence"""
+        import torch
+        X, Y, Z = self.shape
+        f = torch.empty((1, X, Y, Z, denoise.NFEAT), dtype=torch.bfloat16, device="cuda")
+        L = self.net.L
+        st = self.net._stream()
+        denoise._chk(L.mcx_dn_features(denoise._ptr(dphi), denoise._ptr(self.vol), denoise._ptr(self.tab),
+                                       int(self.tab.shape[0]), denoise._ptr(f), X, Y, Z, 1, denoise.LOG_EPS,
+                                       denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "features")
+        pred = self.net.predict(f)
+        out = torch.empty(X * Y * Z, dtype=torch.float32, device="cuda")
+        denoise._chk(L.mcx_dn_inv_features(denoise._ptr(pred), denoise._ptr(out), X, Y, Z, 1, denoise.LOG_EPS,
+                                           denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "inv_features")
+        return out
+
+    def download(self, dout):
+        return dout.cpu().numpy().reshape(self.shape, order="F")
 
 
 def train(net, X, T, steps, batch=8, lr=1e-3, seed=2024, log=None):
@@ -207,6 +234,81 @@ def timing(nb=8, S=64, steps=20, infer_S=256):
     return out
 
 
+def head_eval(net, args, minenergy, log):
+    """Plain runs of increasing photon count and the denoised low-photon run
+    on the head volume, against K high-photon references."""
+    import torch
+    import mcx_amd
+    hc = workloads.head256(nphoton=int(args.head_low), n=args.head, quiet=True, minenergy=minenergy)
+    hc["autopilot"] = 1
+
+    def plain(nph, seed):
+        c = dict(hc)
+        c["nphoton"] = int(nph)
+        c["seed"] = seed
+        r = mcx_amd.run(c)
+        st = mcx_amd.last_stats()
+        return cw_fluence([r], [c])[0], st["kernel_ms"]
+
+    hrefs, href_ms = [], 0.0
+    for k in range(args.refruns):
+        m, ms = plain(args.head_ref, 1000 + k)
+        hrefs.append(m)
+        href_ms += ms
+    log("head (minenergy %g) references: %d x %.0e photons, %.0f ms each" % (minenergy, args.refruns, args.head_ref,
+                                                                          href_ms / args.refruns))
+    ref, mask = reliable_mask(hrefs)
+    ref_loo, mask_loo = reliable_mask(hrefs[1:])
+    e_ref1 = median_error(hrefs[0], ref_loo, mask_loo & mask)
+    plain_rows = []
+    for nph, seed in ((args.head_low, 7), (args.head_low * 10, 8)):
+        m, ms = plain(nph, seed)
+        plain_rows.append({"photons": nph, "kernel_ms": ms, "error": median_error(m, ref, mask)})
+    plain_rows.append({"photons": args.head_ref, "kernel_ms": href_ms / args.refruns, "error": e_ref1})
+    dn = HeadDenoiser(net, hc)
+    den_rows = []
+    for k, nph in enumerate((args.head_low, 2 * args.head_low, 4 * args.head_low)):
+        low_map, low_ms_h = plain(nph, 11 + k)
+        dphi = dn.upload(low_map)
+        dn.run(dphi)  # warm-up (buffer allocation)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dout = dn.run(dphi)
+        torch.cuda.synchronize()
+        dn_ms = (time.perf_counter() - t0) * 1e3
+        dmap = dn.download(dout)
+        den_rows.append({"photons": nph, "kernel_ms": low_ms_h, "denoise_ms": dn_ms,
+                         "low_error": median_error(low_map, ref, mask), "denoised_error": median_error(dmap, ref, mask)})
+    # denoised arm: photons for 5 % by log-log interpolation/extrapolation of
+    # its error, time by interpolation of (kernel + denoise) time
+    dph = np.array([r["photons"] for r in den_rows])
+    der = np.array([r["denoised_error"] for r in den_rows])
+    dms = np.array([r["kernel_ms"] + r["denoise_ms"] for r in den_rows])
+    slope, icpt = np.polyfit(np.log(dph), np.log(der), 1)
+    dn5 = float(np.exp((np.log(0.05) - icpt) / slope)) if slope < 0 else float("inf")
+    dms5 = float(np.exp(np.interp(np.log(dn5), np.log(dph), np.log(dms)))) if np.isfinite(dn5) else float("inf")
+    if np.isfinite(dn5) and dn5 > dph[-1]:
+        # beyond the measured counts: scale the last time linearly in photons
+        dms5 = float(dms[-1] * dn5 / dph[-1])
+    # plain: error ~ c / sqrt(N) fitted over the plain runs; kernel time at
+    # that photon count by log-log interpolation of the measured runs (small
+    # runs are bounded by the longest photon histories, not the count)
+    ph = np.array([r["photons"] for r in plain_rows])
+    er = np.array([r["error"] for r in plain_rows])
+    kms = np.array([r["kernel_ms"] for r in plain_rows])
+    cfit = float(np.exp(np.mean(np.log(er) + 0.5 * np.log(ph))))
+    n5 = (cfit / 0.05) ** 2
+    order = np.argsort(ph)
+    ms5 = float(np.exp(np.interp(np.log(n5), np.log(ph[order]), np.log(kms[order]))))
+    out = {"minenergy": minenergy, "edge": args.head, "reliable_voxels": int(mask.sum()),
+           "ref_photons": args.head_ref * args.refruns, "plain": plain_rows,
+           "plain_photons_for_5pct": n5, "plain_ms_to_5pct": ms5, "denoised": den_rows,
+           "denoised_photons_for_5pct": dn5, "denoised_ms_to_5pct": dms5,
+           "ratio_denoised_over_plain": dms5 / ms5}
+    log("head: %s" % json.dumps(out))
+    return out
+
+
 def main(argv=None):
     if argv is None and len(sys.argv) > 1 and sys.argv[1] == "--timing":
         timing()
@@ -228,6 +330,9 @@ def main(argv=None):
     ap.add_argument("--head-low", type=float, default=1e6)
     ap.add_argument("--head-ref", type=float, default=1e8, help="photons per head reference run (x --refruns)")
     ap.add_argument("--weights", default=None, help="save the trained parameters (npz)")
+    ap.add_argument("--load-weights", default=None, help="skip training: load parameters saved by --weights")
+    ap.add_argument("--head-minenergy", default="0", help="comma list of roulette thresholds for the head runs "
+                                                          "(applied to the plain and the denoised arm alike)")
     args = ap.parse_args(argv)
     import torch
 
@@ -236,10 +341,18 @@ def main(argv=None):
 
     R = {"args": vars(args)}
     t_all = time.perf_counter()
+    if args.load_weights:
+        net = denoise.UNet(seed=2024)
+        net.load_state(np.load(args.load_weights)["p"])
+        R["params"] = net.count
+        R["train"] = {"loaded": args.load_weights}
+        args.ntrain = 0
     # ---- training pairs --------------------------------------------------
     nhead = int(round(args.ntrain * args.head_frac))
     tr = [training_volume(2024, i, args.edge) for i in range(args.ntrain - nhead)]
     tr += [head_crop_volume(5000 + i, 2 * args.edge, args.edge) for i in range(nhead)]
+    if not tr:
+        return _evaluate(net, args, R, log, t_all)
     # input photon counts log-uniform in [low/3, low*10]: the network sees a
     # range of noise levels
     gl = np.random.default_rng(2024)
@@ -259,6 +372,10 @@ def main(argv=None):
     del X, T
     if args.weights:
         np.savez_compressed(args.weights, p=net.state())
+    return _evaluate(net, args, R, log, t_all)
+
+
+def _evaluate(net, args, R, log, t_all):
     # ---- held-out volumes ----------------------------------------------------
     ev = [training_volume(4048, i, args.edge) for i in range(args.neval)]
     refs = [simulate(ev, args.ref, 100 + k)[0] for k in range(args.refruns)]
@@ -281,60 +398,9 @@ def main(argv=None):
     log("held-out: %s" % json.dumps(R["heldout"]["median"]))
     # ---- head volume -------------------------------------------------------
     if args.head:
-        import mcx_amd
-        hc = workloads.head256(nphoton=int(args.head_low), n=args.head, quiet=True)
-        hc["autopilot"] = 1
-
-        def plain(nph, seed):
-            c = dict(hc)
-            c["nphoton"] = int(nph)
-            c["seed"] = seed
-            r = mcx_amd.run(c)
-            st = mcx_amd.last_stats()
-            return cw_fluence([r], [c])[0], st["kernel_ms"]
-
-        hrefs, href_ms = [], 0.0
-        for k in range(args.refruns):
-            m, ms = plain(args.head_ref, 1000 + k)
-            hrefs.append(m)
-            href_ms += ms
-            log("head ref run %d: %.0f ms" % (k, ms))
-        ref, mask = reliable_mask(hrefs)
-        # leave-one-out reference for the error of one reference-sized run
-        ref_loo, mask_loo = reliable_mask(hrefs[1:])
-        e_ref1 = median_error(hrefs[0], ref_loo, mask_loo & mask)
-        plain_rows = []
-        for nph, seed in ((args.head_low, 7), (args.head_low * 10, 8)):
-            m, ms = plain(nph, seed)
-            plain_rows.append({"photons": nph, "kernel_ms": ms, "error": median_error(m, ref, mask)})
-        plain_rows.append({"photons": args.head_ref, "kernel_ms": href_ms / args.refruns, "error": e_ref1})
-        low_map, low_ms_h = plain(args.head_low, 11)
-        torch.cuda.synchronize()
-        denoise_once(net, low_map, hc)  # warm-up (buffer allocation)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        dmap = denoise_once(net, low_map, hc)
-        dn_s = time.perf_counter() - t0
-        e_low = median_error(low_map, ref, mask)
-        e_den = median_error(dmap, ref, mask)
-        # plain: error ~ c / sqrt(N) fitted over the plain runs
-        ph = np.array([r["photons"] for r in plain_rows])
-        er = np.array([r["error"] for r in plain_rows])
-        cfit = float(np.exp(np.mean(np.log(er) + 0.5 * np.log(ph))))
-        n5 = (cfit / 0.05) ** 2
-        # kernel time at n5 photons: log-log interpolation of the measured runs
-        # (small runs are bounded by the longest photon histories, not by
-        # the photon count)
-        kms = np.array([r["kernel_ms"] for r in plain_rows])
-        order = np.argsort(ph)
-        ms5 = float(np.exp(np.interp(np.log(n5), np.log(ph[order]), np.log(kms[order]))))
-        rate = float(np.sum(ph) / np.sum(kms))
-        R["head"] = {"edge": args.head, "reliable_voxels": int(mask.sum()), "ref_photons": args.head_ref * args.refruns,
-                     "plain": plain_rows, "plain_photons_per_ms": rate, "plain_photons_for_5pct": n5,
-                     "plain_ms_to_5pct": ms5, "low_error": e_low, "denoised_error": e_den,
-                     "denoised_ms": low_ms_h + dn_s * 1e3, "denoise_inference_ms": dn_s * 1e3,
-                     "low_kernel_ms": low_ms_h}
-        log("head: %s" % json.dumps(R["head"]))
+        R["head"] = []
+        for me in [float(v) for v in args.head_minenergy.split(",")]:
+            R["head"].append(head_eval(net, args, me, log))
     R["seconds"] = time.perf_counter() - t_all
     with open(args.out, "w") as f:
         json.dump(R, f, indent=1)

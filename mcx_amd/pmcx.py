@@ -29,6 +29,8 @@ SRCTYPES = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fou
 OUTPUTTYPES = ["flux", "fluence", "energy", "jacobian", "nscat", "wl", "wp", "wm", "rf", "length", "rfmus", "wltof",
                "wptof"]
 SAVEFLAGS = "DSPMXVWI"
+SEEDBYTE = 16               # bytes of one saved xorshift128+ state
+MCX_SEED_FROM_FILE = -999
 DEBUGFLAGS = "RMP"
 
 _INT_FIELDS = ["nphoton", "nblocksize", "nthread", "maxdetphoton", "maxgate", "respin", "isreflect", "isref3",
@@ -189,14 +191,28 @@ def _parse_config(user_cfg, cfg):
         if not s or len(s) > 12:
             raise ValueError("the 'bc' field must be a non-empty string with at most 12 characters")
         cfg.bc = s.encode()
-    if "detphotons" in user_cfg:
-        raise ValueError("photon replay is not supported by this engine")
+    replay = None
     if "seed" in user_cfg:
         sv = user_cfg["seed"]
         if isinstance(sv, (int, np.integer, float, np.floating)):
             cfg.seed = int(sv)
         else:
-            raise ValueError("photon replay (array seeds) is not supported by this engine")
+            # src/pmcx.cpp:934-952: a [16, n] uint8 array of saved RNG states
+            seeds = np.asfortranarray(np.asarray(sv, dtype=np.uint8))
+            if seeds.ndim != 2 or seeds.shape[0] != SEEDBYTE:
+                raise ValueError("the row number of cfg.seed does not match RNG seed byte-length")
+            buf = seeds.ravel(order="F").copy()
+            cfg.replay.seed = ctypes.cast(_copy_to_c(buf, ctypes.c_ubyte), ctypes.c_void_p).value
+            cfg.seed = MCX_SEED_FROM_FILE
+            cfg.nphoton = seeds.shape[1]
+    if "detphotons" in user_cfg:
+        d = user_cfg["detphotons"]
+        if isinstance(d, dict):
+            d = d.get("data")
+        d = np.asfortranarray(np.asarray(d, dtype=np.float32))
+        if d.ndim != 2:
+            raise ValueError("the 'detphotons' field must be a 2D array [reclen, n]")
+        replay = d
     if "gpuid" in user_cfg:
         g = user_cfg["gpuid"]
         cfg.deviceid = b"\0" * 256
@@ -211,6 +227,7 @@ def _parse_config(user_cfg, cfg):
         w = np.asarray(user_cfg["workload"], dtype=np.float32).ravel()
         for i, x in enumerate(w[:256]):
             cfg.workload[i] = float(x)
+    return replay
 
 
 def _reclen(cfg):
@@ -228,20 +245,29 @@ def run(cfg=None, **kwargs):
     gpu = ctypes.POINTER(GPUInfo)()
     output = {}
     try:
-        _parse_config(user_cfg, c)
+        detps = _parse_config(user_cfg, c)
         ndev = L.mcx_list_gpu(ctypes.byref(c), ctypes.byref(gpu))
         _check()
         if not ndev:
             raise RuntimeError("No GPU device found")
-        L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+        if detps is not None:
+            # src/pmcx.cpp:1096: the baseline's detected photons drive the replay
+            dims = (ctypes.c_int * 2)(detps.shape[0], detps.shape[1])
+            flat = np.ascontiguousarray(detps.ravel(order="F"), dtype=np.float32)
+            L.mcx_validatecfg(ctypes.byref(c), flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), dims,
+                              SEEDBYTE if c.seed == MCX_SEED_FROM_FILE else 0)
+        else:
+            L.mcx_validatecfg(ctypes.byref(c), None, None, SEEDBYTE if c.seed == MCX_SEED_FROM_FILE else 0)
         _check()
         if not c.vol or c.medianum == 0:
             raise ValueError("You must define 'vol' and 'prop' field.")
         ngates = int((c.tend - c.tstart) / c.tstep + 0.5)
         nvox = c.dim.x * c.dim.y * c.dim.z
         reclen = _reclen(c)
+        # per-detector replay tallies carry one block of gates per detector
+        blocks = c.detnum if (c.seed == MCX_SEED_FROM_FILE and c.replaydet == -1 and c.detnum > 1) else 1
         if c.issave2pt:
-            c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates * c.srcnum, 4), ctypes.POINTER(ctypes.c_float))
+            c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates * c.srcnum * blocks, 4), ctypes.POINTER(ctypes.c_float))
         if c.issavedet:
             c.exportdetected = ctypes.cast(_cmalloc(reclen * c.maxdetphoton * 4), ctypes.POINTER(ctypes.c_float))
         if user_cfg.get("dumpmask", False):
@@ -254,9 +280,17 @@ def run(cfg=None, **kwargs):
             if n > 0:
                 d = np.ctypeslib.as_array(c.exportdetected, shape=(n * reclen,)).copy()
                 output["detp"] = np.asfortranarray(d.reshape((n, reclen)).T)
+        if c.issavedet and c.issaveseed:
+            n = int(c.detectedcount)
+            if n > 0 and c.seeddata:
+                raw = np.ctypeslib.as_array(ctypes.cast(c.seeddata, ctypes.POINTER(ctypes.c_ubyte)), shape=(n * SEEDBYTE,))
+                output["seeds"] = np.asfortranarray(raw.copy().reshape((n, SEEDBYTE)).T)
+            else:
+                output["seeds"] = np.zeros((SEEDBYTE, 0), dtype=np.uint8, order="F")
         if c.issave2pt:
-            f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum,)).copy()
-            output["flux"] = f.reshape((c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates), order="F")
+            f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum * blocks,)).copy()
+            shape = (c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates) + ((blocks,) if blocks > 1 else ())
+            output["flux"] = f.reshape(shape, order="F")
             output["stat"] = {
                 "runtime": int(c.runtime),
                 "nphoton": int(c.nphoton) * (c.respin if c.respin > 1 else 1),

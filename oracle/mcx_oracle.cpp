@@ -51,7 +51,7 @@ const float TWO_PI = 6.28318530717959f;
 const float MAX_ACCUM = 1000.f;
 const float ROULETTE = 10.f;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
-enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL };
+enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
 /* ---- glibc rand(): TYPE_3 additive feedback, r[i] = r[i-3] + r[i-31] --- */
 struct GlibcRand {
@@ -90,6 +90,15 @@ struct Rng {
     void seed(const unsigned int* s) {
         t0 = ((uint64_t)s[0] << 32) | s[1];
         t1 = ((uint64_t)s[2] << 32) | s[3];
+    }
+    /* the state as the 4 words of u64 {s0, s1} in little-endian memory order */
+    void setstate(const unsigned int* w) {
+        t0 = ((uint64_t)w[1] << 32) | w[0];
+        t1 = ((uint64_t)w[3] << 32) | w[2];
+    }
+    void getstate(unsigned int* w) const {
+        w[0] = (unsigned int)t0; w[1] = (unsigned int)(t0 >> 32);
+        w[2] = (unsigned int)t1; w[3] = (unsigned int)(t1 >> 32);
     }
     float next() {
         uint64_t s1 = t0;
@@ -224,6 +233,8 @@ struct Thread {
     unsigned int idx1d, mediaid;
     float w0;
     float* ppath;
+    unsigned int gid = 0;        /* global index of the current photon */
+    unsigned int seedsave[4];    /* launch RNG state of the current photon */
     unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
     int err = 0;
 
@@ -252,6 +263,8 @@ struct Thread {
             if ((flag >> 4) & 1) { rec[k++] = p.x; rec[k++] = p.y; rec[k++] = p.z; }
             if ((flag >> 5) & 1) { rec[k++] = v.x; rec[k++] = v.y; rec[k++] = v.z; }
             if ((flag >> 6) & 1) rec[k++] = ppath[C->w0offset - 2];
+            if (P().saveseed && C->O->seeddata)
+                for (int i = 0; i < 4; i++) C->O->seeddata[(size_t)base * 4 + i] = seedsave[i];
         } else if (P().savedet == 3) {
             C->detected->fetch_sub(1);
         }
@@ -328,6 +341,11 @@ struct Thread {
         if (Q.savedet)
             for (unsigned int i = 0; i < C->partialdata; i++) ppath[i] = 0.f;
         if ((int)f.w >= nphoton - 1) return 1;
+        /* src/mcx_core.cu:1205-1215, with the photon index counted from the
+         * thread's first photon idx*threadphoton + min(idx, oddphotons) */
+        gid = idx * (unsigned)Q.threadphoton + std::min(idx, (unsigned)Q.oddphotons) + (unsigned)((int)f.w + 1);
+        if (Q.replayseed) t.setstate(Q.replayseed + (size_t)gid * 4);
+        if (Q.saveseed) t.getstate(seedsave);
         float* pp = ppath + C->partialdata;
         const bool ispencil = (Q.srctype == 0);
         do {
@@ -485,7 +503,8 @@ struct Thread {
             float oldval = atomic_add_float_ret(fld + i, weight);
             if (fabsf(oldval) > MAX_ACCUM) {
                 atomic_add_float_ret(fld + i, (oldval > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
-                atomic_add_float_ret(fld + i + nvox * Q.maxgate, (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
+                atomic_add_float_ret(fld + i + nvox * Q.maxgate * (Q.replayseed && Q.replaydet == -1 ? Q.detnum : 1),
+                                     (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
             }
         }
     }
@@ -541,6 +560,17 @@ struct Thread {
                     v.w += 1.f;
                     rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
                     nscat++;
+                    /* src/mcx_core.cu:2019-2047 */
+                    if (Q.replayseed && (Q.outputtype == otWP || Q.outputtype == otDCS || Q.outputtype == otWPTOF)) {
+                        const float tof = Q.replaytof[gid];
+                        int ts = (int)floorf((tof - Q.twin0) * Q.Rtstep);
+                        ts = std::max(0, std::min((int)Q.maxgate - 1, ts));
+                        if (Q.replaydet == -1) ts += ((Q.replaydetid[gid] & 0xFFFF) - 1) * (int)Q.maxgate;
+                        float w = (Q.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
+                        w = (Q.outputtype == otWPTOF) ? tof : w;
+                        w *= Q.replayweight[gid];
+                        deposit(idx1d, ts, w);
+                    }
                 }
                 v.w = (float)(int)v.w;
             }
@@ -581,7 +611,14 @@ struct Thread {
                         weight = w0 - p.w;
                     else if (Q.outputtype == otFluence || Q.outputtype == otFlux)
                         weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
-                    else if (Q.outputtype == otL)
+                    else if (Q.replayseed && (Q.outputtype == otJacobian || Q.outputtype == otWLTOF)) {
+                        /* src/mcx_core.cu:2208-2220, gate clamped to the range */
+                        const float tof = Q.replaytof[gid];
+                        weight = Q.replayweight[gid] * f.z;
+                        if (Q.outputtype == otWLTOF) weight = weight * tof;
+                        tshift = std::max(0, std::min((int)Q.maxgate - 1, (int)floorf((tof - Q.twin0) * Q.Rtstep)));
+                        if (Q.replaydet == -1) tshift += ((Q.replaydetid[gid] & 0xFFFF) - 1) * (int)Q.maxgate;
+                    } else if (Q.outputtype == otL)
                         weight = w0 * f.z;
                     if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
                 }

@@ -39,7 +39,7 @@ typedef struct OracleParams {
     int doreflect;               /* cfg.isreflect */
     int isspecular, voidtime, maxvoidstep;
     unsigned int gscatter;
-    int outputtype;              /* otFlux / otFluence / otEnergy / otL */
+    int outputtype;              /* otFlux / otFluence / otEnergy / otL; replay: otJacobian, otWP, otDCS, otWLTOF, otWPTOF */
     int save2pt;
     unsigned char bc[12];        /* numeric bc codes and det flags */
     int is2d;
@@ -52,6 +52,13 @@ typedef struct OracleParams {
     const unsigned int* seeds;   /* 4*nthread seed words */
     unsigned int thread_begin, thread_end;  /* run streams [begin,end) */
     int ompthreads;              /* 0 = OpenMP default */
+    /* seed saving and replay (src/mcx_core.cu:1205-1215, :2019-2047, :2208-2220) */
+    int saveseed;                /* record each detected photon's launch RNG state */
+    const unsigned int* replayseed;  /* NULL, or 4 words per photon: u64 s0, s1 little endian */
+    const float* replayweight;
+    const float* replaytof;
+    const int* replaydetid;
+    int replaydet;
 } OracleParams;
 
 typedef struct OracleOut {
@@ -64,6 +71,7 @@ typedef struct OracleOut {
     int errors;           /* "should never happen" exits */
     unsigned long long nstep, ndeposit, nscatter, nlaunch;
     double seconds;       /* wall time of the transport loop */
+    unsigned int* seeddata;  /* maxdetphoton*4 words (with saveseed), or NULL */
 } OracleOut;
 
 /* Run the transport loop.  Returns 0 on success, <0 on bad input. */

@@ -49,6 +49,12 @@ class OracleParams(ctypes.Structure):
         ("seeds", ctypes.POINTER(ctypes.c_uint)),
         ("thread_begin", ctypes.c_uint), ("thread_end", ctypes.c_uint),
         ("ompthreads", ctypes.c_int),
+        ("saveseed", ctypes.c_int),
+        ("replayseed", ctypes.POINTER(ctypes.c_uint)),
+        ("replayweight", ctypes.POINTER(ctypes.c_float)),
+        ("replaytof", ctypes.POINTER(ctypes.c_float)),
+        ("replaydetid", ctypes.POINTER(ctypes.c_int)),
+        ("replaydet", ctypes.c_int),
     ]
 
 
@@ -64,6 +70,7 @@ class OracleOut(ctypes.Structure):
         ("nstep", ctypes.c_ulonglong), ("ndeposit", ctypes.c_ulonglong),
         ("nscatter", ctypes.c_ulonglong), ("nlaunch", ctypes.c_ulonglong),
         ("seconds", ctypes.c_double),
+        ("seeddata", ctypes.POINTER(ctypes.c_uint)),
     ]
 
 
@@ -139,12 +146,17 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         srctype=0, srcparam1=(0, 0, 0, 0), srcparam2=(0, 0, 0, 0), detpos=None, isreflect=1,
         isspecular=0, unitinmm=1.0, outputtype=0, savedetflag=0x5, issavedet=None,
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
-        thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True):
+        thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
+        saveseed=0, replay=None, replaydet=0):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
     srcpos/detpos: 0-based grid coordinates; srcdir normalised here.
     Returns a dict with field (FP64, [x,y,z,gate]), genergy, detp, counters.
+    saveseed: also return the launch RNG state of each detected photon
+    ("seeds", uint8 [n, 16]).  replay: dict with "seed" (uint8 [n, 16]),
+    "weight", "tof", "detid" per replayed photon (n = nphoton); with
+    replaydet == -1 the field has one block of gates per detector.
     """
     vol = np.asarray(vol)
     nx, ny, nz = vol.shape
@@ -237,9 +249,25 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         thread_range = (0, nthread)
     P.thread_begin, P.thread_end = thread_range
     P.ompthreads = ompthreads
+    P.saveseed = int(saveseed)
+    keep = []
+    blocks = 1
+    if replay is not None:
+        rs = np.ascontiguousarray(np.asarray(replay["seed"], dtype=np.uint8).reshape(-1, 16)).view(np.uint32).copy()
+        rw = np.ascontiguousarray(replay["weight"], dtype=np.float32)
+        rt = np.ascontiguousarray(replay["tof"], dtype=np.float32)
+        rd = np.ascontiguousarray(replay["detid"], dtype=np.int32)
+        keep += [rs, rw, rt, rd]
+        P.replayseed = _ptr(rs, ctypes.c_uint)
+        P.replayweight = _ptr(rw, ctypes.c_float)
+        P.replaytof = _ptr(rt, ctypes.c_float)
+        P.replaydetid = _ptr(rd, ctypes.c_int)
+        P.replaydet = int(replaydet)
+        if replaydet == -1:
+            blocks = max(detnum, 1)
     nvox = nx * ny * nz
-    field = np.zeros(nvox * maxgate, dtype=np.float64) if want_field else None
-    f32 = np.zeros(2 * nvox * maxgate, dtype=np.float32) if field32 else None
+    field = np.zeros(nvox * maxgate * blocks, dtype=np.float64) if want_field else None
+    f32 = np.zeros(2 * nvox * maxgate * blocks, dtype=np.float32) if field32 else None
     genergy = np.zeros(2 * nthread, dtype=np.float32)
     reclen = savedet_reclen(savedetflag, prop.shape[0])
     detdata = np.zeros(max(maxdetphoton, 1) * max(reclen, 1), dtype=np.float32) if issavedet else None
@@ -248,6 +276,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     O.field32 = _ptr(f32, ctypes.c_float) if f32 is not None else None
     O.genergy = _ptr(genergy, ctypes.c_float)
     O.detdata = _ptr(detdata, ctypes.c_float) if detdata is not None else None
+    seedbuf = np.zeros(max(maxdetphoton, 1) * 4, dtype=np.uint32) if (saveseed and issavedet) else None
+    O.seeddata = _ptr(seedbuf, ctypes.c_uint) if seedbuf is not None else None
     rc = lib().mcx_oracle_run(ctypes.byref(P), ctypes.byref(O))
     if rc != 0:
         raise RuntimeError("oracle rejected input (%d)" % rc)
@@ -266,10 +296,13 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     }
     out["absorbed"] = (out["energytot"] - out["energyesc"]) / out["energytot"] if out["energytot"] > 0 else 0.0
     if field is not None:
-        out["field"] = field.reshape((nx, ny, nz, maxgate), order="F")
+        out["field"] = field.reshape((nx, ny, nz, maxgate * blocks), order="F")
     if f32 is not None:
-        raw = f32[: nvox * maxgate] + f32[nvox * maxgate:]
-        out["field32"] = raw.reshape((nx, ny, nz, maxgate), order="F")
+        raw = f32[: nvox * maxgate * blocks] + f32[nvox * maxgate * blocks:]
+        out["field32"] = raw.reshape((nx, ny, nz, maxgate * blocks), order="F")
+    if seedbuf is not None:
+        n = min(out["detected"], maxdetphoton)
+        out["seeds"] = seedbuf[: 4 * n].view(np.uint8).reshape(n, 16).copy()
     if detdata is not None:
         n = min(out["detected"], maxdetphoton)
         out["detp"] = detdata[: n * reclen].reshape((n, reclen)) if reclen else np.zeros((n, 0), np.float32)

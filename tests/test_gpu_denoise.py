@@ -129,33 +129,37 @@ def test_conv3_weight_and_bias_gradient(cin, cout, nsplit):
     assert np.abs(_host_f32(db, (cout,)) - rb).max() <= 1e-5 * np.abs(rb).max()
 
 
-def test_pool_upsample_add():
-    rng = np.random.default_rng(3)
-    shape = (2, 4, 6, 8, 16)
+@pytest.mark.parametrize("c", [16, 4])
+def test_pool_upsample_add(c):
+    """c = 16 runs the 8-channel vector kernels, c = 4 the scalar ones"""
+    rng = np.random.default_rng(3 + c)
+    shape = (2, 4, 6, 8, c)
     x = dn_ref.bf16_round(rng.standard_normal(shape))
     dx = _dev_bf16(x)
     nlow = shape[0] * 2 * 3 * 4
-    y = torch.zeros(nlow * 16, dtype=torch.int16, device="cuda")
-    arg = torch.zeros(nlow * 16, dtype=torch.uint8, device="cuda")
-    assert L.mcx_dn_maxpool_fwd(_p(dx), 16, _p(y), 16, _p(arg), 2, 4, 6, 8, 16, None) == 0
+    y = torch.zeros(nlow * c, dtype=torch.int16, device="cuda")
+    arg = torch.zeros(nlow * c, dtype=torch.uint8, device="cuda")
+    assert L.mcx_dn_maxpool_fwd(_p(dx), c, _p(y), c, _p(arg), 2, 4, 6, 8, c, None) == 0
     ry, rarg = dn_ref.maxpool(x)
     assert np.array_equal(_host_f32(y, ry.shape), ry)
     assert np.array_equal(arg.cpu().numpy().reshape(rarg.shape), rarg)
     gy = dn_ref.bf16_round(rng.standard_normal(ry.shape))
     g = torch.zeros(int(np.prod(shape)), dtype=torch.int16, device="cuda")
-    assert L.mcx_dn_maxpool_bwd(_p(_dev_bf16(gy)), 16, _p(arg), _p(g), 16, 2, 4, 6, 8, 16, None) == 0
+    assert L.mcx_dn_maxpool_bwd(_p(_dev_bf16(gy)), c, _p(arg), _p(g), c, 2, 4, 6, 8, c, None) == 0
     assert np.array_equal(_host_f32(g, shape), dn_ref.maxpool_bwd(gy, rarg, shape))
     # upsample from the pooled grid back up, into a 48-channel buffer slice
     up = torch.zeros(int(np.prod(shape[:4])) * 48, dtype=torch.int16, device="cuda")
-    assert L.mcx_dn_upsample_fwd(_p(y), 16, _p(up, 32), 48, 2, 2, 3, 4, 16, None) == 0
-    assert np.array_equal(_host_f32(up, shape[:4] + (48,))[..., 32:], dn_ref.upsample(ry))
+    assert L.mcx_dn_upsample_fwd(_p(y), c, _p(up, 32), 48, 2, 2, 3, 4, c, None) == 0
+    got = _host_f32(up, shape[:4] + (48,))
+    assert np.array_equal(got[..., 32:32 + c], dn_ref.upsample(ry))
+    assert np.all(got[..., :32] == 0) and np.all(got[..., 32 + c:] == 0)
     gu = dn_ref.bf16_round(rng.standard_normal(shape[:4] + (48,)))
-    gl = torch.zeros(nlow * 16, dtype=torch.int16, device="cuda")
-    assert L.mcx_dn_upsample_bwd(_p(_dev_bf16(gu), 32), 48, _p(gl), 16, 2, 2, 3, 4, 16, None) == 0
-    ref = dn_ref.upsample_bwd(gu[..., 32:].astype(np.float64))
+    gl = torch.zeros(nlow * c, dtype=torch.int16, device="cuda")
+    assert L.mcx_dn_upsample_bwd(_p(_dev_bf16(gu), 32), 48, _p(gl), c, 2, 2, 3, 4, c, None) == 0
+    ref = dn_ref.upsample_bwd(gu[..., 32:32 + c].astype(np.float64))
     _bf_close(_host_f32(gl, ref.shape), ref)
     s = torch.zeros(int(np.prod(shape)), dtype=torch.int16, device="cuda")
-    assert L.mcx_dn_add(_p(dx), 16, _p(g), 16, _p(s), 16, int(np.prod(shape[:4])), 16, None) == 0
+    assert L.mcx_dn_add(_p(dx), c, _p(g), c, _p(s), c, int(np.prod(shape[:4])), c, None) == 0
     _bf_close(_host_f32(s, shape), x.astype(np.float64) + dn_ref.maxpool_bwd(gy, rarg, shape))
 
 

@@ -170,3 +170,17 @@ def test_boundary_detector_faces_bitexact():
     assert 9700 <= detp.shape[0] <= 9999
     assert detp.shape[0] == ora["detected"]
     assert np.array_equal(sort_rows(detp), sort_rows(ora["detp"]))
+
+
+def test_russian_roulette_bitexact():
+    """minenergy > 0 switches on Russian roulette (src/mcx_core.cu:2357-2364):
+    trajectories and energies stay identical to the oracle's."""
+    cfg = workloads.cube60(50000, minenergy=0.01)
+    res = _run(_layout(cfg, nthread=2048))
+    res["_tstep"] = cfg["tstep"]
+    ora = oracle_run(cfg, 2048)
+    assert ora["errors"] == 0
+    _check_energies(res, ora)
+    _check_flux(res, ora, rtol=2e-4, floor=1e-2)
+    a = res["stat"]["energyabs"] / res["stat"]["energytot"]
+    assert 0.27 <= a < 0.28, a
