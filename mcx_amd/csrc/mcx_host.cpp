@@ -43,14 +43,14 @@ static double g_stats[11];
 static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
 
-static bool supported_source(int t) {
-    switch (t) {
-        case MCX_SRC_PENCIL: case MCX_SRC_ISOTROPIC: case MCX_SRC_CONE: case MCX_SRC_GAUSSIAN: case MCX_SRC_PLANAR:
-        case MCX_SRC_FOURIER: case MCX_SRC_ARCSINE: case MCX_SRC_DISK: case MCX_SRC_PENCILARRAY: case MCX_SRC_RING:
-            return true;
-        default:
-            return false;
-    }
+static bool supported_source(int t) { return t >= MCX_SRC_PENCIL && t <= MCX_SRC_RING; }
+
+/* number of pattern weights of a pattern / pattern3d source */
+static size_t pattern_len(const Config* cfg) {
+    if (cfg->srctype == MCX_SRC_PATTERN) return (size_t)cfg->srcparam1.w * (size_t)cfg->srcparam2.w;
+    if (cfg->srctype == MCX_SRC_PATTERN3D)
+        return (size_t)cfg->srcparam1.x * (size_t)cfg->srcparam1.y * (size_t)cfg->srcparam1.z;
+    return 0;
 }
 
 void check_supported(const Config* cfg) {
@@ -65,6 +65,10 @@ void check_supported(const Config* cfg) {
     if (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))
         MCX_THROW(-1, "trajectory output (-D M) is not supported by this engine");
     if (!supported_source(cfg->srctype)) MCX_THROW(-1, "the specified source type is not supported by this engine");
+    if (pattern_len(cfg) && cfg->srcnum > 1)
+        MCX_THROW(-1, "multiple simultaneous source patterns (srcnum > 1) are not supported by this engine");
+    if ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && (!cfg->srcpattern || pattern_len(cfg) == 0))
+        MCX_THROW(-4, "the 'srcpattern' field can not be empty when your 'srctype' is 'pattern'");
     const bool replaytype = cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS ||
                             cfg->outputtype == otWLTOF || cfg->outputtype == otWPTOF;
     if (replaytype && !replay) MCX_THROW(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
@@ -251,7 +255,11 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
 
     Stream st;
     DevMem dkp, dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
-    DevMem dsaved, drseed, drweight, drtof, drdetid;
+    DevMem dsaved, drseed, drweight, drtof, drdetid, dpattern;
+    if (pattern_len(cfg)) {
+        dpattern.alloc(pattern_len(cfg) * sizeof(float));
+        HIP_CHECK(hipMemcpyAsync(dpattern.p, cfg->srcpattern, dpattern.bytes, hipMemcpyHostToDevice, st.s));
+    }
     std::vector<unsigned char> m8;
     if (media32) {
         dmedia.alloc(nvox * 4);
@@ -318,6 +326,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.detdata = ddet.as<float>();
     kp.detcount = dcount.as<uint32_t>();
     kp.counters = g_counters ? dcounters.as<uint64_t>() : nullptr;
+    kp.srcpattern = dpattern.as<float>();
     kp.saveseed = saveseed ? 1u : 0u;
     kp.seeddata = dsaved.as<uint32_t>();
     kp.replay = replay ? 1 : 0;

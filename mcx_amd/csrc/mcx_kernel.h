@@ -55,6 +55,7 @@ struct KParam {
     float* detdata;               /* maxdetphoton*reclen */
     uint32_t* detcount;           /* [0]=detected, [1]=error exits, [2]=lanes done */
     uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
+    const float* srcpattern;      /* pattern / pattern3d source intensities, or NULL */
     /* seed saving and replay */
     uint32_t saveseed;            /* store each detected photon's launch RNG state */
     uint32_t* seeddata;           /* 4 words per detected-photon slot */

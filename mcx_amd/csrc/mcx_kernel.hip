@@ -37,6 +37,7 @@ constexpr uint32_t OUT_MIN = 0xFFFFFFFFu;
 constexpr uint32_t OUT_MAX = 0x7FFFFFFFu;
 constexpr float MAX_ACCUM = 1000.f;
 constexpr float ROULETTE = 10.f;
+constexpr float JUST_BELOW_ONE = 0.9998f;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
 enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
@@ -335,13 +336,28 @@ struct Lane {
         float ox = kp.srcpos[0], oy = kp.srcpos[1], oz = kp.srcpos[2];
         skip = false;
         switch (kp.srctype) {
-            case 4: case 6: case 14: {
+            case 4: case 5: case 6: case 14: case 15: {
                 float rx = rnd();
                 float ry = rnd();
-                p.x = p.x + rx * s1[0] + ry * s2[0];
-                p.y = p.y + rx * s1[1] + ry * s2[1];
-                p.z = p.z + rx * s1[2] + ry * s2[2];
-                if (kp.srctype == 6) {
+                if (kp.srctype == 15) {
+                    const float rz = rnd();
+                    p.x = p.x + rx * s1[0];
+                    p.y = p.y + ry * s1[1];
+                    p.z = p.z + rz * s1[2];
+                    /* src/mcx_core.cu:1298-1302, one pattern (srcnum 1) */
+                    p.w = kp.srcpos[3] * glb(kp.srcpattern)[(int)(rz * JUST_BELOW_ONE * s1[2]) * (int)s1[1] * (int)s1[0] +
+                                                              (int)(ry * JUST_BELOW_ONE * s1[1]) * (int)s1[0] +
+                                                              (int)(rx * JUST_BELOW_ONE * s1[0])];
+                } else {
+                    p.x = p.x + rx * s1[0] + ry * s2[0];
+                    p.y = p.y + rx * s1[1] + ry * s2[1];
+                    p.z = p.z + rx * s1[2] + ry * s2[2];
+                }
+                if (kp.srctype == 5) {
+                    /* src/mcx_core.cu:1284-1287, one pattern (srcnum 1) */
+                    p.w = kp.srcpos[3] *
+                          glb(kp.srcpattern)[(int)(ry * JUST_BELOW_ONE * s2[3]) * (int)s1[3] + (int)(rx * JUST_BELOW_ONE * s1[3])];
+                } else if (kp.srctype == 6) {
                     float s, c;
                     float ang = (floorf(s1[3]) * rx + floorf(s2[3]) * ry + s1[3] - floorf(s1[3]));
                     mcx_sincos2pi(ang, &s, &c);
@@ -355,6 +371,122 @@ struct Lane {
                 ox = ox + (s1[0] + s2[0]) * 0.5f;
                 oy = oy + (s1[1] + s2[1]) * 0.5f;
                 oz = oz + (s1[2] + s2[2]) * 0.5f;
+                break;
+            }
+            case 9: case 10: {
+                /* src/mcx_core.cu:1335-1366 */
+                const float rx = rnd();
+                const float ry = rnd();
+                float v2w = s1[3] * (1.f / __builtin_sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]));
+                const float v2x = v2w * (kp.srcdir[1] * s1[2] - kp.srcdir[2] * s1[1]);
+                const float v2y = v2w * (kp.srcdir[2] * s1[0] - kp.srcdir[0] * s1[2]);
+                const float v2z = v2w * (kp.srcdir[0] * s1[1] - kp.srcdir[1] * s1[0]);
+                p.x = p.x + rx * s1[0] + ry * v2x;
+                p.y = p.y + rx * s1[1] + ry * v2y;
+                p.z = p.z + rx * s1[2] + ry * v2z;
+                if (kp.srctype == 10) {
+                    float sa, ca, sb, cb;
+                    mcx_sincos2pi(s2[0] * rx + s2[2], &sa, &ca);
+                    mcx_sincos2pi(s2[1] * ry + s2[3], &sb, &cb);
+                    p.w = kp.srcpos[3] * (sa * sb + 1.f) * 0.5f;
+                } else {
+                    float sa, ca;
+                    mcx_sincos2pi(s2[0] * rx + s2[1] * ry + s2[2], &sa, &ca);
+                    p.w = kp.srcpos[3] * (ca * (1.f - s2[3]) + 1.f) * 0.5f;
+                }
+                locate_launch();
+                ox = ox + (s1[0] + v2x) * 0.5f;
+                oy = oy + (s1[1] + v2y) * 0.5f;
+                oz = oz + (s1[2] + v2z) * 0.5f;
+                break;
+            }
+            case 11: {
+                /* src/mcx_core.cu:1446-1455 */
+                float sphi, cphi, stheta, ctheta;
+                mcx_sincos2pi(rnd(), &sphi, &cphi);
+                const float ang = __builtin_sqrtf(2.f * scatlen()) * (1.f - 2.f * rnd()) * s1[0];
+                mcx_sincos2pi(ang * 0.159154943091895336f, &stheta, &ctheta);
+                rotatevector(stheta, ctheta, sphi, cphi);
+                canfocus = 0;
+                break;
+            }
+            case 16: {
+                /* src/mcx_core.cu:1457-1505 */
+                float sphi, cphi;
+                mcx_sincos2pi(rnd(), &sphi, &cphi);
+                float r = __builtin_sqrtf(0.5f * scatlen()) * s1[0];
+                const float ax = -s1[1] / s1[2];
+                const float ay = 1.f / __builtin_sqrtf(r * r + s1[2] * s1[2]);
+                float px = r * (cphi - ax * sphi), py = r * (sphi + ax * cphi), pz = 0.f;
+                const float dx = -r * sphi * ay, dy = r * cphi * ay, dz = s1[2] * ay;
+                if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+                    r = 1.f - v.z * v.z;
+                    const float st = __builtin_sqrtf(r);
+                    r = 1.f / __builtin_sqrtf(r);
+                    const float cp = v.x * r, sp = v.y * r;
+                    const float qx = px * cp * v.z - py * sp, qy = px * sp * v.z + py * cp, qz = -px * st;
+                    px = qx; py = qy; pz = qz;
+                    const float nx = dx * cp * v.z - dy * sp + dz * cp * st;
+                    const float ny = dx * sp * v.z + dy * cp + dz * sp * st;
+                    const float nz = -dx * st + dz * v.z;
+                    v.x = nx; v.y = ny; v.z = nz;
+                } else {
+                    v.z = (v.z > 0.f) ? dz : -dz;
+                    v.x = dx; v.y = dy;
+                }
+                p.x = px + kp.srcpos[0];
+                p.y = py + kp.srcpos[1];
+                p.z = pz + kp.srcpos[2];
+                locate_launch();
+                canfocus = 0;
+                break;
+            }
+            case 12: case 13: {
+                /* src/mcx_core.cu:1508-1568 */
+                float r = rnd();
+                p.x = p.x + r * s1[0];
+                p.y = p.y + r * s1[1];
+                p.z = p.z + r * s1[2];
+                if (kp.srctype == 12) {
+                    float sphi, cphi;
+                    r = 1.f / __builtin_sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                    if (s2[0] > 0.f) {
+                        const float ax = s1[0] * r, ay = s1[1] * r, az = s1[2] * r;
+                        r = v.x * ax + v.y * ay + v.z * az;
+                        v.x = v.x - r * ax; v.y = v.y - r * ay; v.z = v.z - r * az;
+                        r = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                        v.x = v.x * r; v.y = v.y * r; v.z = v.z * r;
+                        r = s2[0] * (2.f * rnd() - 1.f);
+                        mcx_sincos2pi(r * 0.159154943091895336f, &sphi, &cphi);
+                        const float cx = ay * v.z - az * v.y, cy = az * v.x - ax * v.z, cz = ax * v.y - ay * v.x;
+                        v.x = v.x * cphi + cx * sphi;
+                        v.y = v.y * cphi + cy * sphi;
+                        v.z = v.z * cphi + cz * sphi;
+                    } else {
+                        v.x = s1[0] * r; v.y = s1[1] * r; v.z = s1[2] * r;
+                        mcx_sincos2pi(rnd(), &sphi, &cphi);
+                        rotatevector(1.f, 0.f, sphi, cphi);
+                    }
+                } else if (s2[0] > 0.f || s2[1] > 0.f) {
+                    float sphi, cphi;
+                    mcx_sincos2pi(rnd(), &sphi, &cphi);
+                    r = __builtin_sqrtf(2.f * scatlen());
+                    cphi *= s2[0] * r;
+                    sphi *= s2[1] * r;
+                    sphi *= 1.f / __builtin_sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                    const float cx = s1[1] * v.z - s1[2] * v.y, cy = s1[2] * v.x - s1[0] * v.z, cz = s1[0] * v.y - s1[1] * v.x;
+                    cphi *= 1.f / __builtin_sqrtf(cx * cx + cy * cy + cz * cz);
+                    v.x += cphi * cx + sphi * s1[0];
+                    v.y += cphi * cy + sphi * s1[1];
+                    v.z += cphi * cz + sphi * s1[2];
+                    r = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                    v.x *= r; v.y *= r; v.z *= r;
+                }
+                locate_launch();
+                ox = kp.srcpos[0] + s1[0] * 0.5f;
+                oy = kp.srcpos[1] + s1[1] * 0.5f;
+                oz = kp.srcpos[2] + s1[2] * 0.5f;
+                canfocus = (kp.srctype == 13);
                 break;
             }
             case 8: case 17: case 3: {

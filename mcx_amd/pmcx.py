@@ -178,7 +178,8 @@ def _parse_config(user_cfg, cfg):
         else:
             cfg.savedetflag = int(v)
     if "srcpattern" in user_cfg:
-        raise ValueError("pattern sources are not supported by this engine")
+        pat = np.asarray(user_cfg["srcpattern"], dtype=np.float32)
+        cfg.srcpattern = ctypes.cast(_copy_to_c(pat.ravel(order="F"), ctypes.c_float), ctypes.c_void_p).value
     if "invcdf" in user_cfg or "angleinvcdf" in user_cfg:
         raise ValueError("user-defined inverse CDFs are not supported by this engine")
     if "shapes" in user_cfg:

@@ -50,6 +50,7 @@ const unsigned int OUT_MAX = 0x7FFFFFFFu;
 const float TWO_PI = 6.28318530717959f;
 const float MAX_ACCUM = 1000.f;
 const float ROULETTE = 10.f;
+constexpr float JUST_BELOW_ONE = 0.9998f;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
 enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
@@ -360,14 +361,29 @@ struct Thread {
                 const float* s2 = Q.srcparam2;
                 switch (Q.srctype) {
                     case 4: /* planar */
+                    case 5: /* pattern */
                     case 6: /* fourier */
-                    case 14: { /* pencilarray */
+                    case 14: /* pencilarray */
+                    case 15: { /* pattern3d */
                         float rx = t.next();
                         float ry = t.next();
-                        p.x = p.x + rx * s1[0] + ry * s2[0];
-                        p.y = p.y + rx * s1[1] + ry * s2[1];
-                        p.z = p.z + rx * s1[2] + ry * s2[2];
-                        if (Q.srctype == 6) {
+                        if (Q.srctype == 15) {
+                            float rz = t.next();
+                            p.x = p.x + rx * s1[0];
+                            p.y = p.y + ry * s1[1];
+                            p.z = p.z + rz * s1[2];
+                            p.w = Q.srcpos[3] * Q.srcpattern[(int)(rz * JUST_BELOW_ONE * s1[2]) * (int)s1[1] * (int)s1[0] +
+                                                             (int)(ry * JUST_BELOW_ONE * s1[1]) * (int)s1[0] +
+                                                             (int)(rx * JUST_BELOW_ONE * s1[0])];
+                        } else {
+                            p.x = p.x + rx * s1[0] + ry * s2[0];
+                            p.y = p.y + rx * s1[1] + ry * s2[1];
+                            p.z = p.z + rx * s1[2] + ry * s2[2];
+                        }
+                        if (Q.srctype == 5) {
+                            p.w = Q.srcpos[3] * Q.srcpattern[(int)(ry * JUST_BELOW_ONE * s2[3]) * (int)s1[3] +
+                                                             (int)(rx * JUST_BELOW_ONE * s1[3])];
+                        } else if (Q.srctype == 6) {
                             float s, c;
                             float ang = (floorf(s1[3]) * rx + floorf(s2[3]) * ry + s1[3] - floorf(s1[3]));
                             om_sincos2pi(ang, &s, &c);
@@ -381,6 +397,120 @@ struct Thread {
                         org[0] = org[0] + (s1[0] + s2[0]) * 0.5f;
                         org[1] = org[1] + (s1[1] + s2[1]) * 0.5f;
                         org[2] = org[2] + (s1[2] + s2[2]) * 0.5f;
+                        break;
+                    }
+                    case 9:    /* fourierx */
+                    case 10: { /* fourierx2d */
+                        float rx = t.next();
+                        float ry = t.next();
+                        float v2w = s1[3] * (1.f / sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]));
+                        float v2x = v2w * (Q.srcdir[1] * s1[2] - Q.srcdir[2] * s1[1]);
+                        float v2y = v2w * (Q.srcdir[2] * s1[0] - Q.srcdir[0] * s1[2]);
+                        float v2z = v2w * (Q.srcdir[0] * s1[1] - Q.srcdir[1] * s1[0]);
+                        p.x = p.x + rx * s1[0] + ry * v2x;
+                        p.y = p.y + rx * s1[1] + ry * v2y;
+                        p.z = p.z + rx * s1[2] + ry * v2z;
+                        if (Q.srctype == 10) {
+                            float sa, ca, sb, cb;
+                            om_sincos2pi(s2[0] * rx + s2[2], &sa, &ca);
+                            om_sincos2pi(s2[1] * ry + s2[3], &sb, &cb);
+                            p.w = Q.srcpos[3] * (sa * sb + 1.f) * 0.5f;
+                        } else {
+                            float sa, ca;
+                            om_sincos2pi(s2[0] * rx + s2[1] * ry + s2[2], &sa, &ca);
+                            p.w = Q.srcpos[3] * (ca * (1.f - s2[3]) + 1.f) * 0.5f;
+                        }
+                        locate_launch();
+                        org[0] = org[0] + (s1[0] + v2x) * 0.5f;
+                        org[1] = org[1] + (s1[1] + v2y) * 0.5f;
+                        org[2] = org[2] + (s1[2] + v2z) * 0.5f;
+                        break;
+                    }
+                    case 11: { /* zgaussian */
+                        float sphi, cphi, stheta, ctheta;
+                        om_sincos2pi(t.next(), &sphi, &cphi);
+                        float ang = sqrtf(2.f * t.scatlen()) * (1.f - 2.f * t.next()) * s1[0];
+                        om_sincos2pi(ang * 0.159154943091895336f, &stheta, &ctheta);
+                        rotatevector(v, stheta, ctheta, sphi, cphi);
+                        canfocus = 0;
+                        break;
+                    }
+                    case 16: { /* hyperboloid gaussian */
+                        float sphi, cphi;
+                        om_sincos2pi(t.next(), &sphi, &cphi);
+                        float r = sqrtf(0.5f * t.scatlen()) * s1[0];
+                        float ax = -s1[1] / s1[2];
+                        float ay = 1.f / sqrtf(r * r + s1[2] * s1[2]);
+                        float px = r * (cphi - ax * sphi), py = r * (sphi + ax * cphi), pz = 0.f;
+                        float dx = -r * sphi * ay, dy = r * cphi * ay, dz = s1[2] * ay;
+                        if (v.z > -1.f + EPSF && v.z < 1.f - EPSF) {
+                            r = 1.f - v.z * v.z;
+                            float st = sqrtf(r);
+                            r = 1.f / sqrtf(r);
+                            float cp = v.x * r, sp = v.y * r;
+                            float qx = px * cp * v.z - py * sp, qy = px * sp * v.z + py * cp, qz = -px * st;
+                            px = qx; py = qy; pz = qz;
+                            float nx = dx * cp * v.z - dy * sp + dz * cp * st;
+                            float ny = dx * sp * v.z + dy * cp + dz * sp * st;
+                            float nz = -dx * st + dz * v.z;
+                            v.x = nx; v.y = ny; v.z = nz;
+                        } else {
+                            v.z = (v.z > 0.f) ? dz : -dz;
+                            v.x = dx; v.y = dy;
+                        }
+                        p.x = px + Q.srcpos[0];
+                        p.y = py + Q.srcpos[1];
+                        p.z = pz + Q.srcpos[2];
+                        locate_launch();
+                        canfocus = 0;
+                        break;
+                    }
+                    case 12:   /* line */
+                    case 13: { /* slit */
+                        float r = t.next();
+                        p.x = p.x + r * s1[0];
+                        p.y = p.y + r * s1[1];
+                        p.z = p.z + r * s1[2];
+                        if (Q.srctype == 12) {
+                            float sphi, cphi;
+                            r = 1.f / sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                            if (s2[0] > 0.f) {
+                                float ax = s1[0] * r, ay = s1[1] * r, az = s1[2] * r;
+                                r = v.x * ax + v.y * ay + v.z * az;
+                                v.x = v.x - r * ax; v.y = v.y - r * ay; v.z = v.z - r * az;
+                                r = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                                v.x = v.x * r; v.y = v.y * r; v.z = v.z * r;
+                                r = s2[0] * (2.f * t.next() - 1.f);
+                                om_sincos2pi(r * 0.159154943091895336f, &sphi, &cphi);
+                                float cx = ay * v.z - az * v.y, cy = az * v.x - ax * v.z, cz = ax * v.y - ay * v.x;
+                                v.x = v.x * cphi + cx * sphi;
+                                v.y = v.y * cphi + cy * sphi;
+                                v.z = v.z * cphi + cz * sphi;
+                            } else {
+                                v.x = s1[0] * r; v.y = s1[1] * r; v.z = s1[2] * r;
+                                om_sincos2pi(t.next(), &sphi, &cphi);
+                                rotatevector(v, 1.f, 0.f, sphi, cphi);
+                            }
+                        } else if (s2[0] > 0.f || s2[1] > 0.f) {
+                            float sphi, cphi;
+                            om_sincos2pi(t.next(), &sphi, &cphi);
+                            r = sqrtf(2.f * t.scatlen());
+                            cphi *= s2[0] * r;
+                            sphi *= s2[1] * r;
+                            sphi *= 1.f / sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+                            float cx = s1[1] * v.z - s1[2] * v.y, cy = s1[2] * v.x - s1[0] * v.z, cz = s1[0] * v.y - s1[1] * v.x;
+                            cphi *= 1.f / sqrtf(cx * cx + cy * cy + cz * cz);
+                            v.x += cphi * cx + sphi * s1[0];
+                            v.y += cphi * cy + sphi * s1[1];
+                            v.z += cphi * cz + sphi * s1[2];
+                            r = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                            v.x *= r; v.y *= r; v.z *= r;
+                        }
+                        locate_launch();
+                        org[0] = Q.srcpos[0] + s1[0] * 0.5f;
+                        org[1] = Q.srcpos[1] + s1[1] * 0.5f;
+                        org[2] = Q.srcpos[2] + s1[2] * 0.5f;
+                        canfocus = (Q.srctype == 13);
                         break;
                     }
                     case 8:  /* disk */

@@ -59,6 +59,7 @@ typedef struct OracleParams {
     const float* replaytof;
     const int* replaydetid;
     int replaydet;
+    const float* srcpattern;     /* pattern / pattern3d source intensities (one pattern), or NULL */
 } OracleParams;
 
 typedef struct OracleOut {

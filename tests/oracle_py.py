@@ -55,6 +55,7 @@ class OracleParams(ctypes.Structure):
         ("replaytof", ctypes.POINTER(ctypes.c_float)),
         ("replaydetid", ctypes.POINTER(ctypes.c_int)),
         ("replaydet", ctypes.c_int),
+        ("srcpattern", ctypes.POINTER(ctypes.c_float)),
     ]
 
 
@@ -147,7 +148,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         isspecular=0, unitinmm=1.0, outputtype=0, savedetflag=0x5, issavedet=None,
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
-        saveseed=0, replay=None, replaydet=0):
+        saveseed=0, replay=None, replaydet=0, srcpattern=None):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -251,6 +252,10 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     P.ompthreads = ompthreads
     P.saveseed = int(saveseed)
     keep = []
+    if srcpattern is not None:
+        pat = np.ascontiguousarray(np.asarray(srcpattern, dtype=np.float32).ravel(order="F"))
+        keep.append(pat)
+        P.srcpattern = _ptr(pat, ctypes.c_float)
     blocks = 1
     if replay is not None:
         rs = np.ascontiguousarray(np.asarray(replay["seed"], dtype=np.uint8).reshape(-1, 16)).view(np.uint32).copy()

@@ -16,9 +16,10 @@ def flag_bits(v):
 def oracle_run(cfg, nthread, **extra):
     """Run the oracle on a pmcx dict whose positions are 0-based (issrcfrom0=1)."""
     assert cfg.get("issrcfrom0", 0) == 1
-    srctypes = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier", "arcsine", "disk"]
+    srctypes = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier", "arcsine", "disk", "fourierx",
+                "fourierx2d", "zgaussian", "line", "slit", "pencilarray", "pattern3d", "hyperboloid", "ring"]
     st = cfg.get("srctype", "pencil")
-    srctype = srctypes.index(st) if isinstance(st, str) and st in srctypes else {"pencilarray": 14, "ring": 17}.get(st, 0)
+    srctype = srctypes.index(st) if isinstance(st, str) else int(st)
     det = cfg.get("detpos")
     issavedet = int(cfg.get("issavedet", 1 if det is not None else 0))
     ot = {"flux": 0, "fluence": 1, "energy": 2, "jacobian": 3, "wl": 3, "wp": 4, "nscat": 4, "wm": 5, "length": 7,
@@ -32,6 +33,7 @@ def oracle_run(cfg, nthread, **extra):
         savedetflag=flag_bits(cfg.get("savedetflag", 0x5)), issavedet=issavedet,
         maxdetphoton=int(cfg.get("maxdetphoton", 1000000)), minenergy=float(cfg.get("minenergy", 0.0)),
         bc=cfg.get("bc", ""),
+        srcpattern=cfg.get("srcpattern"),
     )
     kw.update(extra)
     return oracle_py.run(cfg["vol"], cfg["prop"], cfg["srcpos"], list(cfg["srcdir"]), cfg["nphoton"], nthread,

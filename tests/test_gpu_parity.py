@@ -112,6 +112,17 @@ def test_sphere100_gates_detectors_bitexact():
     dict(srctype="planar", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 0], srcparam2=[0, 40, 0, 0]),
     dict(srctype="pencilarray", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 4], srcparam2=[0, 40, 0, 4]),
     dict(srctype="fourier", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 2], srcparam2=[0, 40, 0, 0]),
+    dict(srctype="fourierx", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 40], srcparam2=[2, 1, 0.1, 0]),
+    dict(srctype="fourierx2d", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 40], srcparam2=[2, 3, 0, 0.25]),
+    dict(srctype="zgaussian", srcpos=[29.5, 29.5, 0], srcparam1=[0.3, 0, 0, 0]),
+    dict(srctype="line", srcpos=[10, 29.5, 29.5], srcparam1=[40, 0, 0, 0]),
+    dict(srctype="line", srcpos=[10, 29.5, 29.5], srcparam1=[40, 0, 0, 0], srcparam2=[1.0, 0, 0, 0]),
+    dict(srctype="slit", srcpos=[10, 29.5, 0], srcparam1=[40, 0, 0, 0], srcparam2=[0.1, 0.05, 0, 0]),
+    dict(srctype="hyperboloid", srcpos=[29.5, 29.5, 0], srcparam1=[1.0, 20.0, 5.0, 0]),
+    dict(srctype="pattern", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 8], srcparam2=[0, 40, 0, 8],
+         srcpattern=np.random.default_rng(1).uniform(size=(8, 8)).astype(np.float32)),
+    dict(srctype="pattern3d", srcpos=[20, 20, 0], srcparam1=[8, 8, 4, 0],
+         srcpattern=np.random.default_rng(2).uniform(size=(8, 8, 4)).astype(np.float32)),
 ])
 def test_source_types_bitexact(src):
     cfg = workloads.cube60(20000, **src)

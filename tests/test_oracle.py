@@ -214,3 +214,30 @@ def test_boundary_detector_faces_known_answer():
     cfg = workloads.cube60(10000, reflect=0, bc="______111111", issavedet=1, savedetflag="DP")
     r = oracle_run(cfg, 2048)
     assert 9700 <= r["detected"] <= 9999
+
+
+@pytest.mark.parametrize("src", [
+    dict(srctype="fourierx", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 40], srcparam2=[2, 1, 0.1, 0]),
+    dict(srctype="fourierx2d", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 40], srcparam2=[2, 3, 0, 0.25]),
+    dict(srctype="zgaussian", srcpos=[29.5, 29.5, 0], srcparam1=[0.3, 0, 0, 0]),
+    dict(srctype="line", srcpos=[10, 29.5, 29.5], srcparam1=[40, 0, 0, 0]),
+    dict(srctype="slit", srcpos=[10, 29.5, 0], srcparam1=[40, 0, 0, 0], srcparam2=[0.1, 0.05, 0, 0]),
+    dict(srctype="hyperboloid", srcpos=[29.5, 29.5, 0], srcparam1=[1.0, 20.0, 5.0, 0]),
+    dict(srctype="pattern", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 8], srcparam2=[0, 40, 0, 8],
+         srcpattern=np.full((8, 8), 0.5, dtype=np.float32)),
+])
+def test_more_source_types_conserve_energy(src):
+    """Every launched photon is accounted for (absorbed + escaped); patterned
+    and modulated sources launch less than one unit of weight per photon
+    (a uniform 0.5 pattern launches exactly 0.5), a line source inside the
+    cube absorbs more than a surface source."""
+    cfg = workloads.cube60(4000, reflect=0, **src)
+    r = oracle_run(cfg, 512, want_field=False)
+    assert r["errors"] == 0
+    tot = seq_sum(r["genergy"][1::2])
+    esc = seq_sum(r["genergy"][0::2])
+    assert 0.0 < tot <= 4000.0 + 1e-6 and 0.0 <= esc <= tot
+    if src["srctype"] == "pattern":
+        assert tot == pytest.approx(2000.0, rel=1e-6)
+    if src["srctype"] == "line":
+        assert (tot - esc) / tot > 0.5
