@@ -60,7 +60,7 @@ void check_supported(const Config* cfg) {
     if (replay && (!cfg->replay.seed || !cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid))
         MCX_THROW(-6, "photon replay needs the saved seeds and the detected-photon data (run mcx_validatecfg with detphotons)");
     if (cfg->nphase || cfg->nangle) MCX_THROW(-1, "user-defined phase/angle inverse CDFs are not supported by this engine");
-    if (cfg->issaveref) MCX_THROW(-1, "diffuse reflectance output (issaveref) is not supported by this engine");
+    if (cfg->issaveref > 1) MCX_THROW(-1, "per-voxel exit partial paths (issaveref > 1) are not supported by this engine");
     if (cfg->extrasrclen) MCX_THROW(-1, "multiple sources are not supported by this engine");
     if (cfg->debuglevel & (MCX_DEBUG_MOVE | MCX_DEBUG_MOVE_ONLY))
         MCX_THROW(-1, "trajectory output (-D M) is not supported by this engine");
@@ -327,6 +327,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.detcount = dcount.as<uint32_t>();
     kp.counters = g_counters ? dcounters.as<uint64_t>() : nullptr;
     kp.srcpattern = dpattern.as<float>();
+    kp.issaveref = cfg->issaveref ? 1u : 0u;
     kp.saveseed = saveseed ? 1u : 0u;
     kp.seeddata = dsaved.as<uint32_t>();
     kp.replay = replay ? 1 : 0;

@@ -56,6 +56,7 @@ struct KParam {
     uint32_t* detcount;           /* [0]=detected, [1]=error exits, [2]=lanes done */
     uint64_t* counters;           /* optional [steps, deposits, scatters, launches] or NULL */
     const float* srcpattern;      /* pattern / pattern3d source intensities, or NULL */
+    uint32_t issaveref;           /* tally escaping weight (negative) in the void voxel a photon leaves into */
     /* seed saving and replay */
     uint32_t saveseed;            /* store each detected photon's launch RNG state */
     uint32_t* seeddata;           /* 4 words per detected-photon slot */

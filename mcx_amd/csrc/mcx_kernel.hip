@@ -583,6 +583,12 @@ struct Lane {
             return 1;
         const bool live = fabsf(p.w) >= 0.f;
         if (live) esc += p.w;
+        /* src/mcx_core.cu:1124-1145: diffuse reflectance, the escaping weight
+         * tallied with a negative sign in the void voxel entered */
+        if (kp.issaveref && mediaid == 0 && idx1d != OUT_MIN && idx1d != OUT_MAX && p.w > 0.f) {
+            const int ts = max(0, min((int)kp.maxgate - 1, (int)floorf((f.y - kp.twin0) * kp.Rtstep)));
+            deposit(idx1d, ts, -p.w);
+        }
         if (kp.savedet) {
             savedet(live && (isdet & DET_MASK) == DET_MASK && mediaid == 0, isdet);
             for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;

@@ -291,6 +291,15 @@ def run(cfg=None, **kwargs):
         if c.issave2pt:
             f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum * blocks,)).copy()
             shape = (c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates) + ((blocks,) if blocks > 1 else ())
+            if c.issaveref:
+                # src/pmcx.cpp:1268-1289: void voxels hold the (negated)
+                # escaping weight; split them out as 'dref'
+                vol = np.ctypeslib.as_array(c.vol, shape=(nvox,)) & 0x7FFFFFFF
+                g = f.reshape(-1, nvox)
+                void = vol == 0
+                dref = np.where(void[None, :], -g, 0.0).astype(np.float32)
+                g[:, void] = 0.0
+                output["dref"] = dref.reshape(-1).reshape(shape, order="F")
             output["flux"] = f.reshape(shape, order="F")
             output["stat"] = {
                 "runtime": int(c.runtime),

@@ -337,6 +337,10 @@ struct Thread {
         if (Q.savedet == 3 && C->detected->load() >= Q.maxdetphoton) return 1;
         if (fabsf(p.w) >= 0.f) {
             ppath[C->partialdata] += p.w;
+            if (Q.issaveref && mediaid == 0 && idx1d != OUT_MIN && idx1d != OUT_MAX && p.w > 0.f) {
+                int ts = std::max(0, std::min((int)Q.maxgate - 1, (int)floorf((f.y - Q.twin0) * Q.Rtstep)));
+                deposit(idx1d, ts, -p.w);
+            }
             if (Q.savedet && (isdet & DET_MASK) == DET_MASK && mediaid == 0) savedet(isdet);
         }
         if (Q.savedet)

@@ -60,6 +60,7 @@ typedef struct OracleParams {
     const int* replaydetid;
     int replaydet;
     const float* srcpattern;     /* pattern / pattern3d source intensities (one pattern), or NULL */
+    int issaveref;               /* diffuse reflectance into void voxels (src/mcx_core.cu:1124-1145) */
 } OracleParams;
 
 typedef struct OracleOut {

@@ -56,6 +56,7 @@ class OracleParams(ctypes.Structure):
         ("replaydetid", ctypes.POINTER(ctypes.c_int)),
         ("replaydet", ctypes.c_int),
         ("srcpattern", ctypes.POINTER(ctypes.c_float)),
+        ("issaveref", ctypes.c_int),
     ]
 
 
@@ -148,7 +149,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         isspecular=0, unitinmm=1.0, outputtype=0, savedetflag=0x5, issavedet=None,
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
-        saveseed=0, replay=None, replaydet=0, srcpattern=None):
+        saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -251,6 +252,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     P.thread_begin, P.thread_end = thread_range
     P.ompthreads = ompthreads
     P.saveseed = int(saveseed)
+    P.issaveref = int(issaveref)
     keep = []
     if srcpattern is not None:
         pat = np.ascontiguousarray(np.asarray(srcpattern, dtype=np.float32).ravel(order="F"))

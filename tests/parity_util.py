@@ -34,6 +34,7 @@ def oracle_run(cfg, nthread, **extra):
         maxdetphoton=int(cfg.get("maxdetphoton", 1000000)), minenergy=float(cfg.get("minenergy", 0.0)),
         bc=cfg.get("bc", ""),
         srcpattern=cfg.get("srcpattern"),
+        issaveref=int(cfg.get("issaveref", 0)),
     )
     kw.update(extra)
     return oracle_py.run(cfg["vol"], cfg["prop"], cfg["srcpos"], list(cfg["srcdir"]), cfg["nphoton"], nthread,

@@ -195,3 +195,19 @@ def test_russian_roulette_bitexact():
     _check_flux(res, ora, rtol=2e-4, floor=1e-2)
     a = res["stat"]["energyabs"] / res["stat"]["energytot"]
     assert 0.27 <= a < 0.28, a
+
+
+def test_diffuse_reflectance_bitexact():
+    vol = np.ones((40, 40, 40), dtype=np.uint8)
+    vol[:, :, :2] = 0
+    cfg = workloads.cube60(20000, vol=vol, srcpos=[19.5, 19.5, 0], reflect=0, issaveref=1)
+    res = _run(_layout(cfg, nthread=1024), tally=2)
+    ora = oracle_run(cfg, 1024)
+    _check_energies(res, ora)
+    scale = normalize_like_engine(np.ones(1), res["stat"]["energytot"], cfg["tstep"])[1]
+    ref = ora["field"] * scale
+    void = vol == 0
+    flux, dref = res["flux"][..., 0].astype(np.float64), res["dref"][..., 0].astype(np.float64)
+    assert np.all(flux[void] == 0) and np.all(dref[~void] == 0)
+    assert np.abs(dref[void] + ref[..., 0][void]).max() <= 1e-5 * np.abs(ref).max()
+    assert np.abs(flux[~void] - ref[..., 0][~void]).max() <= 1e-5 * np.abs(ref).max()

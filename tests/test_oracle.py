@@ -241,3 +241,19 @@ def test_more_source_types_conserve_energy(src):
         assert tot == pytest.approx(2000.0, rel=1e-6)
     if src["srctype"] == "line":
         assert (tot - esc) / tot > 0.5
+
+
+def test_diffuse_reflectance_tally():
+    """issaveref (src/mcx_core.cu:1124-1145): the weight of photons leaving
+    tissue into void voxels of the grid is tallied there with a negative
+    sign; it is part of (and here most of) the escaped energy."""
+    vol = np.ones((40, 40, 40), dtype=np.uint8)
+    vol[:, :, :2] = 0
+    cfg = workloads.cube60(20000, vol=vol, srcpos=[19.5, 19.5, 0], reflect=0, issaveref=1)
+    r = oracle_run(cfg, 1024)
+    f = r["field"][..., 0]
+    void = vol == 0
+    assert np.all(f[void] <= 0) and np.all(f[~void] >= 0)
+    dref = -f[void].sum()
+    esc = seq_sum(r["genergy"][0::2])
+    assert 0.5 * esc < dref <= esc * (1 + 1e-9)
