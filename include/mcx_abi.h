@@ -70,6 +70,8 @@ typedef struct mcx_uint4 { unsigned int x, y, z, w; } __attribute__((aligned(16)
 #define MCX_MAX_ACCUM        1000.f                  /* src/mcx_core.h:53 */
 #define MCX_ROULETTE_SIZE    10.f                    /* src/mcx_core.h:55 */
 
+#include "mcx_media.h"
+
 #define MCX_DEBUG_RNG        1
 #define MCX_DEBUG_MOVE       2
 #define MCX_DEBUG_PROGRESS   4
@@ -246,6 +248,10 @@ void mcx_error(const int id, const char* msg, const char* file, const int linenu
 void mcx_flush(Config* cfg);
 /* src/mcx_utils.h:304, src/mcx_utils.c:1227 -- field *= scale. */
 void mcx_normalize(float field[], float scale, size_t fieldlen, int option, int pidx, int srcnum);
+/* src/mcx_utils.h:340, src/mcx_utils.c:5490 -- two floats packed as the
+ * half-precision word of the muamus_float / asgn_float media formats, with
+ * the reference encoder's exact bit rules. */
+int  mcx_float2half2(float input[2]);
 /* src/mcx_utils.h:300, src/mcx_utils.c:1418 -- load a JSON input file
  * (or, if fname starts with '{', inline JSON text) into cfg and prepare
  * the domain (shapes, preprocess). */

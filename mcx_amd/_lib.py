@@ -112,7 +112,7 @@ class Config(ctypes.Structure):
 ABI_SYMBOLS = [
     "mcx_run_simulation", "mcx_list_gpu", "mcx_initcfg", "mcx_clearcfg", "mcx_preprocess", "mcx_maskdet",
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
-    "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_last_error", "mcx_clear_error",
+    "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
     "mcx_dn_conv3_fwd", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
@@ -142,6 +142,8 @@ def _setup(L):
     L.mcx_savedetphoton.argtypes = [P(ctypes.c_float), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(Config)]
     L.mcx_normalize.argtypes = [P(ctypes.c_float), ctypes.c_float, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int]
+    L.mcx_float2half2.argtypes = [P(ctypes.c_float)]
+    L.mcx_float2half2.restype = ctypes.c_int
     L.mcx_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.mcx_last_error.restype = ctypes.c_int
     L.mcx_rng_test.argtypes = [ctypes.c_uint, P(ctypes.c_float), ctypes.c_uint]
@@ -207,12 +209,29 @@ def check_layout(L):
         raise RuntimeError("Config layout mismatch between libmcx_amd.so %s and the ctypes mirror %s" % (got, want))
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm ships its own HIP runtime under ROCm's soname
+    (libamdhip64.so.7) but links it by the name libamdhip64.so.  If this
+    library loaded ROCm's runtime first, a later `import torch` would load a
+    second HIP and HSA runtime into the process, which then finds no
+    device.  Loading torch's runtime first makes this library bind to it,
+    so both share one runtime (and can share device pointers and streams)."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
 def lib():
     """Load libmcx_amd.so (raises if it was not built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libmcx_amd.so is not built; run __graft_entry__.build() or make -C mcx_amd/csrc")
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         _setup(L)
         check_layout(L)

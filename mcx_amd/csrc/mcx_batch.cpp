@@ -104,6 +104,10 @@ static void check_batch_config(const Config* cfg, int i) {
         snprintf(msg, sizeof(msg), "batch instance %d: diffuse reflectance output is not batched", i);
         MCX_THROW(-1, msg);
     }
+    if (cfg->mediabyte > 4) {
+        snprintf(msg, sizeof(msg), "batch instance %d: a batched launch takes label volumes only", i);
+        MCX_THROW(-1, msg);
+    }
     if (cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) {
         snprintf(msg, sizeof(msg), "batch instance %d: pattern sources are not batched", i);
         MCX_THROW(-1, msg);

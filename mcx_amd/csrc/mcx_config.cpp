@@ -99,6 +99,7 @@ static void maskdet_impl(Config* cfg) {
     const unsigned int nx = cfg->dim.x, ny = cfg->dim.y, nz = cfg->dim.z;
     const size_t dx = nx + 2, dy = ny + 2, dz = nz + 2;
     const int isonecube = (nx == 1) && (ny == 1) && (nz == 1);
+    const bool svmc = cfg->mediabyte == MCX_MEDIA_2LABEL_SPLIT;
     /* a zero-padded copy, so the 26-neighbour test needs no bounds checks */
     std::vector<unsigned int> pad(dx * dy * dz, 0);
     for (size_t z = 1; z <= nz; z++)
@@ -133,14 +134,19 @@ static void maskdet_impl(Config* cfg) {
                     const float lim = radius + 0.5f * (1.f + isonecube);
                     if (mind2 == MCX_VERY_BIG || mind2 >= lim * lim) continue;
                     const size_t i = (size_t)(int)(iz + 1.f) * dy * dx + (size_t)(int)(iy + 1.f) * dx + (size_t)(int)(ix + 1.f);
-                    if (!pad[i]) continue;
-                    bool surrounded = true;
+                    /* split voxels (src/mcx_utils.c:4287-4310): a voxel whose
+                     * lower label is the background is on the surface; one
+                     * with a tissue lower label is when a neighbour's lower
+                     * label is the background */
+                    const unsigned int lower = (pad[i] & 0xFF000000u) >> 24, upper = (pad[i] & 0x00FF0000u) >> 16;
+                    if (svmc ? !(lower || upper) : !pad[i]) continue;
+                    bool surrounded = !(svmc && !lower && upper);
                     for (int oz = -1; oz <= 1 && surrounded; oz++)
                         for (int oy = -1; oy <= 1 && surrounded; oy++)
                             for (int ox = -1; ox <= 1; ox++) {
                                 if (!ox && !oy && !oz) continue;
                                 const size_t j = (size_t)((long)i + (long)oz * (long)(dy * dx) + (long)oy * (long)dx + ox);
-                                if (!pad[j]) {
+                                if (svmc ? !((pad[j] & 0xFF000000u) >> 24) : !pad[j]) {
                                     surrounded = false;
                                     break;
                                 }
@@ -155,6 +161,98 @@ static void maskdet_impl(Config* cfg) {
         if (cfg->issavedet && count == 0)
             fprintf(stderr, "WARNING: detector %u is not located on an interface, please check coordinates.\n", d + 1);
     }
+}
+
+/* src/mcx_utils.c:5490-5546 (mcx_float2half2): the reference's float ->
+ * half encoder, reproduced bit for bit because it defines the words that
+ * the kernel decodes.  Its rules: the mantissa is truncated to 10 bits; an
+ * exponent at or below 2^-15 gives a zero half exponent, except that when
+ * the top mantissa bits are also small (below 16) the value takes a
+ * rounded denormal path.  Because the encoder writes the first half into
+ * the union word it is still reading, the first value's three lowest
+ * mantissa bits are taken from the half just written: exponent bit 3,
+ * exponent bit 4 and the sign, not float bits 13-15.  A packed word of 0
+ * becomes 0x00010000 (mus = the smallest half denormal) and a packed word
+ * of 0x80000000 becomes 0.  For exponents below 2^-45 the denormal path of
+ * the reference shifts by 32 or more (undefined in C); here such values
+ * encode to a signed zero. */
+static unsigned int half_exp(unsigned int b) {
+    const unsigned int e = (b >> 23) & 0xFFu;
+    return e > 0x70u ? ((e - 0x70u) & 0x1Fu) : 0u;
+}
+
+static unsigned int half_denorm(unsigned int b) {
+    const unsigned int e = (b >> 23) & 0xFFu;
+    const unsigned int m = ((b >> 12) & 0x7FFu) | 0x800u;
+    const unsigned int s1 = 114u - e, s2 = 113u - e;
+    const unsigned int q = (s1 < 32u ? (m >> s1) : 0u) + (s2 < 32u ? ((m >> s2) & 1u) : 0u);
+    return ((b >> 16) & 0x8000u) | (q & 0xFFFFu);
+}
+
+unsigned int encode_half2(float a, float b) {
+    unsigned int ba, bb;
+    memcpy(&ba, &a, 4);
+    memcpy(&bb, &b, 4);
+    unsigned int h0, h1;
+    const unsigned int t0 = half_exp(ba), t1 = half_exp(bb);
+    if (((ba >> 13) & 0x3FFu) < 0x10u && t0 == 0u) {
+        h0 = half_denorm(ba);
+    } else {
+        const unsigned int top = ((((ba >> 31) << 5) | t0) << 10) & 0xFFFFu;
+        h0 = top | ((ba >> 13) & 0x3F8u) | ((top >> 13) & 7u);
+    }
+    if (((bb >> 13) & 0x3FFu) < 0x10u && t1 == 0u) {
+        h1 = half_denorm(bb);
+    } else {
+        h1 = (((((bb >> 31) << 5) | t1) << 10) & 0xFFFFu) | ((bb >> 13) & 0x3FFu);
+    }
+    unsigned int w = (h0 & 0xFFFFu) | (h1 << 16);
+    if (w == 0u) w = 0x00010000u;
+    if (w == 0x80000000u) w = 0u;
+    return w;
+}
+
+/* src/mcx_utils.c:3551-3569: a float mua scaled to grid units; 0 becomes
+ * FLT_EPSILON (a 0 word is a void voxel), NaN becomes a void voxel, and
+ * (volume files only) so does -0 */
+unsigned int encode_mua_float(float mua, float unitinmm, bool signed_zero_is_void) {
+    float f = mua * unitinmm;
+    unsigned int w;
+    memcpy(&w, &f, 4);
+    if (w == 0u) {
+        f = MCX_EPS;
+        memcpy(&w, &f, 4);
+    }
+    if (mua != mua || (signed_zero_is_void && w == 0x80000000u)) w = 0u;
+    return w;
+}
+
+/* src/mcx_utils.c:1714-1738: split voxels arrive as 8 bytes per voxel
+ * {lower label, upper label, point x, y, z, normal x, y, z}; the kernel
+ * reads them as two planes of words, the first holding
+ * {point y, point x, upper, lower} from the low byte up and the second
+ * (nvox words on) {normal z, normal y, normal x, point z} */
+static void svmc_planes(unsigned int* vol, size_t n) {
+    std::vector<unsigned int> out(2 * n);
+    const unsigned char* b = (const unsigned char*)vol;
+    for (size_t i = 0; i < n; i++) {
+        const unsigned char* v = b + 8 * i;
+        out[i] = (unsigned int)v[3] | ((unsigned int)v[2] << 8) | ((unsigned int)v[1] << 16) | ((unsigned int)v[0] << 24);
+        out[i + n] = (unsigned int)v[7] | ((unsigned int)v[6] << 8) | ((unsigned int)v[5] << 16) | ((unsigned int)v[4] << 24);
+    }
+    memcpy(vol, out.data(), 8 * n);
+}
+
+/* src/mcx_utils.c:1622-1623 (mcx_convertrow2col64): 8-byte voxels */
+static void row2col64(unsigned int* vol, const mcx_uint3* dim) {
+    const size_t n = (size_t)dim->x * dim->y * dim->z;
+    std::vector<unsigned long long> tmp(n);
+    memcpy(tmp.data(), vol, 8 * n);
+    unsigned long long* v = (unsigned long long*)vol;
+    for (size_t x = 0; x < dim->x; x++)
+        for (size_t y = 0; y < dim->y; y++)
+            for (size_t z = 0; z < dim->z; z++)
+                v[z * dim->y * dim->x + y * dim->x + x] = tmp[x * dim->y * dim->z + y * dim->z + z];
 }
 
 static void row2col(unsigned int* vol, const mcx_uint3* dim) {
@@ -216,8 +314,14 @@ static void preprocess_impl(Config* cfg) {
         }
     }
     if (cfg->isrowmajor) {
-        if (cfg->mediabyte > 4) MCX_THROW(-1, "row-major input is only supported for label volumes here");
-        row2col(cfg->vol, &cfg->dim);
+        /* src/mcx_utils.c:1620-1630: ASGN_F2H keeps two planes of words,
+         * split voxels are 8-byte units */
+        if (cfg->mediabyte == MCX_MEDIA_2LABEL_SPLIT) {
+            row2col64(cfg->vol, &cfg->dim);
+        } else {
+            row2col(cfg->vol, &cfg->dim);
+            if (cfg->mediabyte == MCX_MEDIA_ASGN_F2H) row2col(cfg->vol + (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z, &cfg->dim);
+        }
         cfg->isrowmajor = 0;
     }
     if (cfg->issavedet && cfg->detnum == 0 && isbcdet == 0) cfg->issavedet = 0;
@@ -238,10 +342,23 @@ static void preprocess_impl(Config* cfg) {
     if (cfg->srcpos.w == 0.f) cfg->srcpos.w = 1.f;
     if (cfg->extrasrclen) MCX_THROW(-4, "multiple sources (srcdata) are not supported by this engine");
     if (cfg->vol) {
-        if (cfg->mediabyte > 4) MCX_THROW(-1, "only label-based media formats (mediabyte 1, 2, 4) are supported by this engine");
         const size_t dimxyz = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
         unsigned int maxlabel = 0;
-        for (size_t i = 0; i < dimxyz; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
+        if (cfg->mediabyte <= 4) {
+            for (size_t i = 0; i < dimxyz; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
+        } else if (cfg->mediabyte == MCX_MEDIA_2LABEL_MIX) {
+            /* the two byte labels of each word index the property table */
+            for (size_t i = 0; i < dimxyz; i++)
+                maxlabel = std::max(maxlabel, std::max(cfg->vol[i] & 0xFFu, (cfg->vol[i] >> 8) & 0xFFu));
+        } else if (cfg->mediabyte == MCX_MEDIA_LABEL_HALF) {
+            for (size_t i = 0; i < dimxyz; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & 0x3FFFu);
+        } else if (cfg->mediabyte == MCX_MEDIA_2LABEL_SPLIT) {
+            svmc_planes(cfg->vol, dimxyz);
+            for (size_t i = 0; i < dimxyz; i++) {
+                if (cfg->vol[i] >> 31) MCX_THROW(-4, "split-voxel lower labels must be below 128");
+                maxlabel = std::max(maxlabel, std::max(cfg->vol[i] >> 24, (cfg->vol[i] >> 16) & 0xFFu));
+            }
+        }
         if (cfg->medianum <= maxlabel) MCX_THROW(-4, "input media optical properties are less than the labels in the volume");
     }
     /* point sources: precompute the launch voxel index and its label
@@ -260,10 +377,22 @@ static void preprocess_impl(Config* cfg) {
     if (cfg->issavedet) maskdet_impl(cfg);
     for (int i = 0; i < MCX_MAX_DEVICE; i++)
         if (cfg->deviceid[i] == '0') cfg->deviceid[i] = '\0';
+    if ((cfg->mediabyte == MCX_MEDIA_AS_F2H || cfg->mediabyte == MCX_MEDIA_MUA_FLOAT || cfg->mediabyte == MCX_MEDIA_AS_HALF) &&
+        cfg->medianum < 2)
+        MCX_THROW(-4, "the 'prop' field must contain at least 2 rows for the requested media format");
+    if ((cfg->mediabyte == MCX_MEDIA_ASGN_BYTE || cfg->mediabyte == MCX_MEDIA_AS_SHORT) && cfg->medianum < 3)
+        MCX_THROW(-4, "the 'prop' field must contain at least 3 rows for the requested media format");
     if (cfg->ismomentum) cfg->savedetflag |= (1u << 3);
     if (cfg->issaveexit) cfg->savedetflag |= (1u << 4) | (1u << 5);
     cfg->savedetflag &= ~(1u << 7);
     if (cfg->issavedet && cfg->savedetflag == 0) cfg->savedetflag = 0x5;
+    /* src/mcx_utils.c:1801-1806: property-word media have no medium index
+     * for the per-medium record columns (scattering counts, partial paths,
+     * momentum transfer).  The reference drops them from 100 on; here the
+     * formats below 100 drop them too, since their words carry no index
+     * that fits the columns either. */
+    if (cfg->mediabyte > 4 && cfg->mediabyte != MCX_MEDIA_2LABEL_SPLIT && cfg->savedetflag)
+        cfg->savedetflag &= ~((1u << 1) | (1u << 2) | (1u << 3) | (1u << 7));
     if (cfg->issaveref > 1) {
         if (cfg->issavedet == 0) MCX_THROW(-4, "you must have at least two outputs if issaveref is greater than 1");
         cfg->savedetflag = 0x5;
@@ -377,6 +506,8 @@ void mcx_clear_error(void) {
 void mcx_flush(Config* cfg) {
     if (cfg && cfg->flog) fflush(cfg->flog);
 }
+
+int mcx_float2half2(float input[2]) { return (int)encode_half2(input[0], input[1]); }
 
 void mcx_normalize(float field[], float scale, size_t fieldlen, int option, int pidx, int srcnum) {
 #pragma omp parallel for schedule(static)

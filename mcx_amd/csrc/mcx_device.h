@@ -62,6 +62,8 @@ struct EventPair {
 
 /* throws for Config features this engine does not implement */
 void check_supported(const Config* cfg);
+int media_words(unsigned int mediabyte);
+size_t media_words_per_voxel(unsigned int mediabyte);
 /* waves per SIMD for a kernel using `vgprs` architectural VGPRs */
 int waves_per_simd(int vgprs);
 /* launch constants of one simulation: grid, source, physics, boundary and

@@ -45,6 +45,16 @@ static int g_counters = 0;
 
 static bool supported_source(int t) { return t >= MCX_SRC_PENCIL && t <= MCX_SRC_RING; }
 
+/* property-word media formats the kernel decodes (src/mcx_const.h:72-80) */
+int media_words(unsigned int mediabyte) {
+    return mediabyte >= MCX_MEDIA_ASGN_F2H && mediabyte <= MCX_MEDIA_AS_SHORT;
+}
+
+/* 32-bit words per voxel of the uploaded volume */
+size_t media_words_per_voxel(unsigned int mediabyte) {
+    return (mediabyte == MCX_MEDIA_ASGN_F2H || mediabyte == MCX_MEDIA_2LABEL_SPLIT) ? 2 : 1;
+}
+
 /* number of pattern weights of a pattern / pattern3d source */
 static size_t pattern_len(const Config* cfg) {
     if (cfg->srctype == MCX_SRC_PATTERN) return (size_t)cfg->srcparam1.w * (size_t)cfg->srcparam2.w;
@@ -55,7 +65,8 @@ static size_t pattern_len(const Config* cfg) {
 
 void check_supported(const Config* cfg) {
     if (cfg->polmedianum > 0) MCX_THROW(-1, "polarized photon simulation is not supported by this engine");
-    if (cfg->mediabyte > 4) MCX_THROW(-1, "only label-based media formats are supported by this engine");
+    if (cfg->mediabyte > 4 && !media_words(cfg->mediabyte))
+        MCX_THROW(-1, "the specified media format is not supported by this engine");
     const bool replay = cfg->seed == MCX_SEED_FROM_FILE;
     if (replay && (!cfg->replay.seed || !cfg->replay.weight || !cfg->replay.tof || !cfg->replay.detid))
         MCX_THROW(-6, "photon replay needs the saved seeds and the detected-photon data (run mcx_validatecfg with detphotons)");
@@ -135,6 +146,7 @@ void fill_kparam(const Config* cfg, unsigned int maxgate, unsigned int is2d, KPa
     kp.maxmedia = cfg->medianum - 1;
     kp.detnum = cfg->detnum;
     kp.medianum = cfg->medianum;
+    kp.mediaformat = cfg->mediabyte;
     kp.ntable = ntable;
 }
 
@@ -201,10 +213,14 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     for (int i = 0; i < 6; i++)
         if (cfg->bc[i] == bcReflect || cfg->bc[i] == bcMirror) isref = 1;
 
-    /* media: uint8 with the detector bit in bit 7 when labels fit */
-    unsigned int maxlabel = 0;
-    for (size_t i = 0; i < nvox; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
-    const int media32 = (maxlabel > 127);
+    /* media: uint8 with the detector bit in bit 7 when labels fit, uint32
+     * labels, or the 32-bit property words of the non-label formats */
+    int media32 = cfg->mediabyte == MCX_MEDIA_2LABEL_SPLIT ? 3 : 2;
+    if (cfg->mediabyte <= 4) {
+        unsigned int maxlabel = 0;
+        for (size_t i = 0; i < nvox; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
+        media32 = (maxlabel > 127);
+    }
 
     int numregs = 0, maxthreads = 0;
     HIP_CHECK(mcxk_kernel_attrs(ispencil, isref > 0, tally, media32, &numregs, &maxthreads));
@@ -262,8 +278,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
     std::vector<unsigned char> m8;
     if (media32) {
-        dmedia.alloc(nvox * 4);
-        HIP_CHECK(hipMemcpyAsync(dmedia.p, cfg->vol, nvox * 4, hipMemcpyHostToDevice, st.s));
+        const size_t words = nvox * (media32 >= 2 ? media_words_per_voxel(cfg->mediabyte) : 1);
+        dmedia.alloc(words * 4);
+        HIP_CHECK(hipMemcpyAsync(dmedia.p, cfg->vol, words * 4, hipMemcpyHostToDevice, st.s));
     } else {
         m8.resize(nvox);
         for (size_t i = 0; i < nvox; i++) m8[i] = (unsigned char)((cfg->vol[i] & 0x7Fu) | ((cfg->vol[i] >> 24) & 0x80u));

@@ -50,6 +50,10 @@ unsigned int det_partialdata(const Config* cfg);
 /* rasterize a JSON Shapes description onto cfg->vol (mcx_io.cpp) */
 void apply_shapes(Config* cfg, const char* text);
 
+/* host-side encoders of the property-word media formats (mcx_config.cpp) */
+unsigned int encode_half2(float a, float b);
+unsigned int encode_mua_float(float mua, float unitinmm, bool signed_zero_is_void);
+
 }  // namespace mcx
 
 #define MCX_THROW(id, msg) throw mcx::Error((id), (msg))

@@ -288,20 +288,39 @@ static void rasterize_shapes(const Json& shapes, Grid& g) {
 
 /* ---- JSON -> Config (src/mcx_utils.c:2257-3210) ---------------------- */
 
+/* src/mcx_utils.c:3510-3600 (mcx_loadvolume): labels widen to 32 bits;
+ * float media are encoded into property words; the other non-label
+ * formats are stored as read */
 static void load_volume_file(Config* cfg, const std::string& path) {
     std::ifstream f(path, std::ios::binary);
     if (!f) throw Error(-1, "can not open the volume file " + path);
-    size_t n = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
-    int mb = cfg->mediabyte <= 4 ? (int)cfg->mediabyte : 4;
-    std::vector<unsigned char> raw(n * (size_t)mb);
+    const size_t n = (size_t)cfg->dim.x * cfg->dim.y * cfg->dim.z;
+    const unsigned int fmt = cfg->mediabyte;
+    const size_t per = fmt == MCX_MEDIA_ASGN_F2H ? 16 : (fmt == MCX_MEDIA_AS_F2H || fmt == MCX_MEDIA_2LABEL_SPLIT) ? 8 : std::min(fmt, 4u);
+    const size_t words = (fmt == MCX_MEDIA_ASGN_F2H || fmt == MCX_MEDIA_2LABEL_SPLIT) ? 2 * n : n;
+    std::vector<unsigned char> raw(n * per);
     f.read((char*)raw.data(), (std::streamsize)raw.size());
     if ((size_t)f.gcount() != raw.size()) throw Error(-1, "volume file is smaller than Domain.Dim");
     free(cfg->vol);
-    cfg->vol = (unsigned int*)malloc(n * sizeof(unsigned int));
+    cfg->vol = (unsigned int*)calloc(words, sizeof(unsigned int));
+    const float* fv = (const float*)raw.data();
     for (size_t i = 0; i < n; i++) {
-        if (mb == 1) cfg->vol[i] = raw[i];
-        else if (mb == 2) cfg->vol[i] = ((unsigned short*)raw.data())[i];
-        else cfg->vol[i] = ((unsigned int*)raw.data())[i];
+        if (fmt == 1) {
+            cfg->vol[i] = raw[i];
+        } else if (fmt == 2) {
+            cfg->vol[i] = ((const unsigned short*)raw.data())[i];
+        } else if (fmt == MCX_MEDIA_MUA_FLOAT) {
+            cfg->vol[i] = encode_mua_float(fv[i], cfg->unitinmm, true);
+        } else if (fmt == MCX_MEDIA_AS_F2H || fmt == MCX_MEDIA_ASGN_F2H) {
+            const size_t k = fmt == MCX_MEDIA_ASGN_F2H ? 4 * i : 2 * i;
+            const float mua = fv[k] * cfg->unitinmm, mus = fv[k + 1] * cfg->unitinmm;
+            cfg->vol[i] = (mua != mua || mus != mus) ? 0u : encode_half2(mua, mus);
+            if (fmt == MCX_MEDIA_ASGN_F2H) cfg->vol[i + n] = encode_half2(fv[k + 2], fv[k + 3]);
+        } else if (fmt == MCX_MEDIA_2LABEL_SPLIT) {
+            memcpy(cfg->vol + 2 * i, raw.data() + 8 * i, 8);
+        } else {
+            memcpy(cfg->vol + i, raw.data() + 4 * i, 4);
+        }
     }
 }
 
@@ -491,6 +510,7 @@ static void loadjson(const Json& root, Config* cfg, std::string* volfile) {
         if (cfg->maxgate == 0 || cfg->maxgate > gates) cfg->maxgate = gates;
     }
     if (Shapes) {
+        if (cfg->mediabyte > 4) throw Error(-10, "rasterization of shapes must be used with label-based mediatype");
         Grid g{&cfg->vol, &cfg->dim, 1.f, 1.f, 1.f};
         if (cfg->issrcfrom0) g.ox = g.oy = g.oz = 0.f;
         if (Shapes->is_obj() && Shapes->get("_ArraySize_"))
@@ -670,6 +690,7 @@ static void save_jdet(const float* ppath, const void* seeds, unsigned int count,
  * Shapes member, or a bare Shapes array, rasterized over cfg->vol */
 void apply_shapes(Config* cfg, const char* text) {
     if (!text || !strchr(text, ':')) return;
+    if (cfg->mediabyte > 4) throw Error(-6, "rasterization of shapes must be used with label-based mediatype");
     Json j = Json::parse(text);
     const Json* shapes = j.is_obj() ? j.get("Shapes") : &j;
     if (!shapes) return;

@@ -44,7 +44,7 @@ struct KParam {
     int32_t threadphoton, oddphotons;
     uint32_t nthread;
     /* buffers */
-    const void* media;            /* uint8 (label | det<<7) or uint32 (label | DET_MASK) */
+    const void* media;            /* uint8 (label | det<<7), uint32 (label | DET_MASK) or property words */
     const float* gprop;           /* ntable x 4 floats */
     float* field;                 /* tally (FP32 modes) */
     float* field2;                /* shadow (mode 0) or lo (mode 1) */
@@ -66,6 +66,7 @@ struct KParam {
     const float* replayweight;    /* detected weight of each replayed photon */
     const float* replaytof;       /* time of flight of each replayed photon */
     const int32_t* replaydetid;   /* detector id of each replayed photon */
+    uint32_t mediaformat;         /* 1, 2, 4: labels; 96-104: per-voxel property words */
 };
 
 /* One instance of a batched launch (its launch constants are in KParam) */

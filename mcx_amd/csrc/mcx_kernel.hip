@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mcx_media.h"
 #include "mcx_kernel.h"
 #include "mcx_math.h"
 
@@ -79,6 +80,35 @@ __device__ __forceinline__ uint32_t load_media<uint32_t>(const void* m, uint32_t
     return glb((const uint32_t*)m)[i];
 }
 
+/* media type tag of the non-label formats: 32-bit words decoded by
+ * kp.mediaformat (ASGN_F2H keeps a second word per voxel nvox words on) */
+struct MediaWord {};
+
+template <>
+__device__ __forceinline__ uint32_t load_media<MediaWord>(const void* m, uint32_t i) {
+    return glb((const uint32_t*)m)[i];
+}
+
+/* media type tag of the split-voxel format: per voxel a word with the two
+ * labels and two interface-point bytes, and (nvox words on) a word with
+ * the third point byte and the normal bytes */
+struct MediaSplit {};
+
+template <>
+__device__ __forceinline__ uint32_t load_media<MediaSplit>(const void* m, uint32_t i) {
+    return glb((const uint32_t*)m)[i];
+}
+
+template <typename T> struct is_word_media { static constexpr bool value = false; };
+template <> struct is_word_media<MediaWord> { static constexpr bool value = true; };
+template <typename T> struct is_split_media { static constexpr bool value = false; };
+template <> struct is_split_media<MediaSplit> { static constexpr bool value = true; };
+
+/* IEEE binary16 bits -> float (exact) */
+__device__ __forceinline__ float half_to_float(uint32_t h) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+}
+
 /* XCD executing this wave: s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4),
  * masked so the copy index is always in range */
 __device__ __forceinline__ uint32_t xcc_id() {
@@ -118,6 +148,13 @@ struct Lane {
     double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t gid;         /* global index of the current photon (replay) */
     uint32_t nstep, ndep, nscat, nlaunch;
+    /* split-voxel state (src/mcx_core.h:65-83): interface point and
+     * normal of the current voxel, its two labels, which side the photon
+     * is on, whether the voxel is split; whether the current path segment
+     * still needs the interface test and whether this step hit it */
+    float rpx, rpy, rpz, nvx, nvy, nvz;
+    uint32_t svlo, svup, svupper, svsplit;
+    bool testint, hitintf;
 
     __device__ __forceinline__ Lane(const KParam& k, const F4* tb, float* ppb, uint32_t s, uint32_t id)
         : kp(k), tab(tb), pp(ppb), stride(s), idx(id) {}
@@ -135,7 +172,187 @@ struct Lane {
     }
     __device__ __forceinline__ float scatlen() { return -mcx_logf(rnd() + EPSF); }
 
-    __device__ __forceinline__ void setprop(uint32_t m) { prop = tab[m & MED_MASK]; }
+    static constexpr bool WORDS = is_word_media<MT>::value;
+    static constexpr bool SPLIT = is_split_media<MT>::value;
+
+    __device__ __forceinline__ uint32_t sv_label() const { return svupper ? svup : svlo; }
+
+    /* src/mcx_core.cu:688-729: properties of split-voxel word w (voxel vox)
+     * for a photon at q in voxel (i0, i1, i2).  In a split voxel the side
+     * of the interface plane decides the label, and the normal is turned to
+     * point away from the photon's side. */
+    __device__ __forceinline__ void svmc_update(F4& out, uint32_t w, uint32_t vox, const F4& q, int i0, int i1, int i2) {
+        if (vox == OUT_MIN || vox == OUT_MAX) {
+            out = tab[0];
+            return;
+        }
+        const uint32_t lo = load_media<MT>(kp.media, vox + kp.nvox);
+        const uint32_t hi = w & MED_MASK;
+        svlo = hi >> 24;
+        svup = (hi >> 16) & 0xFFu;
+        if (svup) {
+            rpx = (float)((hi >> 8) & 0xFFu) * (1.f / 255.f);
+            rpy = (float)(hi & 0xFFu) * (1.f / 255.f);
+            rpz = (float)(lo >> 24) * (1.f / 255.f);
+            rpx += (float)i0;
+            rpy += (float)i1;
+            rpz += (float)i2;
+            nvx = (float)((lo >> 16) & 0xFFu) * (2.f / 255.f) - 1.f;
+            nvy = (float)((lo >> 8) & 0xFFu) * (2.f / 255.f) - 1.f;
+            nvz = (float)(lo & 0xFFu) * (2.f / 255.f) - 1.f;
+            const float r = 1.f / __builtin_sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
+            nvx = nvx * r;
+            nvy = nvy * r;
+            nvz = nvz * r;
+            if ((rpx - q.x) * nvx + (rpy - q.y) * nvy + (rpz - q.z) * nvz < 0.f) {
+                out = tab[svup];
+                svupper = 1;
+                nvx = -nvx;
+                nvy = -nvy;
+                nvz = -nvz;
+            } else {
+                out = tab[svlo];
+                svupper = 0;
+            }
+            svsplit = 1;
+        } else {
+            out = tab[svlo];
+            svsplit = 0;
+            svupper = 0;
+        }
+    }
+
+    /* src/mcx_core.cu:747-766: does the step of length len from p cross the
+     * interface plane?  If so, shorten the step to the crossing */
+    __device__ __forceinline__ bool ray_plane(float& len, float& slen) const {
+        if (v.x * nvx + v.y * nvy + v.z * nvz <= 0.f) return false;
+        const float p1x = p.x + len * v.x, p1y = p.y + len * v.y, p1z = p.z + len * v.z;
+        const float d0 = (p.x - rpx) * nvx + (p.y - rpy) * nvy + (p.z - rpz) * nvz;
+        const float d1 = (p1x - rpx) * nvx + (p1y - rpy) * nvy + (p1z - rpz) * nvz;
+        if (d0 * d1 > 0.f) return false;
+        const float len0 = len * d0 / (d0 - d1);
+        len = (len0 > 0.f) ? len0 : len;
+        slen = len * prop.y * ((v.w + 1.f > (float)kp.gscatter) ? (1.f - prop.z) : 1.f);
+        return true;
+    }
+
+    /* src/mcx_core.cu:782-828: Fresnel reflection or refraction at the
+     * interface plane; true when the photon transmits into the background */
+    __device__ __forceinline__ bool reflectray(float n1) {
+        const float Icos = fabsf(v.x * nvx + v.y * nvy + v.z * nvz);
+        const float n2 = tab[svupper ? svup : svlo].w;
+        const float tmp0 = n1 * n1, tmp1 = n2 * n2;
+        float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+        if (tmp2 > 0.f) {
+            float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+            tmp2 = __builtin_sqrtf(tmp2);
+            const float Im = 2.f * n1 * n2 * Icos * tmp2;
+            float Rtotal = (Re - Im) / (Re + Im);
+            Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+            Rtotal = (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+            if (rnd() <= Rtotal) {
+                v.x = v.x + (-2.f * Icos) * nvx;
+                v.y = v.y + (-2.f * Icos) * nvy;
+                v.z = v.z + (-2.f * Icos) * nvz;
+                svupper = !svupper;
+            } else {
+                v.x = v.x + (-Icos) * nvx;
+                v.y = v.y + (-Icos) * nvy;
+                v.z = v.z + (-Icos) * nvz;
+                const float r = n1 / n2;
+                v.x = tmp2 * nvx + r * v.x;
+                v.y = tmp2 * nvy + r * v.y;
+                v.z = tmp2 * nvz + r * v.z;
+                nvx = -nvx;
+                nvy = -nvy;
+                nvz = -nvz;
+                /* the reference tests the side flag itself here, not the
+                 * upper label: only a lower region of label 0 ends the photon */
+                if ((svupper ? svupper : svlo) == 0) return true;
+                prop = tab[svupper ? svup : svlo];
+            }
+        } else {
+            v.x = v.x + (-2.f * Icos) * nvx;
+            v.y = v.y + (-2.f * Icos) * nvy;
+            v.z = v.z + (-2.f * Icos) * nvz;
+            svupper = !svupper;
+        }
+        const float r = 1.f / __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x = v.x * r;
+        v.y = v.y * r;
+        v.z = v.z * r;
+        calc_rv();
+        return false;
+    }
+
+    /* src/mcx_core.cu:582-687: optical properties of voxel word m (voxel
+     * vox) for the non-label formats.  Fields a format does not encode keep
+     * their value in q (a lane's properties start as row 1 of the table).
+     * The mixed-label format draws one random number and narrows m to the
+     * label drawn, so later decodes of the same word keep that label. */
+    __device__ __forceinline__ void decode(F4& q, uint32_t& m, uint32_t vox) {
+        const uint32_t w = m & MED_MASK;
+        const uint32_t fmt = kp.mediaformat;
+        if (fmt == MCX_MEDIA_LABEL_HALF) {
+            q = tab[w & 0x3FFFu];
+            const float h = fabsf(half_to_float(w >> 16));
+            const uint32_t k = (w >> 14) & 3u;
+            if (k == 0) q.x = h;
+            else if (k == 1) q.y = h;
+            else if (k == 2) q.z = h;
+            else q.w = h;
+        } else if (fmt == MCX_MEDIA_MUA_FLOAT) {
+            q.x = fabsf(mcx_bits2f(m));
+            q.w = tab[w != 0u].w;
+        } else if (fmt == MCX_MEDIA_AS_F2H || fmt == MCX_MEDIA_AS_HALF) {
+            q.x = fabsf(half_to_float(w & 0xFFFFu));
+            q.y = fabsf(half_to_float(w >> 16));
+            q.w = tab[w != 0u].w;
+        } else if (fmt == MCX_MEDIA_ASGN_F2H) {
+            if (vox == OUT_MIN || vox == OUT_MAX) {
+                q = tab[0];
+                return;
+            }
+            const uint32_t w2 = load_media<MediaWord>(kp.media, vox + kp.nvox);
+            q.x = fabsf(half_to_float(w & 0xFFFFu));
+            q.y = fabsf(half_to_float(w >> 16));
+            q.z = fabsf(half_to_float(w2 & 0xFFFFu));
+            q.w = fabsf(half_to_float(w2 >> 16));
+        } else if (fmt == MCX_MEDIA_2LABEL_MIX) {
+            const uint32_t frac = w >> 16;
+            if (frac > 0u) {
+                if (rnd() * 32767.f < (float)frac) {
+                    q = tab[(w >> 8) & 0xFFu];
+                    m >>= 8;
+                } else {
+                    q = tab[w & 0xFFu];
+                }
+                m &= 0xFFFFu;
+            } else {
+                q = tab[w & 0xFFu];
+            }
+        } else if (fmt == MCX_MEDIA_ASGN_BYTE) {
+            const F4 a = tab[1], b = tab[2];
+            q.x = (float)(w & 0xFFu) * (1.f / 255.f) * (b.x - a.x) + a.x;
+            q.y = (float)((w >> 8) & 0xFFu) * (1.f / 255.f) * (b.y - a.y) + a.y;
+            q.z = (float)((w >> 16) & 0xFFu) * (1.f / 255.f) * (b.z - a.z) + a.z;
+            q.w = (float)(w >> 24) * (1.f / 127.f) * (b.w - a.w) + a.w;
+        } else if (fmt == MCX_MEDIA_AS_SHORT) {
+            const F4 a = tab[1], b = tab[2];
+            q.x = (float)(w & 0xFFFFu) * (1.f / 65535.f) * (b.x - a.x) + a.x;
+            q.y = (float)(w >> 16) * (1.f / 65535.f) * (b.y - a.y) + a.y;
+            q.w = tab[w != 0u].w;
+        }
+    }
+
+    __device__ __forceinline__ void setprop(uint32_t& m, uint32_t vox) {
+        if (SPLIT)
+            svmc_update(prop, m, vox, p, fd0, fd1, fd2);
+        else if (WORDS)
+            decode(prop, m, vox);
+        else
+            prop = tab[m & MED_MASK];
+    }
     __device__ __forceinline__ void calc_rv() {
         rv0 = 1.f / v.x;
         rv1 = 1.f / v.y;
@@ -301,7 +518,18 @@ struct Lane {
                      * entry (the reference would read an out-of-range label) */
                     if (!((uint32_t)i0 < kp.nx && (uint32_t)i1 < kp.ny && (uint32_t)i2 < kp.nz)) return -1;
                     f.y = kp.voidtime ? f.y : 0.f;
-                    const F4 h = tab[load_media<MT>(kp.media, id) & MED_MASK];
+                    F4 h;
+                    if (SPLIT) {
+                        svmc_update(h, load_media<MT>(kp.media, id), id, p, i0, i1, i2);
+                    } else if (WORDS) {
+                        /* src/mcx_core.cu:921-922 (the word is decoded into a
+                         * local: the volume stays read-only) */
+                        uint32_t w = load_media<MT>(kp.media, id);
+                        h = tab[1];
+                        decode(h, w, id);
+                    } else {
+                        h = tab[load_media<MT>(kp.media, id) & MED_MASK];
+                    }
                     if (kp.isspecular && h.w != tab[0].w) {
                         p.w *= 1.f - reflectcoeff(tab[0].w, h.w, face);
                         if (p.w > EPSF) transmit(tab[0].w, h.w, face);
@@ -590,7 +818,7 @@ struct Lane {
             deposit(idx1d, ts, -p.w);
         }
         if (kp.savedet) {
-            savedet(live && (isdet & DET_MASK) == DET_MASK && mediaid == 0, isdet);
+            savedet(live && (isdet & DET_MASK) == DET_MASK && (mediaid == 0 || (SPLIT && sv_label() == 0)), isdet);
             for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;
         }
         if ((int)f.w >= nphoton - 1) return 1;
@@ -615,6 +843,7 @@ struct Lane {
             f = {0.f, 0.f, kp.minaccumtime, f.w};
             idx1d = mcx_f2bits(kp.srcparam2[2]);
             mediaid = mcx_f2bits(kp.srcparam2[3]);
+            if (SPLIT) svsplit = svlo = svup = svupper = 0;
             /* ISPENCIL 2: decided per launch parameter block (batched launch) */
             if (!(ISPENCIL == 1 || (ISPENCIL == 2 && kp.srctype == 0))) {
                 bool skip;
@@ -634,7 +863,7 @@ struct Lane {
             if (w0 > (float)kp.maxvoidstep) return -1;
         } while ((mediaid & MED_MASK) == 0 || fabsf(p.w) <= kp.minenergy);
         f.w += 1.f;
-        setprop(mediaid);
+        setprop(mediaid, idx1d);
         launched += p.w;
         w0 = p.w;
         winit = p.w;
@@ -713,6 +942,9 @@ struct Lane {
         tally64 = kp.field64;
         if (TALLY == MCX_TALLY_F64_XCD) tally64 += (size_t)xcc_id() * kp.fieldlen;
         nstep = ndep = nscat = nlaunch = 0;
+        svsplit = svlo = svup = svupper = 0;
+        rpx = rpy = rpz = nvx = nvy = nvz = 0.f;
+        testint = hitintf = false;
         uint32_t idx1dold, mediaidold = 0, isdet = 0;
         float n1, len, slen;
         prop = tab[1];
@@ -738,6 +970,7 @@ struct Lane {
                 isdet = mediaid & DET_MASK;
                 mediaid &= MED_MASK;
                 relaunch = false;
+                testint = true;
             }
             if (!(f.w < (float)nphoton)) break;
             nstep++;
@@ -760,8 +993,11 @@ struct Lane {
                         stheta = mcx_sin_from_cos(ctheta);
                     }
                     if (kp.savedet) {
-                        if (NSCAT) ppath((mediaid & MED_MASK) - 1) += 1.f;
-                        if (MOM) ppath(kp.maxmedia * (NSCAT + PPATH) + (mediaid & MED_MASK) - 1) += 1.f - ctheta;
+                        const uint32_t lab = SPLIT ? sv_label() : (mediaid & MED_MASK);
+                        if (!SPLIT || lab > 0) {
+                            if (NSCAT) ppath(lab - 1) += 1.f;
+                            if (MOM) ppath(kp.maxmedia * (NSCAT + PPATH) + lab - 1) += 1.f - ctheta;
+                        }
                     }
                     if (kp.is2d)
                         rotatevector2d((rnd() > 0.5f ? stheta : -stheta), ctheta);
@@ -784,16 +1020,19 @@ struct Lane {
                     }
                 }
                 v.w = (float)(int)v.w;
+                testint = true;
             }
             n1 = prop.w;
-            setprop(mediaid);
+            /* in a split voxel the side's properties are kept */
+            if (!SPLIT || !svsplit) setprop(mediaid, idx1d);
             len = hitgrid(p, fd0, fd1, fd2, fd3, rv0, rv1, rv2);
             const bool gs = (v.w + 1.f > (float)kp.gscatter);
             slen = len * prop.y * (gs ? (1.f - prop.z) : 1.f);
             slen = fminf(slen, f.x);
             len = ((prop.y == 0.f) ? len : (slen / (prop.y * (gs ? (1.f - prop.z) : 1.f))));
+            if (SPLIT) hitintf = (svsplit && testint) ? ray_plane(len, slen) : false;
             p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
-            const bool stop = (slen == f.x);
+            const bool stop = (slen == f.x) || (SPLIT && hitintf);
             if (fd3 == 0) fd0 += stop ? 0 : (v.x > 0.f ? 1 : -1);
             if (fd3 == 1) fd1 += stop ? 0 : (v.y > 0.f ? 1 : -1);
             if (fd3 == 2) fd2 += stop ? 0 : (v.z > 0.f ? 1 : -1);
@@ -801,7 +1040,10 @@ struct Lane {
             f.x -= slen;
             f.y += len * prop.w * kp.oneoverc0;
             f.z += len;
-            if (kp.savedet && PPATH) ppath(kp.maxmedia * NSCAT + (mediaid & MED_MASK) - 1) += len;
+            if (kp.savedet && PPATH) {
+                const uint32_t lab = SPLIT ? sv_label() : (mediaid & MED_MASK);
+                if (!SPLIT || lab > 0) ppath(kp.maxmedia * NSCAT + lab - 1) += len;
+            }
             mediaidold = mediaid | (isdet & DET_MASK);
             idx1dold = idx1d;
             const bool inside = (uint32_t)fd0 < kp.nx && (uint32_t)fd1 < kp.ny && (uint32_t)fd2 < kp.nz;
@@ -812,7 +1054,7 @@ struct Lane {
              * the tally atomic in flight until the next step's load. */
             const uint32_t label = load_media<MT>(kp.media, inside ? vox : 0u);
             idx1d = inside ? vox : ((fd0 < 0 || fd1 < 0 || fd2 < 0) ? OUT_MIN : OUT_MAX);
-            const bool moved = (idx1d != idx1dold && mediaidold);
+            const bool moved = ((idx1d != idx1dold) || (SPLIT && hitintf)) && mediaidold;
             if (moved) {
                 if (kp.save2pt && f.y >= kp.twin0 && f.y < kp.twin1) {
                     float weight = 0.f;
@@ -846,11 +1088,26 @@ struct Lane {
                 isdet = (kp.bcface >> (4 * ((idx1d == OUT_MAX) * 3 + fd3))) & 0xFu;
             }
             if (!moved) mediaid = mediaidold;
+            /* src/mcx_core.cu:2294-2304: a new voxel gets its split state; a
+             * crossing of the interface moves the photon to the other side */
+            if (SPLIT) {
+                if (idx1d != idx1dold) {
+                    setprop(mediaid, idx1d);
+                    testint = true;
+                } else if (hitintf) {
+                    nvx = -nvx;
+                    nvy = -nvy;
+                    nvz = -nvz;
+                    svupper = !svupper;
+                    testint = false;
+                }
+            }
             const uint32_t bcf = isdet & 0xF;
             if ((mediaid == 0 &&
                  ((!ISREFLECT || (ISREFLECT && n1 == tab[0].w)) || (bcf == bcUnknown && !kp.doreflect) ||
                   bcf == bcAbsorb || bcf == bcCyclic) &&
                  bcf != bcMirror) ||
+                (SPLIT && (idx1d != idx1dold || hitintf) && !svupper && !svlo && (!ISREFLECT || (ISREFLECT && n1 == tab[0].w))) ||
                 f.y > kp.twin1) {
                 if (isdet == bcCyclic) {
                     if (fd3 == 0) {
@@ -887,13 +1144,33 @@ struct Lane {
                     continue;
                 }
             }
-            if (ISREFLECT) {
-                setprop(mediaid);
+            if (ISREFLECT && SPLIT && hitintf) {
+                /* src/mcx_core.cu:2386-2404: at the interface plane */
+                if (tab[svlo].w != tab[svup].w) {
+                    nvx = -nvx;
+                    nvy = -nvy;
+                    nvz = -nvz;
+                    if (reflectray(n1)) {
+                        detarg = mediaidold & DET_MASK;
+                        relaunch = true;
+                        continue;
+                    }
+                } else {
+                    prop = tab[sv_label()];
+                }
+            } else if (ISREFLECT) {
+                /* src/mcx_core.cu:2381-2416: formats from 100 on keep one
+                 * refractive index (row 1) and decode the entered voxel only
+                 * when a reflection test follows; split voxels were decoded
+                 * on entry */
+                const bool lt100 = !WORDS || kp.mediaformat < 100;
+                if (lt100 && !SPLIT) setprop(mediaid, idx1d);
                 if (((mediaid && kp.doreflect) ||
                      (mediaid == 0 && ((bcf == bcUnknown && kp.doreflect) || bcf == bcReflect || bcf == bcMirror))) &&
-                    (bcf == bcMirror || n1 != prop.w)) {
+                    (bcf == bcMirror || n1 != (lt100 ? prop.w : tab[mediaid > 0u ? 1 : 0].w))) {
                     float Rtotal = 1.f;
                     float cphi, sphi, stheta, ctheta, tmp0, tmp1;
+                    if (WORDS && !SPLIT) setprop(mediaid, idx1d);
                     tmp0 = n1 * n1;
                     tmp1 = prop.w * prop.w;
                     cphi = fabsf(vcomp(fd3));
@@ -908,7 +1185,7 @@ struct Lane {
                     }
                     if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && rnd() > Rtotal) {
                         transmit(n1, prop.w, fd3);
-                        if (mediaid == 0) {
+                        if (mediaid == 0 || (SPLIT && sv_label() == 0)) {
                             detarg = (((idx1d == OUT_MAX || idx1d == OUT_MIN) && ((kp.bcdet >> ((idx1d == OUT_MAX) * 3 + fd3)) & 1u))
                                           ? OUT_MIN : (mediaidold & DET_MASK));
                             relaunch = true;
@@ -936,12 +1213,19 @@ struct Lane {
                         }
                         idx1d = idx1dold;
                         mediaid = load_media<MT>(kp.media, idx1d) & MED_MASK;
-                        setprop(mediaid);
+                        setprop(mediaid, idx1d);
+                        if (SPLIT && sv_label() == 0) {
+                            detarg = mediaidold & DET_MASK;
+                            relaunch = true;
+                            continue;
+                        }
                         n1 = prop.w;
                     }
-                } else {
-                    setprop(mediaidold);
+                } else if (lt100 && !SPLIT) {
+                    setprop(mediaidold, idx1d);
                 }
+            } else if (SPLIT) {
+                prop = tab[sv_label()];
             }
             if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
                 atom_add_u32(kp.detcount + 1, 1u);
@@ -1149,8 +1433,12 @@ __global__ void mcx_merge_f64(float* __restrict__ out, const double* __restrict_
 
 typedef void (*mcx_kernel_fn)(const KParam*);
 
+/* media kind: 0 uint8 labels, 1 uint32 labels, 2 non-label 32-bit words,
+ * 3 split voxels */
 template <int P, int R, int T>
 static mcx_kernel_fn pick_mt(int media32) {
+    if (media32 == 3) return (mcx_kernel_fn)mcx_transport<P, R, T, MediaSplit>;
+    if (media32 == 2) return (mcx_kernel_fn)mcx_transport<P, R, T, MediaWord>;
     return media32 ? (mcx_kernel_fn)mcx_transport<P, R, T, uint32_t> : (mcx_kernel_fn)mcx_transport<P, R, T, uint8_t>;
 }
 

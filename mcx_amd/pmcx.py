@@ -14,7 +14,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, media
 from ._lib import Config, Float4, GPUInfo, Medium, mpPython
 
 _libc = ctypes.CDLL(None)
@@ -91,24 +91,18 @@ def _parse_config(user_cfg, cfg):
     if "vol" not in user_cfg:
         raise ValueError("Configuration must specify a 2/3/4D volume.")
     vol = np.asarray(user_cfg["vol"])
-    if vol.ndim == 2:
-        vol = vol[:, :, None]
-    if vol.ndim != 3:
-        raise ValueError("only 2D/3D label volumes are supported by this engine")
-    if vol.dtype in (np.int8, np.uint8, np.bool_):
-        cfg.mediabyte = 1
-    elif vol.dtype in (np.int16, np.uint16):
-        cfg.mediabyte = 2
-    elif vol.dtype in (np.int32, np.uint32, np.int64, np.uint64):
-        cfg.mediabyte = 4
-    elif vol.dtype in (np.float32, np.float64):
-        if not np.all(vol == np.round(vol)):
-            raise ValueError("floating-point media formats are not supported by this engine")
-        cfg.mediabyte = 4
+    if "mediabyte" in user_cfg and int(user_cfg["mediabyte"]) > 4:
+        # raw 32-bit property words (e.g. muamus_half, 102) given as is
+        mb = int(user_cfg["mediabyte"])
+        if vol.ndim not in (2, 3) or mb in (media.MEDIA_ASGN_F2H, media.MEDIA_2LABEL_SPLIT):
+            raise ValueError("raw media words must be a 2D/3D uint32 array of a one-word format")
+        if vol.ndim == 2:
+            vol = vol[:, :, None]
+        flat, dims = np.asfortranarray(vol).ravel(order="F").astype(np.uint32), vol.shape
     else:
-        raise ValueError("unsupported volume data type %s" % vol.dtype)
-    cfg.dim.x, cfg.dim.y, cfg.dim.z = (int(s) for s in vol.shape)
-    flat = np.asfortranarray(vol).ravel(order="F").astype(np.uint32)
+        flat, mb, dims = media.encode_volume(vol, cfg.unitinmm)
+    cfg.mediabyte = mb
+    cfg.dim.x, cfg.dim.y, cfg.dim.z = (int(s) for s in dims)
     cfg.vol = _copy_to_c(flat, ctypes.c_uint)
     if "srcpos" in user_cfg:
         sp = np.asarray(user_cfg["srcpos"], dtype=np.float64)

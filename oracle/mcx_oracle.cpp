@@ -1,8 +1,8 @@
 /*
  * mcx_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
  *
- * CPU restatement of the reference's transport kernel (label media,
- * non-polarised, no split voxels), one OpenMP iteration per reference
+ * CPU restatement of the reference's transport kernel (label and
+ * property-word media, non-polarised), one OpenMP iteration per reference
  * virtual thread, each owning its xorshift128+ stream.  Used by tests/ to
  * check the HIP kernel and by bench.py's cpu_baseline leg.  The product
  * never links this file.
@@ -17,6 +17,7 @@
  *   src/mcx_core.cu:470-478    mcx_nextafterf       -> nextafter_mcx()
  *   src/mcx_core.cu:1011-1031  rotatevector         -> rotatevector()
  *   src/mcx_core.cu:970-999    rotatevector2d       -> rotatevector2d()
+ *   src/mcx_core.cu:582-687    updateproperty       -> decode()
  *   src/mcx_core.cu:258-270    finddetector         -> finddetector()
  *   src/mcx_core.cu:365-417    savedetphoton        -> savedet()
  *   src/mcx_core.cu:2191-2292  fluence tally        -> deposit()
@@ -238,6 +239,10 @@ struct Thread {
     unsigned int seedsave[4];    /* launch RNG state of the current photon */
     unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
     int err = 0;
+    /* split-voxel state (src/mcx_core.h:65-83) */
+    float rp[3] = {0.f, 0.f, 0.f}, nv[3] = {0.f, 0.f, 0.f};
+    unsigned int svlo = 0, svup = 0, svupper = 0, svsplit = 0;
+    bool testint = false, hitintf = false;
 
     const OracleParams& P() const { return *C->P; }
 
@@ -271,7 +276,183 @@ struct Thread {
         }
     }
 
-    void setprop(unsigned int m) { prop = C->gprop[m & MED_MASK]; }
+    /* IEEE binary16 bits -> float, exact */
+    static float half_to_float(unsigned int h) {
+        const unsigned int e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+        float v;
+        if (e == 0)
+            v = ldexpf((float)m, -24);
+        else if (e == 31)
+            v = m ? NAN : INFINITY;
+        else
+            v = ldexpf((float)(m | 0x400u), (int)e - 25);
+        return ((h >> 15) & 1u) ? -v : v;
+    }
+
+    bool words() const { return P().mediaformat > 4 && P().mediaformat != 97; }
+    bool split() const { return P().mediaformat == 97; }
+    unsigned int sv_label() const { return svupper ? svup : svlo; }
+
+    /* src/mcx_core.cu:688-729 (updateproperty, split voxels) */
+    void svmc_update(Vec4& out, unsigned int w, unsigned int vox, const Vec4& q, int i0, int i1, int i2) {
+        const Ctx& c = *C;
+        if (vox == OUT_MIN || vox == OUT_MAX) {
+            out = c.gprop[0];
+            return;
+        }
+        const unsigned int lo = P().media[(size_t)vox + (size_t)P().nx * P().ny * P().nz];
+        const unsigned int hi = w & MED_MASK;
+        svlo = hi >> 24;
+        svup = (hi >> 16) & 0xFFu;
+        if (svup) {
+            rp[0] = (float)((hi >> 8) & 0xFFu) * (1.f / 255.f);
+            rp[1] = (float)(hi & 0xFFu) * (1.f / 255.f);
+            rp[2] = (float)(lo >> 24) * (1.f / 255.f);
+            rp[0] += (float)i0;
+            rp[1] += (float)i1;
+            rp[2] += (float)i2;
+            nv[0] = (float)((lo >> 16) & 0xFFu) * (2.f / 255.f) - 1.f;
+            nv[1] = (float)((lo >> 8) & 0xFFu) * (2.f / 255.f) - 1.f;
+            nv[2] = (float)(lo & 0xFFu) * (2.f / 255.f) - 1.f;
+            const float r = 1.f / sqrtf(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            for (int k = 0; k < 3; k++) nv[k] = nv[k] * r;
+            if ((rp[0] - q.x) * nv[0] + (rp[1] - q.y) * nv[1] + (rp[2] - q.z) * nv[2] < 0.f) {
+                out = c.gprop[svup];
+                svupper = 1;
+                for (int k = 0; k < 3; k++) nv[k] = -nv[k];
+            } else {
+                out = c.gprop[svlo];
+                svupper = 0;
+            }
+            svsplit = 1;
+        } else {
+            out = c.gprop[svlo];
+            svsplit = 0;
+            svupper = 0;
+        }
+    }
+
+    /* src/mcx_core.cu:747-766 (ray_plane_intersect) */
+    bool ray_plane(float& len, float& slen) const {
+        if (v.x * nv[0] + v.y * nv[1] + v.z * nv[2] <= 0.f) return false;
+        const float p1x = p.x + len * v.x, p1y = p.y + len * v.y, p1z = p.z + len * v.z;
+        const float d0 = (p.x - rp[0]) * nv[0] + (p.y - rp[1]) * nv[1] + (p.z - rp[2]) * nv[2];
+        const float d1 = (p1x - rp[0]) * nv[0] + (p1y - rp[1]) * nv[1] + (p1z - rp[2]) * nv[2];
+        if (d0 * d1 > 0.f) return false;
+        const float len0 = len * d0 / (d0 - d1);
+        len = (len0 > 0.f) ? len0 : len;
+        slen = len * prop.y * ((v.w + 1.f > (float)P().gscatter) ? (1.f - prop.z) : 1.f);
+        return true;
+    }
+
+    /* src/mcx_core.cu:782-828 (reflectray); true: transmitted into the background */
+    bool reflectray(float n1) {
+        const Ctx& c = *C;
+        const float Icos = fabsf(v.x * nv[0] + v.y * nv[1] + v.z * nv[2]);
+        const float n2 = c.gprop[svupper ? svup : svlo].w;
+        const float tmp0 = n1 * n1, tmp1 = n2 * n2;
+        float tmp2 = 1.f - tmp0 / tmp1 * (1.f - Icos * Icos);
+        float* vv[3] = {&v.x, &v.y, &v.z};
+        if (tmp2 > 0.f) {
+            float Re = tmp0 * Icos * Icos + tmp1 * tmp2;
+            tmp2 = sqrtf(tmp2);
+            const float Im = 2.f * n1 * n2 * Icos * tmp2;
+            float Rtotal = (Re - Im) / (Re + Im);
+            Re = tmp1 * Icos * Icos + tmp0 * tmp2 * tmp2;
+            Rtotal = (Rtotal + (Re - Im) / (Re + Im)) * 0.5f;
+            if (t.next() <= Rtotal) {
+                for (int k = 0; k < 3; k++) *vv[k] = *vv[k] + (-2.f * Icos) * nv[k];
+                svupper = !svupper;
+            } else {
+                for (int k = 0; k < 3; k++) *vv[k] = *vv[k] + (-Icos) * nv[k];
+                const float r = n1 / n2;
+                for (int k = 0; k < 3; k++) *vv[k] = tmp2 * nv[k] + r * *vv[k];
+                for (int k = 0; k < 3; k++) nv[k] = -nv[k];
+                if ((svupper ? svupper : svlo) == 0) return true;
+                prop = c.gprop[svupper ? svup : svlo];
+            }
+        } else {
+            for (int k = 0; k < 3; k++) *vv[k] = *vv[k] + (-2.f * Icos) * nv[k];
+            svupper = !svupper;
+        }
+        const float r = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        v.x = v.x * r;
+        v.y = v.y * r;
+        v.z = v.z * r;
+        rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
+        return false;
+    }
+
+    /* src/mcx_core.cu:582-687 (updateproperty, non-label formats): fields a
+     * format does not encode keep their value in q; the mixed-label format
+     * draws one number and narrows m to the label drawn */
+    void decode(Vec4& q, unsigned int& m, unsigned int vox) {
+        const Ctx& c = *C;
+        const unsigned int w = m & MED_MASK;
+        const int fmt = P().mediaformat;
+        if (fmt == 99) {                       /* label + one half-precision property */
+            q = c.gprop[w & 0x3FFFu];
+            const float h = fabsf(half_to_float(w >> 16));
+            switch ((w >> 14) & 3u) {
+                case 0: q.x = h; break;
+                case 1: q.y = h; break;
+                case 2: q.z = h; break;
+                default: q.w = h;
+            }
+        } else if (fmt == 101) {               /* float mua */
+            float f;
+            memcpy(&f, &m, 4);
+            q.x = fabsf(f);
+            q.w = c.gprop[w != 0u ? 1 : 0].w;
+        } else if (fmt == 100 || fmt == 102) { /* half mua, mus */
+            q.x = fabsf(half_to_float(w & 0xFFFFu));
+            q.y = fabsf(half_to_float(w >> 16));
+            q.w = c.gprop[w != 0u ? 1 : 0].w;
+        } else if (fmt == 96) {                /* half mua, mus, g, n in two words */
+            if (vox == OUT_MIN || vox == OUT_MAX) {
+                q = c.gprop[0];
+                return;
+            }
+            const unsigned int w2 = P().media[(size_t)vox + (size_t)P().nx * P().ny * P().nz];
+            q.x = fabsf(half_to_float(w & 0xFFFFu));
+            q.y = fabsf(half_to_float(w >> 16));
+            q.z = fabsf(half_to_float(w2 & 0xFFFFu));
+            q.w = fabsf(half_to_float(w2 >> 16));
+        } else if (fmt == 98) {                /* two labels mixed by a fraction */
+            const unsigned int frac = w >> 16;
+            if (frac > 0u) {
+                if (t.next() * 32767.f < (float)frac) {
+                    q = c.gprop[(w >> 8) & 0xFFu];
+                    m >>= 8;
+                } else {
+                    q = c.gprop[w & 0xFFu];
+                }
+                m &= 0xFFFFu;
+            } else {
+                q = c.gprop[w & 0xFFu];
+            }
+        } else if (fmt == 103) {               /* byte fractions between rows 1 and 2 */
+            const Vec4 a = c.gprop[1], b = c.gprop[2];
+            q.x = (float)(w & 0xFFu) * (1.f / 255.f) * (b.x - a.x) + a.x;
+            q.y = (float)((w >> 8) & 0xFFu) * (1.f / 255.f) * (b.y - a.y) + a.y;
+            q.z = (float)((w >> 16) & 0xFFu) * (1.f / 255.f) * (b.z - a.z) + a.z;
+            q.w = (float)(w >> 24) * (1.f / 127.f) * (b.w - a.w) + a.w;
+        } else if (fmt == 104) {               /* short fractions between rows 1 and 2 */
+            const Vec4 a = c.gprop[1], b = c.gprop[2];
+            q.x = (float)(w & 0xFFFFu) * (1.f / 65535.f) * (b.x - a.x) + a.x;
+            q.y = (float)(w >> 16) * (1.f / 65535.f) * (b.y - a.y) + a.y;
+            q.w = c.gprop[w != 0u ? 1 : 0].w;
+        }
+    }
+
+    void setprop(unsigned int& m, unsigned int vox) {
+        if (split())
+            svmc_update(prop, m, vox, p, flipdir[0], flipdir[1], flipdir[2]);
+        else if (words())
+            decode(prop, m, vox);
+        else
+            prop = C->gprop[m & MED_MASK];
+    }
 
     /* src/mcx_core.cu:866-956 */
     int skipvoid() {
@@ -301,7 +482,17 @@ struct Thread {
                     /* entry search gave up outside the grid: no tissue entry */
                     if (!((unsigned)id[0] < Q.nx && (unsigned)id[1] < Q.ny && (unsigned)id[2] < Q.nz)) return -1;
                     f.y = Q.voidtime ? f.y : 0.f;
-                    Vec4 h = C->gprop[Q.media[i1] & MED_MASK];
+                    Vec4 h;
+                    if (split()) {
+                        svmc_update(h, Q.media[i1], i1, p, id[0], id[1], id[2]);
+                    } else if (words()) {
+                        /* src/mcx_core.cu:921-922, decoded into a local */
+                        unsigned int w = Q.media[i1];
+                        h = C->gprop[1];
+                        decode(h, w, i1);
+                    } else {
+                        h = C->gprop[Q.media[i1] & MED_MASK];
+                    }
                     if (Q.isspecular && h.w != C->gprop[0].w) {
                         p.w *= 1.f - reflectcoeff(v, C->gprop[0].w, h.w, id[3]);
                         if (p.w > EPSF) transmit(v, C->gprop[0].w, h.w, id[3]);
@@ -341,7 +532,7 @@ struct Thread {
                 int ts = std::max(0, std::min((int)Q.maxgate - 1, (int)floorf((f.y - Q.twin0) * Q.Rtstep)));
                 deposit(idx1d, ts, -p.w);
             }
-            if (Q.savedet && (isdet & DET_MASK) == DET_MASK && mediaid == 0) savedet(isdet);
+            if (Q.savedet && (isdet & DET_MASK) == DET_MASK && (mediaid == 0 || (split() && sv_label() == 0))) savedet(isdet);
         }
         if (Q.savedet)
             for (unsigned int i = 0; i < C->partialdata; i++) ppath[i] = 0.f;
@@ -359,6 +550,7 @@ struct Thread {
             f = {0.f, 0.f, Q.minaccumtime, f.w};
             memcpy(&idx1d, &Q.srcparam2[2], 4);
             memcpy(&mediaid, &Q.srcparam2[3], 4);
+            if (split()) svsplit = svlo = svup = svupper = 0;
             float org[3] = {Q.srcpos[0], Q.srcpos[1], Q.srcpos[2]};
             if (!ispencil) {
                 const float* s1 = Q.srcparam1;
@@ -616,7 +808,7 @@ struct Thread {
             if (w0 > (float)Q.maxvoidstep) return -1;
         } while ((mediaid & MED_MASK) == 0 || fabsf(p.w) <= Q.minenergy);
         f.w += 1.f;
-        setprop(mediaid);
+        setprop(mediaid, idx1d);
         pp[1] += p.w;
         w0 = p.w;
         pp[2] = p.w;
@@ -684,8 +876,11 @@ struct Thread {
                         stheta = om_sin_from_cos(ctheta);
                     }
                     if (Q.savedet) {
-                        if (NSCAT) ppath[(mediaid & MED_MASK) - 1] += 1.f;
-                        if (MOM) ppath[c.maxmedia * (NSCAT + PPATH) + (mediaid & MED_MASK) - 1] += 1.f - ctheta;
+                        const unsigned int lab = split() ? sv_label() : (mediaid & MED_MASK);
+                        if (!split() || lab > 0) {
+                            if (NSCAT) ppath[lab - 1] += 1.f;
+                            if (MOM) ppath[c.maxmedia * (NSCAT + PPATH) + lab - 1] += 1.f - ctheta;
+                        }
                     }
                     if (Q.is2d)
                         rotatevector2d(v, Q.is2d, (t.next() > 0.5f ? stheta : -stheta), ctheta);
@@ -707,23 +902,30 @@ struct Thread {
                     }
                 }
                 v.w = (float)(int)v.w;
+                testint = true;
             }
             n1 = prop.w;
-            setprop(mediaid);
+            if (!split() || !svsplit) setprop(mediaid, idx1d);
             len = hitgrid(p, v, rv, flipdir);
             const bool gs = (v.w + 1.f > (float)Q.gscatter);
             slen = len * prop.y * (gs ? (1.f - prop.z) : 1.f);
             slen = fminf(slen, f.x);
             len = ((prop.y == 0.f) ? len : (slen / (prop.y * (gs ? (1.f - prop.z) : 1.f))));
+            /* src/mcx_core.cu:2112-2117: the interface plane of a split voxel */
+            hitintf = split() && svsplit && testint && ray_plane(len, slen);
             p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
-            if (flipdir[3] == 0) flipdir[0] += (slen == f.x) ? 0 : (v.x > 0.f ? 1 : -1);
-            if (flipdir[3] == 1) flipdir[1] += (slen == f.x) ? 0 : (v.y > 0.f ? 1 : -1);
-            if (flipdir[3] == 2) flipdir[2] += (slen == f.x) ? 0 : (v.z > 0.f ? 1 : -1);
+            const bool stay = (slen == f.x) || hitintf;
+            if (flipdir[3] == 0) flipdir[0] += stay ? 0 : (v.x > 0.f ? 1 : -1);
+            if (flipdir[3] == 1) flipdir[1] += stay ? 0 : (v.y > 0.f ? 1 : -1);
+            if (flipdir[3] == 2) flipdir[2] += stay ? 0 : (v.z > 0.f ? 1 : -1);
             p.w *= om_expf(-prop.x * len);
             f.x -= slen;
             f.y += len * prop.w * Q.oneoverc0;
             f.z += len;
-            if (Q.savedet && PPATH) ppath[c.maxmedia * NSCAT + (mediaid & MED_MASK) - 1] += len;
+            if (Q.savedet && PPATH) {
+                const unsigned int lab = split() ? sv_label() : (mediaid & MED_MASK);
+                if (!split() || lab > 0) ppath[c.maxmedia * NSCAT + lab - 1] += len;
+            }
             mediaidold = mediaid | (isdet & DET_MASK);
             idx1dold = idx1d;
             idx1d = (unsigned)(flipdir[2] * (int)c.dimlen_y + flipdir[1] * (int)c.dimlen_x + flipdir[0]);
@@ -736,7 +938,7 @@ struct Thread {
                 isdet = mediaid & DET_MASK;
                 mediaid &= MED_MASK;
             }
-            if (idx1d != idx1dold && mediaidold) {
+            if ((idx1d != idx1dold || hitintf) && mediaidold) {
                 if (Q.save2pt && f.y >= Q.twin0 && f.y < Q.twin1) {
                     float weight = 0.f;
                     int tshift = (int)floorf((f.y - Q.twin0) * Q.Rtstep);
@@ -761,11 +963,24 @@ struct Thread {
             } else {
                 mediaid = mediaidold;
             }
+            /* src/mcx_core.cu:2294-2304 */
+            if (split()) {
+                if (idx1d != idx1dold) {
+                    setprop(mediaid, idx1d);
+                    testint = true;
+                } else if (hitintf) {
+                    for (int k = 0; k < 3; k++) nv[k] = -nv[k];
+                    svupper = !svupper;
+                    testint = false;
+                }
+            }
             const unsigned int bcf = isdet & 0xF;
             if ((mediaid == 0 &&
                  ((!Q.isreflect || (Q.isreflect && n1 == c.gprop[0].w)) || (bcf == bcUnknown && !Q.doreflect) ||
                   bcf == bcAbsorb || bcf == bcCyclic) &&
                  bcf != bcMirror) ||
+                (split() && (idx1d != idx1dold || hitintf) && !svupper && !svlo &&
+                 (!Q.isreflect || (Q.isreflect && n1 == c.gprop[0].w))) ||
                 f.y > Q.twin1) {
                 if (isdet == bcCyclic) {
                     if (flipdir[3] == 0) {
@@ -805,14 +1020,29 @@ struct Thread {
                     continue;
                 }
             }
-            if (Q.isreflect) {
-                setprop(mediaid);
+            if (Q.isreflect && hitintf) {
+                /* src/mcx_core.cu:2386-2404 */
+                if (c.gprop[svlo].w != c.gprop[svup].w) {
+                    for (int k = 0; k < 3; k++) nv[k] = -nv[k];
+                    if (reflectray(n1)) {
+                        if (launch(mediaidold & DET_MASK)) break;
+                        isdet = mediaid & DET_MASK;
+                        mediaid &= MED_MASK;
+                        continue;
+                    }
+                } else {
+                    prop = c.gprop[sv_label()];
+                }
+            } else if (Q.isreflect) {
+                /* src/mcx_core.cu:2381-2416 */
+                const bool lt100 = Q.mediaformat < 100;
+                if (lt100 && !split()) setprop(mediaid, idx1d);
                 if (((mediaid && Q.doreflect) ||
                      (mediaid == 0 && ((bcf == bcUnknown && Q.doreflect) || bcf == bcReflect || bcf == bcMirror))) &&
-                    (bcf == bcMirror || n1 != prop.w)) {
+                    (bcf == bcMirror || n1 != (lt100 ? prop.w : c.gprop[mediaid > 0u ? 1 : 0].w))) {
                     float Rtotal = 1.f;
                     float cphi, sphi, stheta, ctheta, tmp0, tmp1;
-                    setprop(mediaid);
+                    if (!split()) setprop(mediaid, idx1d);
                     tmp0 = n1 * n1;
                     tmp1 = prop.w * prop.w;
                     cphi = fabsf(comp(v, flipdir[3]));
@@ -827,7 +1057,7 @@ struct Thread {
                     }
                     if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && t.next() > Rtotal) {
                         transmit(v, n1, prop.w, flipdir[3]);
-                        if (mediaid == 0) {
+                        if (mediaid == 0 || (split() && sv_label() == 0)) {
                             unsigned int detarg = (((idx1d == OUT_MAX && Q.bc[9 + flipdir[3]]) || (idx1d == OUT_MIN && Q.bc[6 + flipdir[3]]))
                                                        ? OUT_MIN : (mediaidold & DET_MASK));
                             if (launch(detarg)) break;
@@ -852,12 +1082,20 @@ struct Thread {
                         }
                         idx1d = idx1dold;
                         mediaid = Q.media[idx1d] & MED_MASK;
-                        setprop(mediaid);
+                        setprop(mediaid, idx1d);
+                        if (split() && sv_label() == 0) {
+                            if (launch(mediaidold & DET_MASK)) break;
+                            isdet = mediaid & DET_MASK;
+                            mediaid &= MED_MASK;
+                            continue;
+                        }
                         n1 = prop.w;
                     }
-                } else {
-                    setprop(mediaidold);
+                } else if (lt100 && !split()) {
+                    setprop(mediaidold, idx1d);
                 }
+            } else if (split()) {
+                prop = c.gprop[sv_label()];
             }
             if (mediaid == 0 || idx1d == OUT_MIN || idx1d == OUT_MAX) {
                 err++;
@@ -899,7 +1137,7 @@ void mcx_oracle_rng_floats(const unsigned int* seed4, unsigned int n, float* out
 }
 
 void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, unsigned int nz,
-                        const float* detpos, unsigned int detnum) {
+                        const float* detpos, unsigned int detnum, int svmc) {
     const float corners[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
     unsigned int dx = nx + 2, dy = ny + 2, dz = nz + 2;
     int isonecube = (nx == 1) && (ny == 1) && (nz == 1);
@@ -932,15 +1170,18 @@ void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, uns
                     float lim = radius + 0.5f * (1.f + isonecube);
                     if (mind2 == 1.f / EPSF || mind2 >= lim * lim) continue;
                     size_t i = (size_t)(int)(iz + 1.f) * dy * dx + (size_t)(int)(iy + 1.f) * dx + (size_t)(int)(ix + 1.f);
-                    if (!pad[i]) continue;
-                    bool interior = true;
+                    /* split voxels (src/mcx_utils.c:4287-4310): background is a
+                     * lower label of 0 */
+                    const unsigned int lower = (pad[i] & 0xFF000000u) >> 24, upper = (pad[i] & 0x00FF0000u) >> 16;
+                    if (svmc ? !(lower || upper) : !pad[i]) continue;
+                    bool interior = !(svmc && !lower && upper);
                     for (int oz = -1; oz <= 1 && interior; oz++)
                         for (int oy = -1; oy <= 1 && interior; oy++)
                             for (int ox = -1; ox <= 1; ox++) {
                                 int nzero = (ox != 0) + (oy != 0) + (oz != 0);
                                 if (nzero == 0) continue;
                                 size_t j = i + (long)oz * dy * dx + (long)oy * dx + ox;
-                                if (!pad[j]) { interior = false; break; }
+                                if (svmc ? !((pad[j] & 0xFF000000u) >> 24) : !pad[j]) { interior = false; break; }
                             }
                     if (!interior)
                         vol[(size_t)(int)iz * ny * nx + (size_t)(int)iy * nx + (int)ix] |= DET_MASK;

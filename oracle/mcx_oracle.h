@@ -22,7 +22,7 @@ extern "C" {
 typedef struct OracleParams {
     /* domain */
     unsigned int nx, ny, nz;
-    const unsigned int* media;   /* nx*ny*nz labels, x fastest, DET_MASK in bit 31 */
+    const unsigned int* media;   /* nx*ny*nz labels or words, x fastest, DET_MASK in bit 31 */
     const float* prop;           /* medianum x {mua,mus,g,n}, grid units */
     unsigned int medianum;
     const float* detpos;         /* detnum x {x,y,z,r}, 0-based grid units */
@@ -61,6 +61,7 @@ typedef struct OracleParams {
     int replaydet;
     const float* srcpattern;     /* pattern / pattern3d source intensities (one pattern), or NULL */
     int issaveref;               /* diffuse reflectance into void voxels (src/mcx_core.cu:1124-1145) */
+    int mediaformat;             /* 0-4: labels; 96-104: property words (96, 97: 2 words per voxel) */
 } OracleParams;
 
 typedef struct OracleOut {
@@ -94,7 +95,7 @@ void mcx_oracle_rng_floats(const unsigned int* seed4, unsigned int n, float* out
 /* The detector mask (src/mcx_utils.c:4214-4340) on a uint32 column-major
  * label volume, in place. */
 void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, unsigned int nz,
-                        const float* detpos, unsigned int detnum);
+                        const float* detpos, unsigned int detnum, int svmc);
 
 /* Math entry points for tests. */
 float mcx_oracle_expf(float x);

@@ -57,6 +57,7 @@ class OracleParams(ctypes.Structure):
         ("replaydet", ctypes.c_int),
         ("srcpattern", ctypes.POINTER(ctypes.c_float)),
         ("issaveref", ctypes.c_int),
+        ("mediaformat", ctypes.c_int),
     ]
 
 
@@ -95,7 +96,7 @@ def lib():
         L.mcx_oracle_seedwords.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
         L.mcx_oracle_rng_floats.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.POINTER(ctypes.c_float)]
         L.mcx_oracle_maskdet.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
-                                         ctypes.POINTER(ctypes.c_float), ctypes.c_uint]
+                                         ctypes.POINTER(ctypes.c_float), ctypes.c_uint, ctypes.c_int]
         L.mcx_oracle_expf.argtypes = [ctypes.c_float]
         L.mcx_oracle_expf.restype = ctypes.c_float
         L.mcx_oracle_logf.argtypes = [ctypes.c_float]
@@ -128,13 +129,14 @@ def rng_floats(seed4, n):
     return o
 
 
-def maskdet(vol, detpos):
-    """vol: uint32 array indexed [x,y,z] (F-order); detpos: (n,4) 0-based."""
+def maskdet(vol, detpos, svmc=0):
+    """vol: uint32 array indexed [x,y,z] (F-order); detpos: (n,4) 0-based;
+    svmc: vol is the first plane of split-voxel words."""
     v = np.asfortranarray(vol, dtype=np.uint32).copy(order="F")
     flat = v.ravel(order="F")
     d = np.ascontiguousarray(detpos, dtype=np.float32)
     nx, ny, nz = v.shape
-    lib().mcx_oracle_maskdet(_ptr(flat, ctypes.c_uint), nx, ny, nz, _ptr(d, ctypes.c_float), d.shape[0])
+    lib().mcx_oracle_maskdet(_ptr(flat, ctypes.c_uint), nx, ny, nz, _ptr(d, ctypes.c_float), d.shape[0], int(svmc))
     return flat.reshape((nx, ny, nz), order="F")
 
 
@@ -149,7 +151,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         isspecular=0, unitinmm=1.0, outputtype=0, savedetflag=0x5, issavedet=None,
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
-        saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0):
+        saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0, mediaformat=0, words2=None):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -159,6 +161,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     ("seeds", uint8 [n, 16]).  replay: dict with "seed" (uint8 [n, 16]),
     "weight", "tof", "detid" per replayed photon (n = nphoton); with
     replaydet == -1 the field has one block of gates per detector.
+    mediaformat > 4: vol holds the 32-bit property words of that format
+    (words2: the second plane of the two-word formats 96 and 97).
     """
     vol = np.asarray(vol)
     nx, ny, nz = vol.shape
@@ -181,7 +185,13 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     if detnum:
         dp[:detnum] = np.array(detpos, dtype=np.float32)
         if issavedet:
-            media = maskdet(media.reshape((nx, ny, nz), order="F"), dp[:detnum]).ravel(order="F").copy()
+            media = maskdet(media.reshape((nx, ny, nz), order="F"), dp[:detnum],
+                            mediaformat == 97).ravel(order="F").copy()
+    if mediaformat > 4 and mediaformat != 97:
+        # property-word media keep no per-medium record columns
+        savedetflag &= ~(2 | 4 | 8 | 128)
+    if mediaformat in (96, 97):
+        media = np.concatenate([media, np.asfortranarray(np.asarray(words2, dtype=np.uint32)).ravel(order="F")])
     sd = np.array(srcdir, dtype=np.float64)
     nrm = 1.0 / np.sqrt((sd[:3] ** 2).sum())
     srcdir4 = np.array([sd[0] * np.float32(nrm), sd[1] * np.float32(nrm), sd[2] * np.float32(nrm),
@@ -253,6 +263,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     P.ompthreads = ompthreads
     P.saveseed = int(saveseed)
     P.issaveref = int(issaveref)
+    P.mediaformat = int(mediaformat)
     keep = []
     if srcpattern is not None:
         pat = np.ascontiguousarray(np.asarray(srcpattern, dtype=np.float32).ravel(order="F"))

@@ -36,8 +36,26 @@ def oracle_run(cfg, nthread, **extra):
         srcpattern=cfg.get("srcpattern"),
         issaveref=int(cfg.get("issaveref", 0)),
     )
+    vol = cfg["vol"]
+    if np.asarray(vol).ndim == 4 or int(cfg.get("mediabyte", 0)) > 4:
+        # property-word media: the words the engine receives (mcx_amd.media,
+        # pinned by tests/test_media.py)
+        from mcx_amd import media
+        if int(cfg.get("mediabyte", 0)) > 4:
+            words, mb = np.asarray(vol, dtype=np.uint32), int(cfg["mediabyte"])
+            dims = words.shape
+        else:
+            words, mb, dims = media.encode_volume(vol, float(cfg.get("unitinmm", 1.0)))
+        nvox = int(np.prod(dims))
+        flat = np.asarray(words, dtype=np.uint32).ravel(order="F")
+        if mb == media.MEDIA_2LABEL_SPLIT:
+            flat = media.svmc_planes(flat)
+        vol = flat[:nvox].reshape(dims, order="F")
+        kw["mediaformat"] = mb
+        if mb in (media.MEDIA_ASGN_F2H, media.MEDIA_2LABEL_SPLIT):
+            kw["words2"] = flat[nvox:].reshape(dims, order="F")
     kw.update(extra)
-    return oracle_py.run(cfg["vol"], cfg["prop"], cfg["srcpos"], list(cfg["srcdir"]), cfg["nphoton"], nthread,
+    return oracle_py.run(vol, cfg["prop"], cfg["srcpos"], list(cfg["srcdir"]), cfg["nphoton"], nthread,
                          cfg["seed"], **kw)
 
 
