@@ -257,3 +257,30 @@ def test_diffuse_reflectance_tally():
     dref = -f[void].sum()
     esc = seq_sum(r["genergy"][0::2])
     assert 0.5 * esc < dref <= esc * (1 + 1e-9)
+
+
+def _planar(**kw):
+    """cube60planar (src/mcx_bench.c:158-205): a 40x40 planar source 10
+    voxels above the cube, reflection on"""
+    c = dict(srctype="planar", srcpos=[10.0, 10.0, -10.0], srcparam1=[40, 0, 0, 0], srcparam2=[0, 40, 0, 0], reflect=1)
+    c.update(kw)
+    return c
+
+
+@pytest.mark.parametrize("src,band", [
+    (_planar(), (25.0, 26.0)),                                                        # test/testmcx.sh:85
+    (dict(srctype="isotropic", srcpos=[29, 29, 29], reflect=0), (88.0, 89.0)),        # :89
+    (dict(srctype="cone", srcparam1=[0.5, 0, 0, 0], reflect=0,
+          prop=np.array([[0, 0, 1, 1], [0.001, 0.001, 0, 1]], np.float32)), (6.0, 7.0)),  # :93
+    (_planar(srctype="fourier", srcparam1=[40, 0, 0, 2]), (25.0, 26.0)),              # :97
+    (_planar(srctype="pencilarray", srcparam1=[40, 0, 0, 4], srcparam2=[0, 20, 0, 2]), (23.0, 24.0)),  # :101
+])
+def test_source_type_known_answers(src, band):
+    """Absorbed fractions the reference's test script greps for each source
+    type ('absorbed:.*25\\.[0-9]+%' etc.), 2e5 photons."""
+    src = dict(src)
+    reflect = src.pop("reflect")
+    cfg = workloads.cube60(200000, reflect=reflect, **src)
+    r = oracle_run(cfg, 1024, want_field=False)
+    assert r["errors"] == 0
+    assert band[0] <= 100 * r["absorbed"] < band[1], r["absorbed"]
