@@ -60,10 +60,16 @@ def inv_log_feature(f):
     return np.maximum(np.exp(np.asarray(f, dtype=np.float64) * LOG_SCALE - LOG_SHIFT) - LOG_EPS, 0.0)
 
 
-def features(phi, vol, prop, unitinmm=1.0):
+def photon_feature(nphoton):
+    """Input channel 6: the photon count the map was made with, (log10 N - 6) / 2
+    (0 when unknown), so the network can weigh the map's noise level."""
+    return 0.0 if not nphoton else (np.log10(float(nphoton)) - 6.0) / 2.0
+
+
+def features(phi, vol, prop, unitinmm=1.0, photons=None):
     """Input features [X, Y, Z, 8] (float32) of a CW fluence map phi [X, Y, Z]
-    on a label volume with media table prop (per mm; converted to per
-    voxel)."""
+    made with `photons` photons on a label volume with media table prop (per
+    mm; converted to per voxel)."""
     vol = np.asarray(vol)
     prop = np.asarray(prop, dtype=np.float64)
     f = np.zeros(vol.shape + (NFEAT,), dtype=np.float32)
@@ -77,6 +83,7 @@ def features(phi, vol, prop, unitinmm=1.0):
     f[..., 3] = np.where(tissue, (prop[lab, 2] - 0.8) * 10.0, 0.0)
     f[..., 4] = np.where(tissue, (prop[lab, 3] - 1.4) * 10.0, 0.0)
     f[..., 5] = tissue
+    f[..., 6] = photon_feature(photons)
     return f
 
 
@@ -88,7 +95,7 @@ def _media_table(prop, unitinmm):
     return np.ascontiguousarray(tab)
 
 
-def features_gpu(phi, vol, prop, unitinmm=1.0, flip_z=False):
+def features_gpu(phi, vol, prop, unitinmm=1.0, flip_z=False, photons=None):
     """features() computed on the GPU by mcx_dn_features: a BF16 device
     tensor [1, X, Y, Z, 8].  phi and vol are [X, Y, Z] arrays."""
     torch = _torch()
@@ -101,6 +108,8 @@ def features_gpu(phi, vol, prop, unitinmm=1.0, flip_z=False):
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     _chk(L.mcx_dn_features(_ptr(dphi), _ptr(dvol), _ptr(dtab), int(dtab.shape[0]), _ptr(f), X, Y, Z, int(flip_z), LOG_EPS,
                            LOG_SHIFT, LOG_SCALE, st), "features")
+    if photons:
+        f[..., 6].fill_(photon_feature(photons))
     return f
 
 

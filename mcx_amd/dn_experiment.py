@@ -72,16 +72,17 @@ def cw_fluence(results, cfgs):
     return out
 
 
-def simulate(cfgs, nphoton, seed_salt, tally=2):
+def simulate(cfgs, nphoton, seed_salt, tally=2, minenergy=0.0):
     """Run every configuration with nphoton photons (a number, or one per
-    configuration) and a fresh seed; returns (CW fluence maps, kernel ms,
-    photons)."""
+    configuration) and a fresh seed, with Russian roulette below minenergy;
+    returns (CW fluence maps, kernel ms, photons)."""
     import mcx_amd
     runs = []
     counts = list(nphoton) if np.ndim(nphoton) else [nphoton] * len(cfgs)
     for i, c in enumerate(cfgs):
         d = {k: v for k, v in c.items() if not k.startswith("_")}
         d["nphoton"] = int(counts[i])
+        d["minenergy"] = float(minenergy)
         d["seed"] = int((c["seed"] * 1000003 + seed_salt * 7919 + i) % (2 ** 31 - 2)) + 1
         d["quiet"] = True
         runs.append(d)
@@ -103,10 +104,11 @@ def median_error(est, ref, mask):
     return float(np.median(np.abs(est[mask] - ref[mask]) / ref[mask]))
 
 
-def make_features(phis, cfgs, flip_z=False):
+def make_features(phis, cfgs, photons=None, flip_z=False):
     f = []
-    for phi, c in zip(phis, cfgs):
-        x = denoise.features(crop_flip(phi, c), crop_flip(c["vol"], c), c["prop"], float(c.get("unitinmm", 1.0)))
+    counts = list(photons) if np.ndim(photons) else [photons] * len(cfgs)
+    for phi, c, n in zip(phis, cfgs, counts):
+        x = denoise.features(crop_flip(phi, c), crop_flip(c["vol"], c), c["prop"], float(c.get("unitinmm", 1.0)), n)
         f.append(x[:, :, ::-1] if flip_z else x)
     return np.stack(f)
 
@@ -157,8 +159,9 @@ class HeadDenoiser:
         import torch
         return torch.from_numpy(np.asfortranarray(phi, dtype=np.float32).ravel(order="F")).cuda()
 
-    def run(self, dphi):
-        """device fluence (x-fastest) -> device denoised fluence"""
+    def run(self, dphi, photons=None):
+        """device fluence (x-fastest) made with `photons` photons -> device
+        denoised fluence"""
         import torch
         X, Y, Z = self.shape
         f = torch.empty((1, X, Y, Z, denoise.NFEAT), dtype=torch.bfloat16, device="cuda")
@@ -167,6 +170,8 @@ class HeadDenoiser:
         denoise._chk(L.mcx_dn_features(denoise._ptr(dphi), denoise._ptr(self.vol), denoise._ptr(self.tab),
                                        int(self.tab.shape[0]), denoise._ptr(f), X, Y, Z, 1, denoise.LOG_EPS,
                                        denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "features")
+        if photons:
+            f[..., 6].fill_(denoise.photon_feature(photons))
         pred = self.net.predict(f)
         out = torch.empty(X * Y * Z, dtype=torch.float32, device="cuda")
         denoise._chk(L.mcx_dn_inv_features(denoise._ptr(pred), denoise._ptr(out), X, Y, Z, 1, denoise.LOG_EPS,
@@ -270,10 +275,11 @@ def head_eval(net, args, minenergy, log):
     for k, nph in enumerate((args.head_low, 2 * args.head_low, 4 * args.head_low)):
         low_map, low_ms_h = plain(nph, 11 + k)
         dphi = dn.upload(low_map)
-        dn.run(dphi)  # warm-up (buffer allocation)
+        pf = nph if args.photon_feature else None
+        dn.run(dphi, pf)  # warm-up (buffer allocation)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        dout = dn.run(dphi)
+        dout = dn.run(dphi, pf)
         torch.cuda.synchronize()
         dn_ms = (time.perf_counter() - t0) * 1e3
         dmap = dn.download(dout)
@@ -333,6 +339,9 @@ def main(argv=None):
     ap.add_argument("--load-weights", default=None, help="skip training: load parameters saved by --weights")
     ap.add_argument("--head-minenergy", default="0", help="comma list of roulette thresholds for the head runs "
                                                           "(applied to the plain and the denoised arm alike)")
+    ap.add_argument("--minenergy", type=float, default=0.0, help="roulette threshold of the training and held-out runs")
+    ap.add_argument("--low-span", default="3,10", help="training inputs: photons log-uniform in [low/a, low*b]")
+    ap.add_argument("--photon-feature", type=int, default=0, help="give the network the input photon count")
     args = ap.parse_args(argv)
     import torch
 
@@ -356,12 +365,13 @@ def main(argv=None):
     # input photon counts log-uniform in [low/3, low*10]: the network sees a
     # range of noise levels
     gl = np.random.default_rng(2024)
-    lows = np.exp(gl.uniform(np.log(args.low / 3), np.log(args.low * 10), size=len(tr)))
-    lo, lo_ms, _ = simulate(tr, lows, 1)
-    log("train inputs: %d x %.0e-%.0e photons, %.0f ms" % (len(tr), args.low / 3, args.low * 10, lo_ms))
-    hi, hi_ms, hi_ph = simulate(tr, args.high, 2)
+    sa, sb = (float(v) for v in args.low_span.split(","))
+    lows = np.exp(gl.uniform(np.log(args.low / sa), np.log(args.low * sb), size=len(tr)))
+    lo, lo_ms, _ = simulate(tr, lows, 1, minenergy=args.minenergy)
+    log("train inputs: %d x %.0e-%.0e photons, %.0f ms" % (len(tr), args.low / sa, args.low * sb, lo_ms))
+    hi, hi_ms, hi_ph = simulate(tr, args.high, 2, minenergy=args.minenergy)
     log("train targets: %d x %.0e photons, %.0f ms (%.0f photons/ms)" % (len(tr), args.high, hi_ms, hi_ph / hi_ms))
-    X = to_device_bf16(make_features(lo, tr))
+    X = to_device_bf16(make_features(lo, tr, lows if args.photon_feature else None))
     T = torch.from_numpy(make_targets(hi, tr)).to("cuda")
     del lo, hi
     net = denoise.UNet(seed=2024)
@@ -378,12 +388,13 @@ def main(argv=None):
 def _evaluate(net, args, R, log, t_all):
     # ---- held-out volumes ----------------------------------------------------
     ev = [training_volume(4048, i, args.edge) for i in range(args.neval)]
-    refs = [simulate(ev, args.ref, 100 + k)[0] for k in range(args.refruns)]
-    low, low_ms, _ = simulate(ev, args.low, 3)
-    p10, p10_ms, _ = simulate(ev, args.low * 10, 4)
-    p100, p100_ms, _ = simulate(ev, args.low * 100, 5)
+    me = args.minenergy
+    refs = [simulate(ev, args.ref, 100 + k, minenergy=me)[0] for k in range(args.refruns)]
+    low, low_ms, _ = simulate(ev, args.low, 3, minenergy=me)
+    p10, p10_ms, _ = simulate(ev, args.low * 10, 4, minenergy=me)
+    p100, p100_ms, _ = simulate(ev, args.low * 100, 5, minenergy=me)
     t0 = time.perf_counter()
-    den = denoise_maps(net, make_features(low, ev))
+    den = denoise_maps(net, make_features(low, ev, args.low if args.photon_feature else None))
     den_s = time.perf_counter() - t0
     rows = []
     for i in range(len(ev)):

@@ -316,7 +316,9 @@ def test_denoise_experiment_helpers():
     assert np.array_equal(c1["vol"], c2["vol"]) and c1["vol"].shape == (64, 64, 64)
     assert c1["tend"] == pytest.approx(5e-9) and c1["tstep"] == pytest.approx(5e-9)
     f = denoise.features(np.ones((64, 64, 64)), c1["vol"], c1["prop"])
-    assert f.shape == (64, 64, 64, denoise.NFEAT) and np.all(f[..., 5] == 1.0)
+    assert f.shape == (64, 64, 64, denoise.NFEAT) and np.all(f[..., 5] == 1.0) and np.all(f[..., 6] == 0.0)
+    f = denoise.features(np.ones((4, 4, 4)), c1["vol"][:4, :4, :4], c1["prop"], photons=1e8)
+    assert np.all(f[..., 6] == 1.0) and denoise.photon_feature(1e4) == pytest.approx(-1.0)
     from mcx_amd import workloads
     h = workloads.head256(n=64)
     assert set(np.unique(h["vol"])) <= {0, 1, 2, 3, 4, 5} and h["unitinmm"] == 0.5
