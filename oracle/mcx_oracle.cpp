@@ -1238,6 +1238,7 @@ int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
         T.ppath[c.partialdata + 1] = O->genergy[2 * i + 1];
         T.run();
         nstep += T.nstep; ndep += T.ndep; nscat += T.nscat; nlaunch += T.nlaunch;
+        if (O->thread_steps) O->thread_steps[i] = T.nstep;
         errors += T.err;
     }
     auto t1 = std::chrono::steady_clock::now();

@@ -75,6 +75,7 @@ typedef struct OracleOut {
     unsigned long long nstep, ndeposit, nscatter, nlaunch;
     double seconds;       /* wall time of the transport loop */
     unsigned int* seeddata;  /* maxdetphoton*4 words (with saveseed), or NULL */
+    unsigned long long* thread_steps;  /* nthread step counts (load balance), or NULL */
 } OracleOut;
 
 /* Run the transport loop.  Returns 0 on success, <0 on bad input. */

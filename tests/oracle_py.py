@@ -74,6 +74,7 @@ class OracleOut(ctypes.Structure):
         ("nscatter", ctypes.c_ulonglong), ("nlaunch", ctypes.c_ulonglong),
         ("seconds", ctypes.c_double),
         ("seeddata", ctypes.POINTER(ctypes.c_uint)),
+        ("thread_steps", ctypes.POINTER(ctypes.c_ulonglong)),
     ]
 
 
@@ -151,7 +152,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         isspecular=0, unitinmm=1.0, outputtype=0, savedetflag=0x5, issavedet=None,
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
-        saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0, mediaformat=0, words2=None):
+        saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0, mediaformat=0, words2=None,
+        thread_steps=False):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -296,6 +298,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     O.detdata = _ptr(detdata, ctypes.c_float) if detdata is not None else None
     seedbuf = np.zeros(max(maxdetphoton, 1) * 4, dtype=np.uint32) if (saveseed and issavedet) else None
     O.seeddata = _ptr(seedbuf, ctypes.c_uint) if seedbuf is not None else None
+    tsteps = np.zeros(nthread, dtype=np.uint64) if thread_steps else None
+    O.thread_steps = _ptr(tsteps, ctypes.c_ulonglong) if tsteps is not None else None
     rc = lib().mcx_oracle_run(ctypes.byref(P), ctypes.byref(O))
     if rc != 0:
         raise RuntimeError("oracle rejected input (%d)" % rc)
@@ -318,6 +322,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     if f32 is not None:
         raw = f32[: nvox * maxgate * blocks] + f32[nvox * maxgate * blocks:]
         out["field32"] = raw.reshape((nx, ny, nz, maxgate * blocks), order="F")
+    if tsteps is not None:
+        out["thread_steps"] = tsteps
     if seedbuf is not None:
         n = min(out["detected"], maxdetphoton)
         out["seeds"] = seedbuf[: 4 * n].view(np.uint8).reshape(n, 16).copy()
