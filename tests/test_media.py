@@ -392,3 +392,45 @@ def test_svmc_interface_matches_finer_label_grid(n2):
     fb = (b["field"][:, :, 22:].sum()) / b["field"].sum()
     # (a plane at z = 10.0 or 10.98 instead moves this share by 0.045)
     assert abs(fa - fb) < 0.015, (fa, fb)
+
+
+@pytest.mark.parametrize("fmt,raw_of", [
+    ("mua_float", lambda g: g.uniform(0.001, 0.05, (1, 6, 5, 4)).astype(np.float32)),
+    ("muamus_float", lambda g: g.uniform(0.001, 3.0, (2, 6, 5, 4)).astype(np.float32)),
+    ("asgn_float", lambda g: g.uniform(0.001, 1.4, (4, 6, 5, 4)).astype(np.float32)),
+    ("asgn_byte", lambda g: g.integers(0, 128, (4, 6, 5, 4)).astype(np.uint8)),
+    ("muamus_short", lambda g: g.integers(0, 65536, (2, 6, 5, 4)).astype(np.uint16)),
+    ("svmc", lambda g: g.integers(0, 3, (8, 6, 5, 4)).astype(np.uint8)),
+])
+def test_volume_file_media_formats(tmp_path, fmt, raw_of):
+    """Domain.MediaFormat + VolumeFile (src/mcx_utils.c:3510-3600): the
+    library's file loader and preprocessing give the same words as the pmcx
+    array encoding of the same data."""
+    g = np.random.default_rng(3)
+    arr = raw_of(g)
+    if fmt == "svmc":
+        arr[1] = np.where(arr[1] == arr[0], 0, arr[1])
+    # the file holds the components of each voxel together, x fastest
+    np.ascontiguousarray(np.asfortranarray(arr).reshape(arr.shape[0], -1, order="F").T).tofile(tmp_path / "vol.bin")
+    doc = {"Session": {"ID": "m", "Photons": 1000},
+           "Domain": {"Dim": [6, 5, 4], "MediaFormat": fmt, "VolumeFile": str(tmp_path / "vol.bin"), "LengthUnit": 0.5,
+                      "Media": [[0, 0, 1, 1], [0.01, 1, 0.9, 1.37], [0.02, 2, 0.5, 1.37]]},
+           "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-9},
+           "Optode": {"Source": {"Pos": [2, 2, 0], "Dir": [0, 0, 1]}}}
+    c = _lib.Config()
+    L = _lib.lib()
+    L.mcx_initcfg(ctypes.byref(c))
+    c.parentid = 2
+    import json
+    L.mcx_readconfig(json.dumps(doc).encode(), ctypes.byref(c))
+    code, msg = _lib.last_error()
+    assert code == 0, msg
+    n = 120
+    two = fmt in ("asgn_float", "svmc")
+    got = np.ctypeslib.as_array(c.vol, shape=((2 if two else 1) * n,)).copy()
+    L.mcx_clearcfg(ctypes.byref(c))
+    src = arr.view(np.int8) if arr.dtype == np.uint8 else (arr.view(np.int16) if arr.dtype == np.uint16 else arr)
+    want, mb, _ = media.encode_volume(src, unitinmm=0.5)
+    if fmt == "svmc":
+        want = media.svmc_planes(want)
+    assert np.array_equal(got, want)
