@@ -594,7 +594,8 @@ __global__ void head_bwd(const uint16_t* __restrict__ f, int fcs, const float* _
 }
 
 /* Network input features of a CW fluence map (grid units) on a label
- * volume, BF16 [v][8]: scaled log fluence, then (tissue voxels only) log mua
+ * volume, BF16 [v][8]: scaled log fluence (negative fluence counts as 0),
+ * then (tissue voxels only) log mua
  * + 4, log mus, 10 (g - 0.8), 10 (n - 1.4) from the per-voxel media table
  * tab[label] = {mua, mus, g, n}, the tissue flag, two zeros.  flipz mirrors
  * the z axis (the output voxel (x, y, z) takes input (x, y, D-1-z)). */
@@ -610,7 +611,7 @@ __global__ void features(const float* __restrict__ phi, const uint8_t* __restric
         const long long i = ((long long)zs * ny + y) * nx + x;
         const int lab = vol[i];
         float v[8];
-        v[0] = (logf(phi[i] + eps) + shift) / scale;
+        v[0] = (logf(fmaxf(phi[i], 0.f) + eps) + shift) / scale;
         const bool t = lab > 0 && lab < ntab;
         const float* m = tab + 4 * (t ? lab : 0);
         v[1] = t ? logf(fmaxf(m[0], 1e-6f)) + 4.f : 0.f;

@@ -52,8 +52,10 @@ def _chk(rc, what):
 
 
 def log_feature(phi):
-    """Scaled log fluence used as network input and target."""
-    return (np.log(np.asarray(phi, dtype=np.float64) + LOG_EPS) + LOG_SHIFT) / LOG_SCALE
+    """Scaled log fluence used as network input and target.  Negative
+    voxels (Russian-roulette survivors deposit w0 - w with their boosted
+    weight, as in the reference) count as zero."""
+    return (np.log(np.maximum(np.asarray(phi, dtype=np.float64), 0.0) + LOG_EPS) + LOG_SHIFT) / LOG_SCALE
 
 
 def inv_log_feature(f):
