@@ -321,15 +321,15 @@ def main(argv=None):
         return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--ntrain", type=int, default=256)
+    ap.add_argument("--ntrain", type=int, default=384)
     ap.add_argument("--neval", type=int, default=16)
     ap.add_argument("--edge", type=int, default=64)
     ap.add_argument("--low", type=float, default=1e5)
-    ap.add_argument("--high", type=float, default=1e7)
+    ap.add_argument("--high", type=float, default=2e7)
     ap.add_argument("--ref", type=float, default=1e7, help="photons per reference run (x --refruns)")
     ap.add_argument("--refruns", type=int, default=10)
-    ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--head-frac", type=float, default=0.5, help="fraction of training volumes that are head crops")
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--head-frac", type=float, default=0.6, help="fraction of training volumes that are head crops")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--head", type=int, default=256, help="edge of the head volume (0: skip)")
@@ -337,11 +337,11 @@ def main(argv=None):
     ap.add_argument("--head-ref", type=float, default=1e8, help="photons per head reference run (x --refruns)")
     ap.add_argument("--weights", default=None, help="save the trained parameters (npz)")
     ap.add_argument("--load-weights", default=None, help="skip training: load parameters saved by --weights")
-    ap.add_argument("--head-minenergy", default="0", help="comma list of roulette thresholds for the head runs "
+    ap.add_argument("--head-minenergy", default="1e-4", help="comma list of roulette thresholds for the head runs "
                                                           "(applied to the plain and the denoised arm alike)")
-    ap.add_argument("--minenergy", type=float, default=0.0, help="roulette threshold of the training and held-out runs")
-    ap.add_argument("--low-span", default="3,10", help="training inputs: photons log-uniform in [low/a, low*b]")
-    ap.add_argument("--photon-feature", type=int, default=0, help="give the network the input photon count")
+    ap.add_argument("--minenergy", type=float, default=1e-4, help="roulette threshold of the training and held-out runs")
+    ap.add_argument("--low-span", default="3,40", help="training inputs: photons log-uniform in [low/a, low*b]")
+    ap.add_argument("--photon-feature", type=int, default=1, help="give the network the input photon count")
     args = ap.parse_args(argv)
     import torch
 
