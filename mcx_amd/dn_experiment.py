@@ -272,7 +272,8 @@ def head_eval(net, args, minenergy, log):
     plain_rows.append({"photons": args.head_ref, "kernel_ms": href_ms / args.refruns, "error": e_ref1})
     dn = HeadDenoiser(net, hc)
     den_rows = []
-    for k, nph in enumerate((args.head_low, 2 * args.head_low, 4 * args.head_low)):
+    dn_counts = [float(v) * args.head_low for v in args.head_dn_scale.split(",")]
+    for k, nph in enumerate(dn_counts):
         low_map, low_ms_h = plain(nph, 11 + k)
         dphi = dn.upload(low_map)
         pf = nph if args.photon_feature else None
@@ -342,6 +343,7 @@ def main(argv=None):
     ap.add_argument("--minenergy", type=float, default=1e-4, help="roulette threshold of the training and held-out runs")
     ap.add_argument("--low-span", default="3,40", help="training inputs: photons log-uniform in [low/a, low*b]")
     ap.add_argument("--photon-feature", type=int, default=1, help="give the network the input photon count")
+    ap.add_argument("--head-dn-scale", default="1,2,4", help="denoised head runs: photons as multiples of --head-low")
     args = ap.parse_args(argv)
     import torch
 
@@ -386,7 +388,20 @@ def main(argv=None):
 
 
 def _evaluate(net, args, R, log, t_all):
-    # ---- held-out volumes ----------------------------------------------------
+    if args.neval > 0:
+        _heldout(net, args, R, log)
+    # ---- head volume -------------------------------------------------------
+    if args.head:
+        R["head"] = []
+        for me in [float(v) for v in args.head_minenergy.split(",")]:
+            R["head"].append(head_eval(net, args, me, log))
+    R["seconds"] = time.perf_counter() - t_all
+    with open(args.out, "w") as f:
+        json.dump(R, f, indent=1)
+    log("wrote %s" % args.out)
+
+
+def _heldout(net, args, R, log):
     ev = [training_volume(4048, i, args.edge) for i in range(args.neval)]
     me = args.minenergy
     refs = [simulate(ev, args.ref, 100 + k, minenergy=me)[0] for k in range(args.refruns)]
@@ -407,15 +422,6 @@ def _evaluate(net, args, R, log, t_all):
                                                                                    "plain_x100")},
                     "low_photons": args.low, "ref_photons": args.ref * args.refruns}
     log("held-out: %s" % json.dumps(R["heldout"]["median"]))
-    # ---- head volume -------------------------------------------------------
-    if args.head:
-        R["head"] = []
-        for me in [float(v) for v in args.head_minenergy.split(",")]:
-            R["head"].append(head_eval(net, args, me, log))
-    R["seconds"] = time.perf_counter() - t_all
-    with open(args.out, "w") as f:
-        json.dump(R, f, indent=1)
-    log("wrote %s" % args.out)
 
 
 if __name__ == "__main__":
