@@ -386,10 +386,20 @@ void BatchRun::read_counters(hipStream_t s, double* out) {
     for (int i = 0; i < 4; i++) out[i] = (double)c[i];
 }
 
-/* Autotuner: measured photons/ms of the transport kernel over a reduced
- * copy of the batch (every instance, photon counts scaled down) for each
- * candidate shape; the fastest wins.  Results are cached per batch
- * signature for the process lifetime. */
+/* Autotuner.  Photons per lane: a wave runs as long as its slowest lane,
+ * so more photons per lane balance the lanes of a chunk better, until the
+ * chunks become too few to keep every resident wave busy to the end.  The
+ * rule takes the largest candidate that leaves at least kChunksPerWave
+ * chunks per resident wave at the batch's full photon counts (a reduced
+ * copy of the batch cannot show this trade-off: it has fewer chunks, and
+ * trials on one favour few photons per lane).  Measured on the
+ * 256-instance workload: at 5.5e8 photons 8/16/32/64/128/256 photons per
+ * lane give 13.2/16.4/19.7/22.8/25.4/25.8 k photons/ms (256: 11 chunks per
+ * wave), at 5.5e7 photons 8/16/32/64 give 13.0/15.9/17.7/18.6 k (64: 4.4
+ * chunks per wave).  The workgroup size is then measured on a reduced copy
+ * (every instance, photon counts scaled down); the fastest wins.  Results
+ * are cached per batch signature for the process lifetime. */
+static const double kChunksPerWave = 4.0;
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, long long>, BatchShape> g_tune_cache;
 
@@ -402,26 +412,33 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) return it->second;
     }
-    static const int ppl_c[] = {16, 32, 64, 128};
+    static const int ppl_c[] = {16, 32, 64, 128, 256};
     static const int blk_c[] = {64, 256};
+    int vgprs = 0;
+    HIP_CHECK(mcxk_batch_attrs(tally, &vgprs));
+    hipDeviceProp_t dp;
+    HIP_CHECK(hipGetDeviceProperties(&dp, device));
+    const double waves = (double)dp.multiProcessorCount * 4 * waves_per_simd(vgprs);
     BatchShape best;
-    best.photons_per_lane = 64;
+    best.photons_per_lane = ppl_c[0];
     best.block = 256;
+    for (int ppl : ppl_c) {
+        double chunks = 0.0;
+        for (int i = 0; i < n; i++) chunks += ceil(ceil((double)cfgs[i]->nphoton / ppl) / 64.0);
+        if (chunks >= kChunksPerWave * waves) best.photons_per_lane = ppl;
+    }
     double best_rate = -1.0;
     for (int b : blk_c) {
-        for (int ppl : ppl_c) {
-            BatchShape sh;
-            sh.photons_per_lane = ppl;
-            sh.block = b;
-            BatchRun r(cfgs, n, tally, sh, fraction, device);
-            r.setup();
-            float kms = 0.f;
-            r.run(s, false, 1, &kms);
-            const double rate = r.photons / std::max(1e-3, (double)kms);
-            if (rate > best_rate) {
-                best_rate = rate;
-                best = sh;
-            }
+        BatchShape sh = best;
+        sh.block = b;
+        BatchRun r(cfgs, n, tally, sh, fraction, device);
+        r.setup();
+        float kms = 0.f;
+        r.run(s, false, 1, &kms);
+        const double rate = r.photons / std::max(1e-3, (double)kms);
+        if (rate > best_rate) {
+            best_rate = rate;
+            best.block = b;
         }
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
