@@ -153,7 +153,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
         saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0, mediaformat=0, words2=None,
-        thread_steps=False):
+        thread_steps=False, genergy=None):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -288,7 +288,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     nvox = nx * ny * nz
     field = np.zeros(nvox * maxgate * blocks, dtype=np.float64) if want_field else None
     f32 = np.zeros(2 * nvox * maxgate * blocks, dtype=np.float32) if field32 else None
-    genergy = np.zeros(2 * nthread, dtype=np.float32)
+    # per-thread escaped/launched energies continue from `genergy` (respins)
+    genergy = np.zeros(2 * nthread, dtype=np.float32) if genergy is None else np.array(genergy, dtype=np.float32)
     reclen = savedet_reclen(savedetflag, prop.shape[0])
     detdata = np.zeros(max(maxdetphoton, 1) * max(reclen, 1), dtype=np.float32) if issavedet else None
     O = OracleOut()

@@ -211,3 +211,54 @@ def test_diffuse_reflectance_bitexact():
     assert np.all(flux[void] == 0) and np.all(dref[~void] == 0)
     assert np.abs(dref[void] + ref[..., 0][void]).max() <= 1e-5 * np.abs(ref).max()
     assert np.abs(flux[~void] - ref[..., 0][~void]).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_respin_accumulates_like_the_oracle():
+    """respin 2 (src/mcx_core.cu:3344-3387): a second launch with the next
+    seed words of the host stream; per-thread energies continue in FP32 on
+    the device, the fluence halves are summed on the host."""
+    import oracle_py
+    nt = 1024
+    cfg = workloads.cube60(20000)
+    res = _run(dict(_layout(cfg, nthread=nt), respin=2, isnormalized=0), tally=2)
+    words = oracle_py.seedwords(cfg["seed"], 2 * 4 * nt)
+    a = oracle_run(cfg, nt, seeds=words[:4 * nt])
+    b = oracle_run(cfg, nt, seeds=words[4 * nt:], genergy=a["genergy"])
+    tot, esc = seq_sum(b["genergy"][1::2]), seq_sum(b["genergy"][0::2])
+    st = res["stat"]
+    assert st["energytot"] == tot == 40000.0
+    assert st["energyabs"] == tot - esc
+    ref = a["field"] + b["field"]
+    got = res["flux"].astype(np.float64)
+    sig = ref > 1e-3 * ref.max()
+    assert np.abs(got[sig] - ref[sig]).max() <= 2e-5 * ref.max()
+
+
+def test_gate_groups_like_the_oracle():
+    """maxgate 2 of 5 gates (src/mcx_core.cu:3344-3352): three launches over
+    successive 2-gate windows, each with fresh seed words; every window's
+    tally lands on the first two gates of the output and normalisation is
+    switched off, as in the reference."""
+    import oracle_py
+    nt = 1024
+    cfg = workloads.cube60(20000, tend=5e-9, tstep=1e-9)
+    res = _run(dict(_layout(cfg, nthread=nt), maxgate=2), tally=2)
+    words = oracle_py.seedwords(cfg["seed"], 3 * 4 * nt)
+    tstep = np.float32(1e-9)
+    ref = 0.0
+    gen = []
+    for g in range(3):
+        t0 = float(np.float32(np.float32(0.0) + tstep * np.float32(2 * g)))
+        r = oracle_run(dict(cfg, tstart=t0, tend=t0 + 2e-9, tstep=1e-9), nt, seeds=words[4 * nt * g:4 * nt * (g + 1)],
+                       maxgate=2)
+        ref = ref + r["field"]
+        gen.append(r["genergy"])
+    tot = seq_sum(np.concatenate([x[1::2] for x in gen]))
+    esc = seq_sum(np.concatenate([x[0::2] for x in gen]))
+    st = res["stat"]
+    assert st["energytot"] == tot
+    assert st["energyabs"] == tot - esc
+    got = res["flux"].astype(np.float64)
+    assert got.shape[3] == 5 and np.all(got[..., 2:] == 0)
+    sig = ref > 1e-3 * ref.max()
+    assert np.abs(got[..., :2][sig] - ref[sig]).max() <= 2e-5 * ref.max()
