@@ -25,10 +25,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per photon of the transport kernel on the sphere100 workload with
-# the FP64 tally: (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000) per launch / 1e7
-# photons, scale factors from the FP64->FP32 merge kernel as a copy of known
-# size (profiles/transport_sphere100_1e7_counters.md)
-TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 9201.0}
+# the FP64 tally and the LDS hot-cell box: (FETCH_SIZE x 2.000 + WRITE_SIZE x
+# 1.000) per launch / 1e7 photons, scale factors from the FP64->FP32 merge
+# kernel as a copy of known size (profiles/transport_sphere100_1e7_counters.md)
+TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 8403.0}
 ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 
@@ -211,12 +211,17 @@ def main():
 
     steps_pp = stats_w["nstep"] / nph
     dep_pp = stats_w["ndeposit"] / nph
+    hot_pp = stats_w["ndeposit_hot"] / nph
+    # global tally atomics per photon: the deposits outside the LDS hot-cell
+    # box, plus the box's flush (at most one atomic per cell per workgroup)
+    flush_pp = stats_w["hot_cells"] * (stats_w["nthread"] / stats_w["nblock"]) / nph
+    glob_pp = dep_pp - hot_pp + flush_pp
     reclen = 3 if args.workload == "sphere100" else 0
-    # algorithmic bytes: 1 B label load per step; per deposit the tally
-    # word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8, FP64: 8+8);
-    # detected-photon records
+    # algorithmic bytes: 1 B label load per step; per global tally atomic
+    # the tally word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8,
+    # FP64: 8+8); detected-photon records
     dep_bytes = {0: 8.0, 1: 16.0, 2: 16.0, 3: 16.0}[args.tally]
-    bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
+    bytes_pp = steps_pp * 1.0 + glob_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
     tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally))
@@ -226,7 +231,7 @@ def main():
     ngates = int(round((cfg["tend"] - cfg["tstart"]) / cfg["tstep"]))
     amode = {0: 1, 1: 2, 2: 0, 3: 0}[args.tally]
     aroof = _atomic_roof(amode, nvox * ngates)
-    arate = dep_pp * nph / (k_ms * 1e-3)
+    arate = glob_pp * nph / (k_ms * 1e-3)
     line = {
         "metric": "photons/ms at 1 GPU",
         "value": round(value, 1),
@@ -252,6 +257,9 @@ def main():
             "vgprs": int(stats_w["vgprs"]),
             "steps_per_photon": round(steps_pp, 2),
             "deposits_per_photon": round(dep_pp, 2),
+            "lds_box_deposits_per_photon": round(hot_pp, 2),
+            "lds_box_cells": int(stats_w["hot_cells"]),
+            "global_atomics_per_photon": round(glob_pp, 2),
             "absorbed_fraction": round(absorbed, 6),
         },
         "roofline": {

@@ -32,6 +32,16 @@ int  mcx_amd_set_tally_mode(int mode);
  * subsequent runs; costs one atomic per wave at lane exit. */
 void mcx_amd_set_counters(int enable);
 
+/* Hot-cell box (default on): in subsequent mcx_run_simulation calls with
+ * tally mode 1, 2 or 3, the tally cells of the first gate of each time
+ * window in a box of voxels at the source are summed in FP64 in each
+ * workgroup's LDS and added to the tally when the workgroup ends.  These
+ * cells take the most deposits per cell (about one per photon at a pencil
+ * source's voxel), and global atomics on one address serialise, so
+ * without the box they bound the kernel.  The box is as large as the LDS
+ * left at full occupancy allows.  Returns the previous setting. */
+int  mcx_amd_set_hot_box(int enable);
+
 /* Statistics of the last mcx_run_simulation call on this process:
  *  out[0] kernel time in ms (HIP events, sum over launches)
  *  out[1] launch+sync wall time in ms (the reference's runtime definition)
@@ -39,6 +49,9 @@ void mcx_amd_set_counters(int enable);
  *  out[6] vector registers per lane  out[7] lanes launched
  *  out[8] workgroup size  out[9] kernel launches
  *  out[10] lanes that hit the "should never happen" exit
+ *  out[11] tally cells in the hot-cell box (0: off)
+ *  out[12] dynamic LDS bytes per workgroup
+ *  out[13] deposits summed in the hot-cell box (of out[3]; with counters)
  * Returns the number of values written (<= n). */
 int  mcx_amd_last_stats(double* out, int n);
 

@@ -24,5 +24,11 @@ def set_counters(enable):
     _lib.lib().mcx_amd_set_counters(1 if enable else 0)
 
 
+def set_hot_box(enable):
+    """Sum the tally cells next to the source in each workgroup's LDS
+    (default on; FP64 and hi/lo tallies).  Returns the previous setting."""
+    return bool(_lib.lib().mcx_amd_set_hot_box(1 if enable else 0))
+
+
 def last_stats():
     return _lib.last_stats()

@@ -113,7 +113,7 @@ ABI_SYMBOLS = [
     "mcx_run_simulation", "mcx_list_gpu", "mcx_initcfg", "mcx_clearcfg", "mcx_preprocess", "mcx_maskdet",
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
-    "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_last_stats",
+    "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
     "mcx_dn_conv3_fwd", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
     "mcx_dn_maxpool_fwd", "mcx_dn_maxpool_bwd", "mcx_dn_upsample_fwd", "mcx_dn_upsample_bwd", "mcx_dn_add",
@@ -151,6 +151,8 @@ def _setup(L):
     L.mcx_amd_set_tally_mode.argtypes = [ctypes.c_int]
     L.mcx_amd_set_tally_mode.restype = ctypes.c_int
     L.mcx_amd_set_counters.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_hot_box.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_hot_box.restype = ctypes.c_int
     L.mcx_amd_last_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
     L.mcx_amd_last_stats.restype = ctypes.c_int
     L.mcx_amd_abi_layout.argtypes = [P(ctypes.c_size_t), ctypes.c_int]
@@ -255,8 +257,8 @@ def atomic_bench(mode, nelem, nper, nblocks):
 
 
 def last_stats():
-    out = (ctypes.c_double * 11)()
-    n = lib().mcx_amd_last_stats(out, 11)
+    out = (ctypes.c_double * 14)()
+    n = lib().mcx_amd_last_stats(out, 14)
     keys = ["kernel_ms", "wall_ms", "nstep", "ndeposit", "nscatter", "nlaunch", "vgprs", "nthread", "nblock",
-            "nlaunches", "errlanes"]
+            "nlaunches", "errlanes", "hot_cells", "lds_bytes", "ndeposit_hot"]
     return dict(zip(keys, list(out[:n])))

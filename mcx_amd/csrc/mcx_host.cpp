@@ -22,6 +22,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <mutex>
 #include <vector>
@@ -39,9 +40,10 @@ extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, co
 namespace mcx {
 
 static std::mutex g_statmu;
-static double g_stats[11];
+static double g_stats[14];
 static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
+static int g_hotbox = 1;
 
 static bool supported_source(int t) { return t >= MCX_SRC_PENCIL && t <= MCX_SRC_RING; }
 
@@ -98,6 +100,69 @@ int waves_per_simd(int vgprs) {
     return std::max(1, std::min(8, 512 / g));
 }
 
+
+/* Largest box of cells that can be summed in LDS without lowering the
+ * occupancy: c cells over three axes of extents d, each box edge at most
+ * its axis' extent, the shortest axis first so the other two can take
+ * what it cannot. */
+static void box_dims(const uint32_t d[3], uint32_t c, uint32_t b[3]) {
+    int ord[3] = {0, 1, 2};
+    std::sort(ord, ord + 3, [&](int a, int q) { return d[a] < d[q]; });
+    uint32_t left = c;
+    for (int k = 0; k < 3; k++) {
+        const int ax = ord[k];
+        const double root = k == 0 ? std::cbrt((double)left) : (k == 1 ? std::sqrt((double)left) : (double)left);
+        b[ax] = std::max(1u, std::min(d[ax], (uint32_t)std::floor(root + 1e-9)));
+        left = std::max(1u, left / b[ax]);
+    }
+}
+
+/* The hot-cell box of a single-simulation launch (KParam::hot_*): gate 0
+ * of the time window, voxels around the source's voxel, reaching into the
+ * medium along the launch direction, as large as the LDS a workgroup can
+ * use at the kernel's full occupancy allows (at most 64 KB in all).
+ * Returns the workgroup's total dynamic LDS bytes. */
+static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int numregs, size_t shmem, KParam& kp) {
+    kp.hot_cells = 0;
+    if (!g_hotbox || tally == MCX_TALLY_F32_SHADOW || kp.maxgate == 0 || kp.nvox == 0) return shmem;
+    int dev = 0, lds_cu = 0, lds_blk = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds_cu <= 0 ||
+        lds_blk <= 0)
+        return shmem;
+    const unsigned int waves_cu = 4u * (unsigned int)waves_per_simd(numregs);
+    const unsigned int wgs = std::max(1u, waves_cu / std::max(1u, block / 64u));
+    size_t per = std::min<size_t>({(size_t)65536, (size_t)lds_blk, (size_t)lds_cu / wgs});
+    per &= ~(size_t)511; /* LDS is allocated in 512-byte granules */
+    const size_t off = (shmem + 7) & ~(size_t)7;
+    if (per <= off + 64 * sizeof(double)) return shmem;
+    const uint32_t d[3] = {kp.nx, kp.ny, kp.nz};
+    uint32_t b[3];
+    box_dims(d, (uint32_t)((per - off) / sizeof(double)), b);
+    const uint32_t cells = b[0] * b[1] * b[2];
+    if (cells < 64) return shmem;
+    const float* src = &cfg->srcpos.x;
+    const float* dir = &cfg->srcdir.x;
+    uint32_t o[3];
+    for (int k = 0; k < 3; k++) {
+        const double c = std::floor((double)src[k]) + (double)dir[k] * (0.5 * b[k] - 1.0);
+        const double lo = std::floor(c - 0.5 * b[k] + 0.5);
+        o[k] = (uint32_t)std::max(0.0, std::min((double)(d[k] - b[k]), lo));
+    }
+    kp.hot_cells = cells;
+    kp.hot_off = (uint32_t)off;
+    kp.hot_g = 1;
+    kp.hot_x0 = o[0];
+    kp.hot_y0 = o[1];
+    kp.hot_z0 = o[2];
+    kp.hot_bx = b[0];
+    kp.hot_by = b[1];
+    kp.hot_bz = b[2];
+    kp.hot_lo = (o[2] * kp.ny + o[1]) * kp.nx + o[0];
+    kp.hot_span = ((o[2] + b[2] - 1) * kp.ny + (o[1] + b[1] - 1)) * kp.nx + (o[0] + b[0] - 1) - kp.hot_lo;
+    return off + (size_t)cells * sizeof(double);
+}
 
 /* launch constants of one simulation (buffers, work split and time
  * window are set by the caller) */
@@ -310,7 +375,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     dseeds.alloc((size_t)nthread * 4 * sizeof(uint32_t));
     if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
     dcount.alloc(4 * sizeof(uint32_t));
-    dcounters.alloc(4 * sizeof(uint64_t));
+    dcounters.alloc(5 * sizeof(uint64_t));
     const bool saveseed = savedet && cfg->issaveseed;
     if (saveseed) dsaved.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton) * 16);
     if (replay) {
@@ -357,8 +422,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const unsigned int block = (unsigned int)G.autoblock;
     const unsigned int grid = nthread / block;
     /* per lane in LDS: the partial paths, then (seed saving) the launch RNG state */
-    const size_t shmem = (size_t)ntable * 16 + (size_t)block * (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
-    if (shmem > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
+    const size_t shmem0 = (size_t)ntable * 16 + (size_t)block * (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
+    if (shmem0 > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
+    const size_t shmem = set_hot_box(cfg, tally, block, numregs, shmem0, kp);
 
     if (cfg->flog && cfg->printnum >= 0) {
         fprintf(cfg->flog, "GPU=%d (%s) threadph=%d extra=%d np=%.0f nthread=%u maxgate=%u repetition=%d\n", gpuid + 1, G.name,
@@ -469,7 +535,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
     cfg->runtime = (unsigned int)llround(wall_ms);
 
-    uint64_t ctr[4] = {0, 0, 0, 0};
+    uint64_t ctr[5] = {0, 0, 0, 0, 0};
     if (g_counters) {
         HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
     }
@@ -486,6 +552,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         g_stats[8] = block;
         g_stats[9] = nlaunch;
         g_stats[10] = errlanes;
+        g_stats[11] = kp.hot_cells;
+        g_stats[12] = (double)shmem;
+        g_stats[13] = (double)ctr[4];
     }
 
     /* normalisation (src/mcx_core.cu:3823-3925) */
@@ -680,9 +749,15 @@ int mcx_amd_set_tally_mode(int mode) {
 
 void mcx_amd_set_counters(int enable) { g_counters = enable ? 1 : 0; }
 
+int mcx_amd_set_hot_box(int enable) {
+    int old = g_hotbox;
+    g_hotbox = enable ? 1 : 0;
+    return old;
+}
+
 int mcx_amd_last_stats(double* out, int n) {
     std::lock_guard<std::mutex> lk(g_statmu);
-    int k = std::min(n, 11);
+    int k = std::min(n, 14);
     for (int i = 0; i < k; i++) out[i] = g_stats[i];
     return k;
 }

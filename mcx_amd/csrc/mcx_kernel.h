@@ -67,6 +67,15 @@ struct KParam {
     const float* replaytof;       /* time of flight of each replayed photon */
     const int32_t* replaydetid;   /* detector id of each replayed photon */
     uint32_t mediaformat;         /* 1, 2, 4: labels; 96-104: per-voxel property words */
+    /* Hot-cell box: the tally cells of gates [0, hot_g) over voxels
+     * [hot_x0, hot_x0 + hot_bx) x [hot_y0, ..) x [hot_z0, ..) are summed in
+     * FP64 in each workgroup's LDS (at byte offset hot_off of the dynamic
+     * LDS) and added to the tally once per workgroup at its end.  These
+     * cells, next to the source, take the most deposits per cell, and
+     * global atomics on one address serialise.  hot_cells == 0: off. */
+    uint32_t hot_cells, hot_off, hot_g;
+    uint32_t hot_x0, hot_y0, hot_z0, hot_bx, hot_by, hot_bz;
+    uint32_t hot_lo, hot_span;    /* linear voxel index of the box's first voxel, last minus first */
 };
 
 /* One instance of a batched launch (its launch constants are in KParam) */

@@ -147,7 +147,7 @@ struct Lane {
     float pend_old, pend_w;
     double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t gid;         /* global index of the current photon (replay) */
-    uint32_t nstep, ndep, nscat, nlaunch;
+    uint32_t nstep, ndep, nscat, nlaunch, nhot;
     /* split-voxel state (src/mcx_core.h:65-83): interface point and
      * normal of the current voxel, its two labels, which side the photon
      * is on, whether the voxel is split; whether the current path segment
@@ -898,8 +898,20 @@ struct Lane {
     }
 
     __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight) {
-        const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
         ndep++;
+        if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
+            const uint32_t z = vox / kp.dimxy, r = vox - z * kp.dimxy, y = r / kp.nx;
+            const uint32_t lx = r - y * kp.nx - kp.hot_x0, ly = y - kp.hot_y0, lz = z - kp.hot_z0;
+            if (lx < kp.hot_bx && ly < kp.hot_by && lz < kp.hot_bz) {
+                extern __shared__ F4 lds[];
+                double* box = (double*)((char*)lds + kp.hot_off);
+                const uint32_t c = (((uint32_t)tshift * kp.hot_bz + lz) * kp.hot_by + ly) * kp.hot_bx + lx;
+                __hip_atomic_fetch_add(box + c, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                nhot++;
+                return;
+            }
+        }
+        const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
         if (TALLY == MCX_TALLY_F64) {
             atom_add_nr(tally64 + i, (double)weight);  /* no return: fire and forget */
         } else if (TALLY == MCX_TALLY_F64_XCD) {
@@ -921,6 +933,7 @@ struct Lane {
             atomicAdd((unsigned long long*)kp.counters + 1, (unsigned long long)ndep);
             atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)nscat);
             atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)nlaunch);
+            if (kp.hot_cells) atomicAdd((unsigned long long*)kp.counters + 4, (unsigned long long)nhot);
         }
     }
 
@@ -941,7 +954,7 @@ struct Lane {
         pend_i = ~(size_t)0;
         tally64 = kp.field64;
         if (TALLY == MCX_TALLY_F64_XCD) tally64 += (size_t)xcc_id() * kp.fieldlen;
-        nstep = ndep = nscat = nlaunch = 0;
+        nstep = ndep = nscat = nlaunch = nhot = 0;
         svsplit = svlo = svup = svupper = 0;
         rpx = rpy = rpz = nvx = nvy = nvz = 0.f;
         testint = hitintf = false;
@@ -1257,16 +1270,50 @@ __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(con
         const MCX_GLB float* g = glb(kp.gprop) + 4 * i;
         lds[i] = F4{g[0], g[1], g[2], g[3]};
     }
+    const bool hot = TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells;
+    double* box = (double*)((char*)lds + kp.hot_off);
+    if (hot)
+        for (uint32_t c = threadIdx.x; c < kp.hot_cells; c += blockDim.x) box[c] = 0.0;
     __syncthreads();
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
-    if ((long long)idx >= total || idx >= kp.nthread) return;
-    float* ppbase = (float*)(lds + kp.ntable) + threadIdx.x;
-    Lane<ISPENCIL, ISREFLECT, TALLY, MT> L(kp, lds, ppbase, blockDim.x, idx);
-    L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
-    for (uint32_t i = 0; i < kp.partialdata; i++) L.ppath(i) = 0.f;
-    L.run();
-    L.add_counters();
+    if ((long long)idx < total && idx < kp.nthread) {
+        float* ppbase = (float*)(lds + kp.ntable) + threadIdx.x;
+        Lane<ISPENCIL, ISREFLECT, TALLY, MT> L(kp, lds, ppbase, blockDim.x, idx);
+        L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
+        for (uint32_t i = 0; i < kp.partialdata; i++) L.ppath(i) = 0.f;
+        L.run();
+        L.add_counters();
+    }
+    if (!hot) return;
+    /* every lane of the workgroup has finished: add the box to the tally */
+    __syncthreads();
+    double* t64 = kp.field64;
+    if (TALLY == MCX_TALLY_F64_XCD) t64 += (size_t)xcc_id() * kp.fieldlen;
+    const uint32_t plane = kp.hot_bx * kp.hot_by, vol = plane * kp.hot_bz;
+    for (uint32_t c = threadIdx.x; c < kp.hot_cells; c += blockDim.x) {
+        const double sum = box[c];
+        if (sum == 0.0) continue;
+        const uint32_t t = c / vol, r = c - t * vol, lz = r / plane, r2 = r - lz * plane, ly = r2 / kp.hot_bx;
+        const uint32_t lx = r2 - ly * kp.hot_bx;
+        const size_t i = (size_t)((kp.hot_z0 + lz) * kp.dimxy + (kp.hot_y0 + ly) * kp.nx + kp.hot_x0 + lx) + (size_t)t * kp.nvox;
+        if (TALLY == MCX_TALLY_F64) {
+            atom_add_nr(t64 + i, sum);
+        } else if (TALLY == MCX_TALLY_F64_XCD) {
+            __hip_atomic_fetch_add(glb(t64) + i, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            /* hi/lo: the FP32 rounding of the sum and the TwoSum error of
+             * the hi add both go to the lo half */
+            const float w = (float)sum;
+            const float lo = (float)(sum - (double)w);
+            const float a = atom_add_ret(kp.field + i, w);
+            const float s = a + w;
+            const float bb = s - a;
+            const float e = (a - (s - bb)) + (w - bb);
+            if (e != 0.f) atom_add_nr(kp.field2 + i, e);
+            if (lo != 0.f) atom_add_nr(kp.field2 + i, lo);
+        }
+    }
 }
 
 /* Batched persistent launch: every wave repeatedly takes the next chunk of
