@@ -266,8 +266,9 @@ def head_eval(net, args, minenergy, log):
     ref_loo, mask_loo = reliable_mask(hrefs[1:])
     e_ref1 = median_error(hrefs[0], ref_loo, mask_loo & mask)
     plain_rows = []
-    for nph, seed in ((args.head_low, 7), (args.head_low * 10, 8)):
-        m, ms = plain(nph, seed)
+    for k, v in enumerate(args.head_plain_scale.split(",")):
+        nph = float(v) * args.head_low
+        m, ms = plain(nph, 7 + k if k < 4 else 50 + k)
         plain_rows.append({"photons": nph, "kernel_ms": ms, "error": median_error(m, ref, mask)})
     plain_rows.append({"photons": args.head_ref, "kernel_ms": href_ms / args.refruns, "error": e_ref1})
     dn = HeadDenoiser(net, hc)
@@ -344,6 +345,8 @@ def main(argv=None):
     ap.add_argument("--low-span", default="3,40", help="training inputs: photons log-uniform in [low/a, low*b]")
     ap.add_argument("--photon-feature", type=int, default=1, help="give the network the input photon count")
     ap.add_argument("--head-dn-scale", default="1,2,4", help="denoised head runs: photons as multiples of --head-low")
+    ap.add_argument("--head-plain-scale", default="1,2,4,10",
+                    help="plain head runs (besides the references): photons as multiples of --head-low")
     args = ap.parse_args(argv)
     import torch
 
