@@ -83,8 +83,49 @@ __device__ __forceinline__ unsigned int xcd_logical(unsigned int h, unsigned int
 /* ---- 3x3x3 convolution, forward (and input gradient) --------------------
  * y[m][co] = act(bias[co] + sum_{t,ci} x[m + off_t][ci] * w[co][t*cin + ci])
  * where x is multiplied by (mask > 0) channel-wise when mask is given.
- * w rows are kpad long (27*cin rounded up to 32, zero padded). */
-template <int NT>
+ * w rows are kpad long (27*cin rounded up to 32, zero padded).
+ *
+ * A wave computes MT tiles of 16 consecutive voxels against all NT tiles of
+ * 16 output channels, so every weight fragment it loads feeds MT MFMAs and
+ * every input fragment NT.  The fragments of the next 32-wide K chunk are
+ * loaded before the MFMAs of the current one (register double buffering),
+ * so their latency overlaps the matrix work.  A lane tracks the tap and
+ * channel of its 8-channel slice of the chunk incrementally (cin is a
+ * multiple of 8, so a slice never straddles two taps). */
+template <int MT, int NT>
+struct ConvFrags {
+    Frag a[MT];
+    Frag b[NT];
+};
+
+template <int MT, int NT>
+__device__ __forceinline__ void conv_load(ConvFrags<MT, NT>& f, const uint16_t* __restrict__ x, int xcs,
+                                          const uint16_t* __restrict__ mask, int mcs, const uint16_t* __restrict__ w,
+                                          int kpad, int k0, int t, int ci, const long long* mv, const int* cx,
+                                          const int* cy, const int* cz, int D, int H, int W, int r) {
+#pragma unroll
+    for (int j = 0; j < NT; j++) f.b[j] = load8(w + (long long)(j * 16 + r) * kpad + k0);
+    int dz = 0, dy = 0, dx = 0;
+    tap_offsets(t, dz, dy, dx);
+    const long long off = ((long long)dz * H + dy) * W + dx;
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        f.a[i] = zero8();
+        const int zz = cz[i] + dz, yy = cy[i] + dy, xx = cx[i] + dx;
+        if (t < 27 && mv[i] >= 0 && (unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
+            const long long v = mv[i] + off;
+            f.a[i] = load8(x + v * xcs + ci);
+            if (mask) {
+                const Frag mk = load8(mask + v * mcs + ci);
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    if (!(bf2f(mk.h[q]) > 0.f)) f.a[i].h[q] = 0;
+            }
+        }
+    }
+}
+
+template <int MT, int NT>
 __global__ void __launch_bounds__(256) conv3_fwd(const uint16_t* __restrict__ x, int xcs, const uint16_t* __restrict__ mask,
                                                  int mcs, const uint16_t* __restrict__ w, int kpad,
                                                  const float* __restrict__ bias, uint16_t* __restrict__ y, int ycs, int nb,
@@ -93,58 +134,203 @@ __global__ void __launch_bounds__(256) conv3_fwd(const uint16_t* __restrict__ x,
     const int r = lane & 15, g = lane >> 4;
     const long long mtot = (long long)nb * D * H * W;
     const unsigned int blk = xcd_logical(blockIdx.x, gridDim.x);
-    const long long m = (long long)blk * 64 + wave * 16 + r;
-    int vx = 0, vy = 0, vz = 0;
-    long long vn = 0;
-    const bool mvalid = m < mtot;
-    if (mvalid) {
-        long long q = m;
-        vx = (int)(q % W); q /= W;
-        vy = (int)(q % H); q /= H;
-        vz = (int)(q % D); vn = q / D;
+    const long long mbase = ((long long)blk * 4 + wave) * (16 * MT);
+    long long mv[MT];
+    int cx[MT], cy[MT], cz[MT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        const long long m = mbase + 16 * i + r;
+        mv[i] = m < mtot ? m : -1;
+        cx[i] = cy[i] = cz[i] = 0;
+        if (m < mtot) {
+            long long q = m;
+            cx[i] = (int)(q % W); q /= W;
+            cy[i] = (int)(q % H); q /= H;
+            cz[i] = (int)(q % D);
+        }
     }
-    f32x4 acc[NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int j = 0; j < NT; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int ktot = 27 * cin;
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int nch = kpad / 32;
+    int t = (8 * g) / cin, ci = 8 * g - t * cin;
+    ConvFrags<MT, NT> cur, nxt;
+    conv_load<MT, NT>(cur, x, xcs, mask, mcs, w, kpad, 8 * g, t, ci, mv, cx, cy, cz, D, H, W, r);
     for (int kc = 0; kc < nch; kc++) {
-        const int k0 = kc * 32 + 8 * g;
-        Frag a = zero8();
-        if (mvalid && k0 < ktot) {
-            const int t = k0 / cin, ci0 = k0 - t * cin;
-            int dz, dy, dx;
-            tap_offsets(t, dz, dy, dx);
-            const int zz = vz + dz, yy = vy + dy, xx = vx + dx;
-            if ((unsigned)zz < (unsigned)D && (unsigned)yy < (unsigned)H && (unsigned)xx < (unsigned)W) {
-                const long long v = ((vn * D + zz) * H + yy) * W + xx;
-                a = load8(x + v * xcs + ci0);
-                if (mask) {
-                    const Frag mk = load8(mask + v * mcs + ci0);
+        int tn = t, cn = ci + 32;
+        while (cn >= cin) {
+            cn -= cin;
+            tn++;
+        }
+        if (kc + 1 < nch)
+            conv_load<MT, NT>(nxt, x, xcs, mask, mcs, w, kpad, (kc + 1) * 32 + 8 * g, tn, cn, mv, cx, cy, cz, D, H, W, r);
 #pragma unroll
-                    for (int i = 0; i < 8; i++)
-                        if (!(bf2f(mk.h[i]) > 0.f)) a.h[i] = 0;
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < NT; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.a[i].v, cur.b[j].v, acc[i][j], 0, 0, 0);
+        cur = nxt;
+        t = tn;
+        ci = cn;
+    }
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int co = j * 16 + r;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long long mr = mbase + 16 * i + 4 * g + q;
+                if (mr < mtot && co < cout) {
+                    float v = acc[i][j][q] + bv;
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    y[mr * ycs + co] = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
+/* ---- 3x3x3 convolution with an LDS halo tile (cout <= 32) ------------------
+ * Same result as conv3_fwd.  A workgroup owns 16 x 4 x 4 output voxels
+ * (x, y, z); wave wv the z-slice wv, its M tile i the row y = i.  The input
+ * halo of the tile (18 x 6 x 6 voxels) is staged in LDS for CG channels at
+ * a time (zero outside the grid, masked when a mask is given), so each
+ * input voxel is read from memory once per channel group instead of once
+ * per tap, and every MFMA operand is one 16-byte LDS read.  The voxel
+ * stride in LDS is an odd number of 16-byte units (CG + 8 or CG + 16), so
+ * the 16 lanes of a row, which read 16 consecutive voxels, hit distinct
+ * banks.  The weights of the channel group (16*NT output channels x 27
+ * taps x CG channels) are staged in LDS with it, rows padded the same way,
+ * so no MFMA waits on a global load.  K is ordered (channel group, tap,
+ * 8-channel slice); lane group g takes slice-tap pair 4*kc + g of chunk kc,
+ * padded with zeros past 27 taps. */
+template <int CG, int NT>
+__global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict__ x, int xcs,
+                                                      const uint16_t* __restrict__ mask, int mcs,
+                                                      const uint16_t* __restrict__ w, int kpad,
+                                                      const float* __restrict__ bias, uint16_t* __restrict__ y, int ycs,
+                                                      int nb, int D, int H, int W, int cin, int cout, int relu) {
+    constexpr int TX = 16, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    constexpr int CS = (CG / 8) % 2 ? CG + 16 : CG + 8; /* LDS channel stride per voxel: odd 16-byte units */
+    constexpr int SPT = CG / 8;          /* 8-channel slices per tap */
+    constexpr int NPAIR = 27 * SPT;
+    constexpr int NCH = (NPAIR + 3) / 4; /* K chunks per channel group */
+    constexpr int WS = 27 * CG + ((27 * CG / 8) % 2 ? 0 : 8); /* LDS weight row: odd 16-byte units */
+    __shared__ __attribute__((aligned(16))) uint16_t tile[HX * HY * HZ * CS];
+    __shared__ __attribute__((aligned(16))) uint16_t wtile[16 * NT * WS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + TZ - 1) / TZ;
+    unsigned int b = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = (int)(b % nbx); b /= nbx;
+    const int by = (int)(b % nby); b /= nby;
+    const int bz = (int)(b % nbz);
+    const int bn = (int)(b / nbz);
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < cin; c0 += CG) {
+        /* all of a lane's global loads are issued before any LDS store, so
+         * their latencies overlap */
+        constexpr int NH = HX * HY * HZ * SPT, NW = 16 * NT * NPAIR;
+        constexpr int LH = (NH + 255) / 256, LW = (NW + 255) / 256;
+        Frag hv[LH], wf[LW];
+#pragma unroll
+        for (int k = 0; k < LH; k++) {
+            const int e = threadIdx.x + 256 * k;
+            hv[k] = zero8();
+            if (e < NH) {
+                const int sl = e % SPT, vox = e / SPT;
+                const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) {
+                    const long long m = (((long long)bn * D + gz) * H + gy) * W + gx;
+                    hv[k] = load8(x + m * xcs + c0 + 8 * sl);
+                    if (mask) {
+                        const Frag mk = load8(mask + m * mcs + c0 + 8 * sl);
+#pragma unroll
+                        for (int q = 0; q < 8; q++)
+                            if (!(bf2f(mk.h[q]) > 0.f)) hv[k].h[q] = 0;
+                    }
                 }
             }
         }
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const Frag b = load8(w + (long long)(j * 16 + r) * kpad + k0);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j], 0, 0, 0);
+        for (int k = 0; k < LW; k++) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < NW) {
+                const int co = e / NPAIR, pq = e - co * NPAIR;
+                const int t = pq / SPT, sl = pq - t * SPT;
+                wf[k] = load8(w + (long long)co * kpad + t * cin + c0 + 8 * sl);
+            }
+        }
+        __syncthreads(); /* the previous channel group's reads of the tiles are done */
+#pragma unroll
+        for (int k = 0; k < LH; k++) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < NH) *(uint4*)(tile + (e / SPT) * CS + 8 * (e % SPT)) = hv[k].u;
+        }
+#pragma unroll
+        for (int k = 0; k < LW; k++) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < NW) {
+                const int co = e / NPAIR, pq = e - co * NPAIR;
+                const int t = pq / SPT, sl = pq - t * SPT;
+                *(uint4*)(wtile + co * WS + t * CG + 8 * sl) = wf[k].u;
+            }
+        }
+        __syncthreads();
+        for (int kc = 0; kc < NCH; kc++) {
+            const int pr = 4 * kc + g;
+            const int t = pr / SPT, sl = pr - t * SPT;
+            Frag bf[NT], af[4];
+            if (t < 27) {
+                int dz, dy, dx;
+                tap_offsets(t, dz, dy, dx);
+#pragma unroll
+                for (int j = 0; j < NT; j++) bf[j].u = *(const uint4*)(wtile + (j * 16 + r) * WS + t * CG + 8 * sl);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int hv = ((wv + dz + 1) * HY + (i + dy + 1)) * HX + (r + dx + 1);
+                    af[i].u = *(const uint4*)(tile + hv * CS + 8 * sl);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < NT; j++) bf[j] = zero8();
+#pragma unroll
+                for (int i = 0; i < 4; i++) af[i] = zero8();
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i].v, bf[j].v, acc[i][j], 0, 0, 0);
         }
     }
-    const long long mbase = (long long)blk * 64 + wave * 16;
+    const int oz = z0 + wv;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
         const int co = j * 16 + r;
         const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const long long mr = mbase + 4 * g + i;
-            if (mr < mtot && co < cout) {
-                float v = acc[j][i] + bv;
-                if (relu) v = v > 0.f ? v : 0.f;
-                y[mr * ycs + co] = f2bf(v);
+            const int oy = y0 + i;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int ox = x0 + 4 * g + q;
+                if (ox < W && oy < H && oz < D && co < cout) {
+                    const long long mr = (((long long)bn * D + oz) * H + oy) * W + ox;
+                    float v = acc[i][j][q] + bv;
+                    if (relu) v = v > 0.f ? v : 0.f;
+                    y[mr * ycs + co] = f2bf(v);
+                }
             }
         }
     }
@@ -403,21 +589,25 @@ __global__ void upsample_bwd(const uint16_t* __restrict__ dy, int dycs, uint16_t
 }
 
 /* ---- 8-channel (16-byte) forms of the elementwise kernels, used when
- * channel counts, strides and pointers allow ------------------------------ */
+ * channel counts, strides and pointers allow.  I is the index type:
+ * uint32_t whenever every element index and address offset fits (the
+ * common case: 32-bit division by the grid extents is a short sequence,
+ * 64-bit division a long one), else long long. ------------------------- */
 __device__ __forceinline__ void store8(uint16_t* p, const Frag& f) { *(uint4*)p = f.u; }
 
+template <typename I>
 __global__ void maxpool_fwd8(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs,
                              uint8_t* __restrict__ arg, int nb, int D, int H, int W, int c) {
     const int d2 = D / 2, h2 = H / 2, w2 = W / 2, c8 = c / 8;
-    const long long n = (long long)nb * d2 * h2 * w2 * c8;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const I n = (I)nb * d2 * h2 * w2 * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
         const int ch = (int)(i % c8) * 8;
-        const long long ov = i / c8;
-        long long q = ov;
+        const I ov = i / c8;
+        I q = ov;
         const int ox = (int)(q % w2); q /= w2;
         const int oy = (int)(q % h2); q /= h2;
         const int oz = (int)(q % d2);
-        const long long on = q / d2;
+        const I on = q / d2;
         float best[8];
         uint8_t bi[8];
 #pragma unroll
@@ -442,19 +632,20 @@ __global__ void maxpool_fwd8(const uint16_t* __restrict__ x, int xcs, uint16_t* 
     }
 }
 
+template <typename I>
 __global__ void maxpool_bwd8(const uint16_t* __restrict__ dy, int dycs, const uint8_t* __restrict__ arg,
                              uint16_t* __restrict__ dx, int dxcs, int nb, int D, int H, int W, int c) {
     const int d2 = D / 2, h2 = H / 2, w2 = W / 2, c8 = c / 8;
-    const long long n = (long long)nb * D * H * W * c8;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const I n = (I)nb * D * H * W * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
         const int ch = (int)(i % c8) * 8;
-        const long long v = i / c8;
-        long long q = v;
+        const I v = i / c8;
+        I q = v;
         const int xx = (int)(q % W); q /= W;
         const int yy = (int)(q % H); q /= H;
         const int zz = (int)(q % D);
-        const long long nn = q / D;
-        const long long o = ((nn * d2 + zz / 2) * h2 + yy / 2) * w2 + xx / 2;
+        const I nn = q / D;
+        const I o = ((nn * d2 + zz / 2) * h2 + yy / 2) * w2 + xx / 2;
         const int k = ((zz & 1) << 2) | ((yy & 1) << 1) | (xx & 1);
         const uint2 a = *(const uint2*)(arg + o * c + ch);
         const Frag g = load8(dy + o * dycs + ch);
@@ -468,34 +659,36 @@ __global__ void maxpool_bwd8(const uint16_t* __restrict__ dy, int dycs, const ui
     }
 }
 
+template <typename I>
 __global__ void upsample_fwd8(const uint16_t* __restrict__ x, int xcs, uint16_t* __restrict__ y, int ycs, int nb, int D,
                               int H, int W, int c) {
     const int D2 = 2 * D, H2 = 2 * H, W2 = 2 * W, c8 = c / 8;
-    const long long n = (long long)nb * D2 * H2 * W2 * c8;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const I n = (I)nb * D2 * H2 * W2 * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
         const int ch = (int)(i % c8) * 8;
-        const long long v = i / c8;
-        long long q = v;
+        const I v = i / c8;
+        I q = v;
         const int xx = (int)(q % W2); q /= W2;
         const int yy = (int)(q % H2); q /= H2;
         const int zz = (int)(q % D2);
-        const long long nn = q / D2;
+        const I nn = q / D2;
         store8(y + v * ycs + ch, load8(x + (((nn * D + zz / 2) * H + yy / 2) * W + xx / 2) * xcs + ch));
     }
 }
 
+template <typename I>
 __global__ void upsample_bwd8(const uint16_t* __restrict__ dy, int dycs, uint16_t* __restrict__ dx, int dxcs, int nb, int D,
                               int H, int W, int c) {
     const int H2 = 2 * H, W2 = 2 * W, c8 = c / 8;
-    const long long n = (long long)nb * D * H * W * c8;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const I n = (I)nb * D * H * W * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
         const int ch = (int)(i % c8) * 8;
-        const long long v = i / c8;
-        long long q = v;
+        const I v = i / c8;
+        I q = v;
         const int xx = (int)(q % W); q /= W;
         const int yy = (int)(q % H); q /= H;
         const int zz = (int)(q % D);
-        const long long nn = q / D;
+        const I nn = q / D;
         float s8[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) s8[j] = 0.f;
@@ -512,12 +705,13 @@ __global__ void upsample_bwd8(const uint16_t* __restrict__ dy, int dycs, uint16_
     }
 }
 
+template <typename I>
 __global__ void add_bf16_8(const uint16_t* __restrict__ a, int acs, const uint16_t* __restrict__ b, int bcs,
-                           uint16_t* __restrict__ out, int ocs, long long nvox, int c) {
+                           uint16_t* __restrict__ out, int ocs, I nvox, int c) {
     const int c8 = c / 8;
-    const long long n = nvox * c8;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const long long v = i / c8;
+    const I n = nvox * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
+        const I v = i / c8;
         const int ch = (int)(i % c8) * 8;
         const Frag fa = load8(a + v * acs + ch), fb = load8(b + v * bcs + ch);
         Frag r;
@@ -547,9 +741,19 @@ __global__ void head_fwd(const uint16_t* __restrict__ f, int fcs, const float* _
                          float* __restrict__ dpred, float* __restrict__ lpart, long long nvox, int c) {
     __shared__ float red[256];
     float l = 0.f;
+    const bool vec = ((fcs | c) & 7) == 0 && ((uintptr_t)f & 15u) == 0;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long long)gridDim.x * blockDim.x) {
         float s = b[0];
-        for (int k = 0; k < c; k++) s += bf2f(f[v * fcs + k]) * w[k];
+        if (vec) {
+            /* 8 channels per 16-byte load, summed in channel order */
+            for (int k = 0; k < c; k += 8) {
+                const Frag q = load8(f + v * fcs + k);
+#pragma unroll
+                for (int j = 0; j < 8; j++) s += bf2f(q.h[j]) * w[k + j];
+            }
+        } else {
+            for (int k = 0; k < c; k++) s += bf2f(f[v * fcs + k]) * w[k];
+        }
         s += bf2f(res[v * rcs]);
         pred[v] = s;
         if (target) {
@@ -680,6 +884,13 @@ static int dn_ret() {
 
 static bool al16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+/* every element offset of nvox voxels with channel strides up to
+ * max(cs1, cs2) fits in 32 bits */
+static bool fits32(long long nvox, int cs1, int cs2) {
+    const long long cs = cs1 > cs2 ? cs1 : cs2;
+    return nvox > 0 && cs > 0 && nvox <= (long long)0xFFFFFFFFu / (cs + 8);
+}
+
 /* 8 channels per thread: channel count and strides multiples of 8, both
  * pointers 16-byte aligned */
 static bool vec8(int c, int s1, int s2, const void* p1, const void* p2) {
@@ -692,19 +903,38 @@ extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mc
     if (cin <= 0 || cin % 8 || cout <= 0 || cout % 16 || xcs < cin || xcs % 8 || ycs < cout) return -22;
     if (kpad < 27 * cin || kpad % 32 || !al16(x) || !al16(w) || (mask && (!al16(mask) || mcs % 8 || mcs < cin))) return -22;
     const long long m = (long long)nb * d * h * wd;
-    const unsigned int grid = (unsigned int)((m + 63) / 64);
     hipStream_t st = (hipStream_t)stream;
     const uint16_t *xp = (const uint16_t*)x, *mp = (const uint16_t*)mask, *wp = (const uint16_t*)w;
     uint16_t* yp = (uint16_t*)y;
-#define DN_FWD(NT) hipLaunchKernelGGL(conv3_fwd<NT>, dim3(grid), dim3(256), 0, st, xp, xcs, mp, mcs, wp, kpad, bias, yp, ycs, nb, d, h, wd, cin, cout, relu)
+    if (cout <= 32 && (cin == 8 || cin % 16 == 0)) {
+        /* halo-tiled kernel: 16 x 4 x 4 output voxels per workgroup */
+        const long long nblk = (long long)((wd + 15) / 16) * ((h + 3) / 4) * ((d + 3) / 4) * nb;
+        if (nblk > 0x7fffffffLL) return -22;
+#define DN_HALO(CG, NT)                                                                                            \
+    hipLaunchKernelGGL((conv3_fwd_halo<CG, NT>), dim3((unsigned int)nblk), dim3(256), 0, st, xp, xcs, mp, mcs, wp, kpad, \
+                       bias, yp, ycs, nb, d, h, wd, cin, cout, relu)
+        if (cin == 8) {
+            if (cout == 16) DN_HALO(8, 1);
+            else DN_HALO(8, 2);
+        } else {
+            if (cout == 16) DN_HALO(16, 1);
+            else DN_HALO(16, 2);
+        }
+#undef DN_HALO
+        return dn_ret();
+    }
+    /* voxel tiles per wave: more where the weight fragments are few */
+#define DN_FWD(MT, NT)                                                                                              \
+    hipLaunchKernelGGL((conv3_fwd<MT, NT>), dim3((unsigned int)((m + 64 * MT - 1) / (64 * MT))), dim3(256), 0, st, xp, \
+                       xcs, mp, mcs, wp, kpad, bias, yp, ycs, nb, d, h, wd, cin, cout, relu)
     switch (cout / 16) {
-        case 1: DN_FWD(1); break;
-        case 2: DN_FWD(2); break;
-        case 3: DN_FWD(3); break;
-        case 4: DN_FWD(4); break;
-        case 6: DN_FWD(6); break;
-        case 8: DN_FWD(8); break;
-        case 12: DN_FWD(12); break;
+        case 1: DN_FWD(4, 1); break;
+        case 2: DN_FWD(4, 2); break;
+        case 3: DN_FWD(2, 3); break;
+        case 4: DN_FWD(2, 4); break;
+        case 6: DN_FWD(2, 6); break;
+        case 8: DN_FWD(2, 8); break;
+        case 12: DN_FWD(1, 12); break;
         default: return -22;
     }
 #undef DN_FWD
@@ -768,13 +998,22 @@ extern "C" int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int 
     return dn_ret();
 }
 
+/* launch an 8-channel kernel with 32-bit indices when they fit */
+#define DN_LAUNCH8(K, small, n8, ...)                                                                          \
+    do {                                                                                                       \
+        if (small)                                                                                             \
+            hipLaunchKernelGGL(K<uint32_t>, dim3(dn_grid(n8)), dim3(256), 0, (hipStream_t)stream, __VA_ARGS__); \
+        else                                                                                                   \
+            hipLaunchKernelGGL(K<long long>, dim3(dn_grid(n8)), dim3(256), 0, (hipStream_t)stream, __VA_ARGS__); \
+    } while (0)
+
 extern "C" int mcx_dn_maxpool_fwd(const void* x, int xcs, void* y, int ycs, unsigned char* arg, int nb, int d, int h, int wd,
                                   int c, void* stream) {
     if (!x || !y || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || xcs < c || ycs < c) return -22;
     const long long n = (long long)nb * (d / 2) * (h / 2) * (wd / 2) * c;
     if (vec8(c, xcs, ycs, x, y) && ((uintptr_t)arg & 7u) == 0)
-        hipLaunchKernelGGL(maxpool_fwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs,
-                           (uint16_t*)y, ycs, (uint8_t*)arg, nb, d, h, wd, c);
+        DN_LAUNCH8(maxpool_fwd8, fits32((long long)nb * d * h * wd, xcs, ycs), n / 8, (const uint16_t*)x, xcs, (uint16_t*)y, ycs,
+                   (uint8_t*)arg, nb, d, h, wd, c);
     else
         hipLaunchKernelGGL(maxpool_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
                            ycs, (uint8_t*)arg, nb, d, h, wd, c);
@@ -786,8 +1025,8 @@ extern "C" int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char*
     if (!dy || !dx || !arg || d % 2 || h % 2 || wd % 2 || c <= 0 || dycs < c || dxcs < c) return -22;
     const long long n = (long long)nb * d * h * wd * c;
     if (vec8(c, dycs, dxcs, dy, dx) && ((uintptr_t)arg & 7u) == 0)
-        hipLaunchKernelGGL(maxpool_bwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
-                           (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+        DN_LAUNCH8(maxpool_bwd8, fits32((long long)nb * d * h * wd, dycs, dxcs), n / 8, (const uint16_t*)dy, dycs,
+                   (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
     else
         hipLaunchKernelGGL(maxpool_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
                            (const uint8_t*)arg, (uint16_t*)dx, dxcs, nb, d, h, wd, c);
@@ -799,8 +1038,8 @@ extern "C" int mcx_dn_upsample_fwd(const void* x, int xcs, void* y, int ycs, int
     if (!x || !y || c <= 0 || xcs < c || ycs < c) return -22;
     const long long n = (long long)nb * 8 * d * h * wd * c;
     if (vec8(c, xcs, ycs, x, y))
-        hipLaunchKernelGGL(upsample_fwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs,
-                           (uint16_t*)y, ycs, nb, d, h, wd, c);
+        DN_LAUNCH8(upsample_fwd8, fits32((long long)nb * 8 * d * h * wd, xcs, ycs), n / 8, (const uint16_t*)x, xcs, (uint16_t*)y,
+                   ycs, nb, d, h, wd, c);
     else
         hipLaunchKernelGGL(upsample_fwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)x, xcs, (uint16_t*)y,
                            ycs, nb, d, h, wd, c);
@@ -811,8 +1050,8 @@ extern "C" int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs,
     if (!dy || !dx || c <= 0 || dycs < c || dxcs < c) return -22;
     const long long n = (long long)nb * d * h * wd * c;
     if (vec8(c, dycs, dxcs, dy, dx))
-        hipLaunchKernelGGL(upsample_bwd8, dim3(dn_grid(n / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
-                           (uint16_t*)dx, dxcs, nb, d, h, wd, c);
+        DN_LAUNCH8(upsample_bwd8, fits32((long long)nb * 8 * d * h * wd, dycs, dxcs), n / 8, (const uint16_t*)dy, dycs,
+                   (uint16_t*)dx, dxcs, nb, d, h, wd, c);
     else
         hipLaunchKernelGGL(upsample_bwd, dim3(dn_grid(n)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)dy, dycs,
                            (uint16_t*)dx, dxcs, nb, d, h, wd, c);
@@ -822,13 +1061,14 @@ extern "C" int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs,
 extern "C" int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* out, int ocs, long long nvox, int c, void* stream) {
     if (!a || !b || !out || c <= 0 || nvox <= 0) return -22;
     if (vec8(c, acs, bcs, a, b) && ocs % 8 == 0 && al16(out))
-        hipLaunchKernelGGL(add_bf16_8, dim3(dn_grid(nvox * c / 8)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
-                           (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
+        DN_LAUNCH8(add_bf16_8, fits32(nvox, acs, bcs > ocs ? bcs : ocs), nvox * c / 8, (const uint16_t*)a, acs,
+                   (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
     else
         hipLaunchKernelGGL(add_bf16, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
                            (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
     return dn_ret();
 }
+#undef DN_LAUNCH8
 
 /* lpart: gridsize floats (gridsize returned through *nblocks) */
 extern "C" int mcx_dn_head_fwd(const void* f, int fcs, const float* w, const float* b, const void* res, int rcs,

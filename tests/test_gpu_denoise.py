@@ -54,16 +54,20 @@ def _weights(rng, cin, cout):
 SHAPE = (2, 6, 10, 12)
 
 
-@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 32), (48, 16), (32, 64), (192, 64)])
-def test_conv3_forward(cin, cout):
+@pytest.mark.parametrize("cin,cout,shape", [(8, 16, SHAPE), (16, 32, SHAPE), (48, 16, SHAPE), (32, 64, SHAPE),
+                                            (192, 64, SHAPE), (8, 16, (1, 9, 7, 37)), (16, 16, (1, 9, 7, 37)),
+                                            (48, 32, (2, 5, 9, 33)), (96, 32, (1, 8, 8, 16))])
+def test_conv3_forward(cin, cout, shape):
+    """Both kernels (the LDS halo tile for cout <= 32, the per-wave tiles
+    otherwise), on shapes that are and are not multiples of the tile."""
     rng = np.random.default_rng(cin * 100 + cout)
-    x = dn_ref.bf16_round(rng.standard_normal(SHAPE + (cin,)))
+    x = dn_ref.bf16_round(rng.standard_normal(shape + (cin,)))
     w, wpad, kp = _weights(rng, cin, cout)
     b = rng.standard_normal(cout).astype(np.float32)
     dx, dw = _dev_bf16(x), _dev_bf16(wpad)
     db = torch.from_numpy(b).cuda()
-    y = torch.zeros(int(np.prod(SHAPE)) * cout, dtype=torch.int16, device="cuda")
-    n, d, h, wd = SHAPE
+    y = torch.zeros(int(np.prod(shape)) * cout, dtype=torch.int16, device="cuda")
+    n, d, h, wd = shape
     assert L.mcx_dn_conv3_fwd(_p(dx), cin, None, 0, _p(dw), kp, _p(db), _p(y), cout, n, d, h, wd, cin, cout, 1, None) == 0
     ref = dn_ref.conv3_fwd(x, w, b, relu=True)
     _bf_close(_host_f32(y, ref.shape), ref)
