@@ -32,6 +32,18 @@ TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 8403.0}
 ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 
+VALU_PEAK_TFLOPS = 157.3  # FP32 vector peak of MI355X
+
+
+def _valu(steps_pp, scat_pp, photons, seconds):
+    """VALU fraction by the stated convention: 30 FP32 ops per propagation
+    step and 60 per scattering event, per photon, against the FP32 vector
+    peak.  The issued-instruction count is in the counter profile."""
+    tflops = photons * (30.0 * steps_pp + 60.0 * scat_pp) / seconds / 1e12
+    return {"achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / VALU_PEAK_TFLOPS, 5), "convention": "30 flop/step + 60 flop/scatter"}
+
+
 def _atomic_roof(mode, nelem):
     """Scattered-atomic rate (atomics/s) over an array of nelem elements."""
     from mcx_amd import _lib
@@ -99,7 +111,8 @@ def bench_batch(args):
     for _ in range(args.steps):
         t0 = time.perf_counter()
         _, info = mcx_amd.run_batch(cfgs, tally=tally, photons_per_lane=info_w["photons_per_lane"],
-                                    block_size=info_w["block_size"], use_graph=True, repeat=1, download=False)
+                                    block_size=info_w["block_size"], use_graph=True, repeat=1, download=False,
+                                    tally_cache=info_w["tally_cache_cells"] or -1)
         times.append(((time.perf_counter() - t0) * 1e3, info["total_ms"]))
     nph = info_w["photons"]
     g_ms = sum(t[1] for t in times) / len(times)
@@ -118,12 +131,14 @@ def bench_batch(args):
                                "0-3 ellipsoids, 10 gates x 0.5 ns), %.4g photons, %s tally, one persistent launch "
                                "in a hipGraph" % (nph, ["FP32+shadow", "FP32 hi/lo", "FP64"][tally]),
                    "graph_ms": round(g_ms, 3), "lanes": info_w["lanes"], "photons_per_lane": info_w["photons_per_lane"],
-                   "block_size": info_w["block_size"], "vgprs": info_w["vgprs"],
+                   "block_size": info_w["block_size"], "tally_cache_cells": info_w["tally_cache_cells"],
+                   "vgprs": info_w["vgprs"],
                    "steps_per_photon": round(steps_pp, 2), "deposits_per_photon": round(dep_pp, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3),
-                                "unit": "G atomics/s", "frac": round(arate / aroof, 4)}},
+                                "unit": "G atomics/s", "frac": round(arate / aroof, 4)},
+                     "valu": _valu(steps_pp, ctr["nscatter"] / nph, nph, g_ms * 1e-3)},
     }
     if not args.no_cpu:
         from parity_util import oracle_run
@@ -272,6 +287,7 @@ def main():
             "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3), "unit": "G atomics/s",
                        "frac": round(arate / aroof, 4),
                        "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},
+            "valu": _valu(steps_pp, stats_w["nscatter"] / nph, nph, k_ms * 1e-3),
         },
     }
     if not args.no_cpu:

@@ -85,6 +85,10 @@ typedef struct McxBatchOptions {
     double lanes;            /* out: virtual threads over all instances */
     double counters[4];      /* out, with mcx_amd_set_counters(1): steps, deposits, scatters, launches of the
                                 last execution; zeros otherwise */
+    int   tally_cache_cells; /* per-wave LDS tally cache (hot-cell box at each instance's source): < 0 none,
+                                > 0 at most this many cells, 0 autotuned (the largest that fits when the
+                                autotuner does not run) */
+    int   chosen_tally_cache_cells; /* out: cells per wave used (0: none) */
 } McxBatchOptions;
 
 /* Runs `ncfg` independent simulations in one persistent kernel launch.

@@ -194,7 +194,8 @@ class McxBatchOptions(ctypes.Structure):
         ("tune_fraction", ctypes.c_float), ("chosen_photons_per_lane", ctypes.c_int),
         ("chosen_block_size", ctypes.c_int), ("vgprs", ctypes.c_int), ("kernel_ms", ctypes.c_float),
         ("total_ms", ctypes.c_float), ("photons", ctypes.c_double), ("lanes", ctypes.c_double),
-        ("counters", ctypes.c_double * 4),
+        ("counters", ctypes.c_double * 4), ("tally_cache_cells", ctypes.c_int),
+        ("chosen_tally_cache_cells", ctypes.c_int),
     ]
 
 

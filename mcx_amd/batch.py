@@ -15,7 +15,9 @@ from .pmcx import _check, _libc, _parse_config
 
 
 def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, repeat=1, download=True,
-              tune_fraction=0.0):
+              tune_fraction=0.0, tally_cache=0):
+    """tally_cache: per-wave LDS tally cache cells (< 0 none, > 0 at most
+    that many, 0 the autotuner's choice or the largest that fits)."""
     L = _lib.lib()
     n = len(cfgs)
     if n == 0:
@@ -50,6 +52,7 @@ def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, 
         opt.repeat = int(repeat)
         opt.download = 1 if download else 0
         opt.tune_fraction = float(tune_fraction)
+        opt.tally_cache_cells = int(tally_cache)
         rc = L.mcx_run_batch(arr, n, gpu, ctypes.byref(opt))
         if rc != 0:
             _check()
@@ -67,6 +70,7 @@ def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, 
         info = {"photons": opt.photons, "lanes": int(opt.lanes), "kernel_ms": float(opt.kernel_ms),
                 "total_ms": float(opt.total_ms), "photons_per_lane": int(opt.chosen_photons_per_lane),
                 "block_size": int(opt.chosen_block_size), "vgprs": int(opt.vgprs),
+                "tally_cache_cells": int(opt.chosen_tally_cache_cells),
                 "counters": dict(zip(("nstep", "ndeposit", "nscatter", "nlaunch"), list(opt.counters)))}
     finally:
         if gpu:

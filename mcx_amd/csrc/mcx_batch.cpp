@@ -24,6 +24,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <climits>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -33,8 +34,8 @@
 #include "mcx_device.h"
 
 extern "C" hipError_t mcxk_launch_batch(int tally, const KParam* params, const uint32_t* chunk_start, const uint32_t* order,
-                                        uint32_t ninst, uint32_t total_chunks, uint32_t maxtab, uint32_t* queue,
-                                        unsigned int grid, unsigned int block, hipStream_t stream);
+                                        uint32_t ninst, uint32_t total_chunks, uint32_t maxtab, uint32_t boxcells,
+                                        uint32_t* queue, unsigned int grid, unsigned int block, hipStream_t stream);
 extern "C" hipError_t mcxk_batch_attrs(int tally, int* numregs);
 extern "C" hipError_t mcxk_launch_batch_post(int tally, const KParam* params, BatchInst* bi, uint32_t ninst, hipStream_t stream);
 extern "C" hipError_t mcxk_fill_zero16(void* p, size_t bytes, hipStream_t stream);
@@ -46,7 +47,18 @@ static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 struct BatchShape {
     int photons_per_lane = 0;  /* 0: each Config's own nthread (autopilot off) */
     int block = 256;
+    int boxcells = 0;          /* hot-cell box cells per wave (tally cache in LDS); 0: none */
 };
+
+/* Largest hot-cell box per wave: the LDS a wave can take at the batched
+ * kernel's full occupancy (and within 64 KB per workgroup), less its
+ * property table, at 8 bytes per cell. */
+static int batch_box_budget(int device, int vgprs, unsigned int block, uint32_t maxtab) {
+    const unsigned int waves_cu = 4u * (unsigned int)waves_per_simd(vgprs);
+    const size_t per_wave = lds_share(device, waves_cu, 65536 / std::max(1u, block / 64u));
+    const size_t tab = (size_t)maxtab * 16;
+    return per_wave > tab ? (int)((per_wave - tab) / 8) : 0;
+}
 
 /* Host-side description of one instance */
 struct InstPlan {
@@ -76,6 +88,7 @@ class BatchRun {
     double photons = 0.0;
     unsigned long long lanes = 0;
     int vgprs = 0;
+    int box_cells() const { return (int)boxcells_; }
 
    private:
     Config** cfgs_;
@@ -85,7 +98,7 @@ class BatchRun {
     int device_;
     std::vector<InstPlan> plan_;
     std::vector<uint32_t> order_, chunk_start_;
-    uint32_t total_chunks_ = 0, maxtab_ = 0;
+    uint32_t total_chunks_ = 0, maxtab_ = 0, boxcells_ = 0;
     unsigned int grid_ = 0;
     DevMem dmedia_, dprop_, dseed_, dzero_, dout_, dkp_, dchunk_, dorder_, dbi_;
     size_t queue_off_ = 0, counters_off_ = 0;
@@ -212,6 +225,9 @@ void BatchRun::setup() {
     const unsigned int block = (unsigned int)shape_.block;
     if (block < 64 || block > 1024 || block % 64) MCX_THROW(-1, "batched launch: block size must be a multiple of 64 in [64, 1024]");
     if ((size_t)(block / 64) * maxtab_ * 16 > 65536) MCX_THROW(-1, "batched launch: property tables exceed the LDS budget");
+    if (shape_.boxcells < 0) MCX_THROW(-1, "batched launch: negative tally cache size");
+    boxcells_ = (uint32_t)std::min(shape_.boxcells, batch_box_budget(device_, vgprs, block, maxtab_));
+    if (boxcells_ < 64) boxcells_ = 0;
     const unsigned long long resident = (unsigned long long)dp.multiProcessorCount * 4 * waves_per_simd(vgprs) * 64;
     grid_ = (unsigned int)std::max<unsigned long long>(1, resident / block);
 
@@ -275,6 +291,10 @@ void BatchRun::setup() {
         kp.seeds = (const uint32_t*)(dseed_.as<char>() + P.off_seed);
         kp.detcount = (uint32_t*)(zb + P.off_energy + 64); /* error-exit counter ([1]) */
         kp.counters = counters_enabled() ? (uint64_t*)(zb + counters_off_) : nullptr;
+        /* each instance's hot-cell box lives in the wave's LDS slice; the
+         * kernel passes the slice's offset */
+        hot_box_geometry(cfg, boxcells_, kp);
+        kp.hot_off = 0;
         BatchInst& b = bis[i];
         memset(&b, 0, sizeof(b));
         b.out = (float*)(dout_.as<char>() + P.off_out);
@@ -298,7 +318,7 @@ void BatchRun::setup() {
 void BatchRun::enqueue(hipStream_t s) {
     HIP_CHECK(mcxk_fill_zero16(dzero_.p, dzero_.bytes, s));
     HIP_CHECK(mcxk_launch_batch(tally_, dkp_.as<KParam>(), dchunk_.as<uint32_t>(), dorder_.as<uint32_t>(), (uint32_t)n_,
-                                total_chunks_, maxtab_, (uint32_t*)(dzero_.as<char>() + queue_off_), grid_,
+                                total_chunks_, maxtab_, boxcells_, (uint32_t*)(dzero_.as<char>() + queue_off_), grid_,
                                 (unsigned int)shape_.block, s));
     HIP_CHECK(mcxk_launch_batch_post(tally_, dkp_.as<KParam>(), dbi_.as<BatchInst>(), (uint32_t)n_, s));
 }
@@ -329,7 +349,7 @@ float BatchRun::run(hipStream_t s, bool graph, int repeat, float* kernel_ms) {
             HIP_CHECK(mcxk_fill_zero16(dzero_.p, dzero_.bytes, s));
             HIP_CHECK(hipEventRecord(kev.a, s));
             HIP_CHECK(mcxk_launch_batch(tally_, dkp_.as<KParam>(), dchunk_.as<uint32_t>(), dorder_.as<uint32_t>(), (uint32_t)n_,
-                                        total_chunks_, maxtab_, (uint32_t*)(dzero_.as<char>() + queue_off_), grid_,
+                                        total_chunks_, maxtab_, boxcells_, (uint32_t*)(dzero_.as<char>() + queue_off_), grid_,
                                         (unsigned int)shape_.block, s));
             HIP_CHECK(hipEventRecord(kev.b, s));
             HIP_CHECK(mcxk_launch_batch_post(tally_, dkp_.as<KParam>(), dbi_.as<BatchInst>(), (uint32_t)n_, s));
@@ -396,9 +416,10 @@ void BatchRun::read_counters(hipStream_t s, double* out) {
  * 256-instance workload: at 5.5e8 photons 8/16/32/64/128/256 photons per
  * lane give 13.2/16.4/19.7/22.8/25.4/25.8 k photons/ms (256: 11 chunks per
  * wave), at 5.5e7 photons 8/16/32/64 give 13.0/15.9/17.7/18.6 k (64: 4.4
- * chunks per wave).  The workgroup size is then measured on a reduced copy
- * (every instance, photon counts scaled down); the fastest wins.  Results
- * are cached per batch signature for the process lifetime. */
+ * chunks per wave).  The workgroup size and the tally cache (no hot-cell
+ * box, or the largest that fits the LDS) are then measured on a reduced
+ * copy (every instance, photon counts scaled down); the fastest pair wins.
+ * Results are cached per batch signature for the process lifetime. */
 static const double kChunksPerWave = 4.0;
 static std::mutex g_tune_mu;
 static std::map<std::pair<int, long long>, BatchShape> g_tune_cache;
@@ -428,17 +449,22 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
         if (chunks >= kChunksPerWave * waves) best.photons_per_lane = ppl;
     }
     double best_rate = -1.0;
+    const BatchShape base = best;
     for (int b : blk_c) {
-        BatchShape sh = best;
-        sh.block = b;
-        BatchRun r(cfgs, n, tally, sh, fraction, device);
-        r.setup();
-        float kms = 0.f;
-        r.run(s, false, 1, &kms);
-        const double rate = r.photons / std::max(1e-3, (double)kms);
-        if (rate > best_rate) {
-            best_rate = rate;
-            best.block = b;
+        for (int box : {0, INT_MAX}) {
+            BatchShape sh = base;
+            sh.block = b;
+            sh.boxcells = box;
+            BatchRun r(cfgs, n, tally, sh, fraction, device);
+            r.setup();
+            float kms = 0.f;
+            r.run(s, false, 1, &kms);
+            const double rate = r.photons / std::max(1e-3, (double)kms);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best.block = b;
+                best.boxcells = box;
+            }
         }
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -462,6 +488,9 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     BatchShape shape;
     shape.photons_per_lane = opt->photons_per_lane;
     shape.block = opt->block_size;
+    /* tally cache: < 0 none, > 0 at most that many cells per wave, 0 the
+     * autotuner's choice (the largest that fits when it does not run) */
+    shape.boxcells = opt->tally_cache_cells < 0 ? 0 : (opt->tally_cache_cells > 0 ? opt->tally_cache_cells : INT_MAX);
     bool need_ppl = false;
     for (int i = 0; i < n; i++)
         if (cfgs[i]->autopilot || cfgs[i]->nthread <= 0) need_ppl = true;
@@ -471,6 +500,7 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
         BatchShape t = autotune(cfgs, n, tally, device, frac, st.s);
         if (shape.photons_per_lane == 0 && need_ppl) shape.photons_per_lane = t.photons_per_lane;
         if (shape.block == 0) shape.block = t.block;
+        if (opt->tally_cache_cells == 0) shape.boxcells = t.boxcells;
     }
     if (!need_ppl && opt->photons_per_lane == 0) shape.photons_per_lane = 0;
     BatchRun r(cfgs, n, tally, shape, 1.0, device);
@@ -487,6 +517,7 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     opt->chosen_block_size = shape.block;
     opt->lanes = (double)r.lanes;
     opt->vgprs = r.vgprs;
+    opt->chosen_tally_cache_cells = r.box_cells();
     return 0;
 }
 

@@ -71,6 +71,11 @@ int waves_per_simd(int vgprs);
 void fill_kparam(const Config* cfg, unsigned int maxgate, unsigned int is2d, KParam& kp);
 /* whether work counters are requested (mcx_amd_set_counters) */
 int counters_enabled();
+/* hot-cell box of at most `cells` cells at the source (KParam::hot_*;
+ * hot_off is left to the caller) */
+void hot_box_geometry(const Config* cfg, uint32_t cells, KParam& kp);
+/* LDS bytes per co-resident group when `groups` share one CU, at most cap */
+size_t lds_share(int device, unsigned int groups, size_t cap);
 
 }  // namespace mcx
 

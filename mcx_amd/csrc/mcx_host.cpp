@@ -117,31 +117,17 @@ static void box_dims(const uint32_t d[3], uint32_t c, uint32_t b[3]) {
     }
 }
 
-/* The hot-cell box of a single-simulation launch (KParam::hot_*): gate 0
- * of the time window, voxels around the source's voxel, reaching into the
- * medium along the launch direction, as large as the LDS a workgroup can
- * use at the kernel's full occupancy allows (at most 64 KB in all).
- * Returns the workgroup's total dynamic LDS bytes. */
-static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int numregs, size_t shmem, KParam& kp) {
+/* Geometry of a hot-cell box of at most `cells` cells (KParam::hot_*):
+ * gate 0 of the time window, voxels around the source's voxel, reaching
+ * into the medium along the launch direction, clamped to the grid.  Sets
+ * hot_cells = 0 when fewer than 64 cells fit. */
+void hot_box_geometry(const Config* cfg, uint32_t cells, KParam& kp) {
     kp.hot_cells = 0;
-    if (!g_hotbox || tally == MCX_TALLY_F32_SHADOW || kp.maxgate == 0 || kp.nvox == 0) return shmem;
-    int dev = 0, lds_cu = 0, lds_blk = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
-        hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds_cu <= 0 ||
-        lds_blk <= 0)
-        return shmem;
-    const unsigned int waves_cu = 4u * (unsigned int)waves_per_simd(numregs);
-    const unsigned int wgs = std::max(1u, waves_cu / std::max(1u, block / 64u));
-    size_t per = std::min<size_t>({(size_t)65536, (size_t)lds_blk, (size_t)lds_cu / wgs});
-    per &= ~(size_t)511; /* LDS is allocated in 512-byte granules */
-    const size_t off = (shmem + 7) & ~(size_t)7;
-    if (per <= off + 64 * sizeof(double)) return shmem;
+    if (cells < 64 || kp.maxgate == 0 || kp.nvox == 0) return;
     const uint32_t d[3] = {kp.nx, kp.ny, kp.nz};
     uint32_t b[3];
-    box_dims(d, (uint32_t)((per - off) / sizeof(double)), b);
-    const uint32_t cells = b[0] * b[1] * b[2];
-    if (cells < 64) return shmem;
+    box_dims(d, cells, b);
+    if (b[0] * b[1] * b[2] < 64) return;
     const float* src = &cfg->srcpos.x;
     const float* dir = &cfg->srcdir.x;
     uint32_t o[3];
@@ -150,8 +136,7 @@ static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int 
         const double lo = std::floor(c - 0.5 * b[k] + 0.5);
         o[k] = (uint32_t)std::max(0.0, std::min((double)(d[k] - b[k]), lo));
     }
-    kp.hot_cells = cells;
-    kp.hot_off = (uint32_t)off;
+    kp.hot_cells = b[0] * b[1] * b[2];
     kp.hot_g = 1;
     kp.hot_x0 = o[0];
     kp.hot_y0 = o[1];
@@ -161,7 +146,37 @@ static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int 
     kp.hot_bz = b[2];
     kp.hot_lo = (o[2] * kp.ny + o[1]) * kp.nx + o[0];
     kp.hot_span = ((o[2] + b[2] - 1) * kp.ny + (o[1] + b[1] - 1)) * kp.nx + (o[0] + b[0] - 1) - kp.hot_lo;
-    return off + (size_t)cells * sizeof(double);
+}
+
+/* LDS bytes each of `groups` co-resident workgroups (or waves) per CU can
+ * take: the CU's LDS shared evenly, at most `cap`, in 512-byte granules */
+size_t lds_share(int device, unsigned int groups, size_t cap) {
+    int lds_cu = 0, lds_blk = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lds_cu <= 0 ||
+        lds_blk <= 0)
+        return 0;
+    size_t per = std::min<size_t>({cap, (size_t)lds_blk, (size_t)lds_cu / std::max(1u, groups)});
+    return per & ~(size_t)511;
+}
+
+/* The hot-cell box of a single-simulation launch: as large as the LDS a
+ * workgroup can take at the kernel's full occupancy allows (at most 64 KB
+ * in all).  Returns the workgroup's total dynamic LDS bytes. */
+static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int numregs, size_t shmem, KParam& kp) {
+    kp.hot_cells = 0;
+    if (!g_hotbox || tally == MCX_TALLY_F32_SHADOW || kp.maxgate == 0 || kp.nvox == 0) return shmem;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return shmem;
+    const unsigned int waves_cu = 4u * (unsigned int)waves_per_simd(numregs);
+    const unsigned int wgs = std::max(1u, waves_cu / std::max(1u, block / 64u));
+    const size_t per = lds_share(dev, wgs, 65536);
+    const size_t off = (shmem + 7) & ~(size_t)7;
+    if (per <= off + 64 * sizeof(double)) return shmem;
+    hot_box_geometry(cfg, (uint32_t)((per - off) / sizeof(double)), kp);
+    if (!kp.hot_cells) return shmem;
+    kp.hot_off = (uint32_t)off;
+    return off + (size_t)kp.hot_cells * sizeof(double);
 }
 
 /* launch constants of one simulation (buffers, work split and time

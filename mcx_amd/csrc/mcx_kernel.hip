@@ -148,6 +148,7 @@ struct Lane {
     double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t gid;         /* global index of the current photon (replay) */
     uint32_t nstep, ndep, nscat, nlaunch, nhot;
+    uint32_t hoff;        /* byte offset of this lane's hot-cell box in the dynamic LDS */
     /* split-voxel state (src/mcx_core.h:65-83): interface point and
      * normal of the current voxel, its two labels, which side the photon
      * is on, whether the voxel is split; whether the current path segment
@@ -157,7 +158,7 @@ struct Lane {
     bool testint, hitintf;
 
     __device__ __forceinline__ Lane(const KParam& k, const F4* tb, float* ppb, uint32_t s, uint32_t id)
-        : kp(k), tab(tb), pp(ppb), stride(s), idx(id) {}
+        : kp(k), tab(tb), pp(ppb), stride(s), idx(id), hoff(k.hot_off) {}
 
     __device__ __forceinline__ float& ppath(uint32_t i) { return pp[i * stride]; }
 
@@ -904,9 +905,20 @@ struct Lane {
             const uint32_t lx = r - y * kp.nx - kp.hot_x0, ly = y - kp.hot_y0, lz = z - kp.hot_z0;
             if (lx < kp.hot_bx && ly < kp.hot_by && lz < kp.hot_bz) {
                 extern __shared__ F4 lds[];
-                double* box = (double*)((char*)lds + kp.hot_off);
                 const uint32_t c = (((uint32_t)tshift * kp.hot_bz + lz) * kp.hot_by + ly) * kp.hot_bx + lx;
-                __hip_atomic_fetch_add(box + c, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (TALLY == MCX_TALLY_F32_HILO) {
+                    /* FP32 hi/lo cells: the returned old value gives the
+                     * TwoSum error of the hi add, which goes to the lo half */
+                    float* box = (float*)((char*)lds + hoff);
+                    const float a = __hip_atomic_fetch_add(box + c, weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const float s = a + weight;
+                    const float bb = s - a;
+                    const float e = (a - (s - bb)) + (weight - bb);
+                    if (e != 0.f) __hip_atomic_fetch_add(box + kp.hot_cells + c, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    double* box = (double*)((char*)lds + hoff);
+                    __hip_atomic_fetch_add(box + c, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
                 nhot++;
                 return;
             }
@@ -1256,6 +1268,52 @@ struct Lane {
 
 using namespace mcxk;
 
+/* Hot-cell box (KParam::hot_*) in LDS at byte offset `off`: FP64 cells for
+ * the FP64 tallies, FP32 hi then lo halves for the hi/lo tally (8 bytes per
+ * cell either way).  Both helpers are run by `n` lanes with indices
+ * `t` = 0..n-1 of one workgroup (single simulation) or one wave (batch). */
+template <int TALLY>
+__device__ __forceinline__ void hot_box_zero(const KParam& kp, uint32_t off, uint32_t t, uint32_t n) {
+    extern __shared__ F4 lds[];
+    uint2* box = (uint2*)((char*)lds + off);
+    for (uint32_t c = t; c < kp.hot_cells; c += n) box[c] = uint2{0u, 0u};
+}
+
+/* adds the box to the tally: one global atomic per non-zero cell (hi/lo:
+ * the hi half by a returning add whose TwoSum error joins the lo half) */
+template <int TALLY>
+__device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, uint32_t t, uint32_t n) {
+    extern __shared__ F4 lds[];
+    const char* base = (const char*)lds + off;
+    double* t64 = kp.field64;
+    if (TALLY == MCX_TALLY_F64_XCD) t64 += (size_t)xcc_id() * kp.fieldlen;
+    const uint32_t plane = kp.hot_bx * kp.hot_by, vol = plane * kp.hot_bz;
+    for (uint32_t c = t; c < kp.hot_cells; c += n) {
+        const uint32_t g = c / vol, r = c - g * vol, lz = r / plane, r2 = r - lz * plane, ly = r2 / kp.hot_bx;
+        const uint32_t lx = r2 - ly * kp.hot_bx;
+        const size_t i = (size_t)((kp.hot_z0 + lz) * kp.dimxy + (kp.hot_y0 + ly) * kp.nx + kp.hot_x0 + lx) + (size_t)g * kp.nvox;
+        if (TALLY == MCX_TALLY_F32_HILO) {
+            const float w = ((const float*)base)[c];
+            const float lo = ((const float*)base)[kp.hot_cells + c];
+            if (w != 0.f) {
+                const float a = atom_add_ret(kp.field + i, w);
+                const float s = a + w;
+                const float bb = s - a;
+                const float e = (a - (s - bb)) + (w - bb);
+                if (e != 0.f) atom_add_nr(kp.field2 + i, e);
+            }
+            if (lo != 0.f) atom_add_nr(kp.field2 + i, lo);
+        } else {
+            const double sum = ((const double*)base)[c];
+            if (sum == 0.0) continue;
+            if (TALLY == MCX_TALLY_F64)
+                atom_add_nr(t64 + i, sum);
+            else
+                __hip_atomic_fetch_add(glb(t64) + i, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+}
+
 /* The parameter block is read through a device pointer: every field is a
  * wave-uniform load served by the scalar cache.  (Binding a by-value kernel
  * argument to a reference would copy the whole block to scratch.) */
@@ -1271,9 +1329,7 @@ __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(con
         lds[i] = F4{g[0], g[1], g[2], g[3]};
     }
     const bool hot = TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells;
-    double* box = (double*)((char*)lds + kp.hot_off);
-    if (hot)
-        for (uint32_t c = threadIdx.x; c < kp.hot_cells; c += blockDim.x) box[c] = 0.0;
+    if (hot) hot_box_zero<TALLY>(kp, kp.hot_off, threadIdx.x, blockDim.x);
     __syncthreads();
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
@@ -1288,32 +1344,7 @@ __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(con
     if (!hot) return;
     /* every lane of the workgroup has finished: add the box to the tally */
     __syncthreads();
-    double* t64 = kp.field64;
-    if (TALLY == MCX_TALLY_F64_XCD) t64 += (size_t)xcc_id() * kp.fieldlen;
-    const uint32_t plane = kp.hot_bx * kp.hot_by, vol = plane * kp.hot_bz;
-    for (uint32_t c = threadIdx.x; c < kp.hot_cells; c += blockDim.x) {
-        const double sum = box[c];
-        if (sum == 0.0) continue;
-        const uint32_t t = c / vol, r = c - t * vol, lz = r / plane, r2 = r - lz * plane, ly = r2 / kp.hot_bx;
-        const uint32_t lx = r2 - ly * kp.hot_bx;
-        const size_t i = (size_t)((kp.hot_z0 + lz) * kp.dimxy + (kp.hot_y0 + ly) * kp.nx + kp.hot_x0 + lx) + (size_t)t * kp.nvox;
-        if (TALLY == MCX_TALLY_F64) {
-            atom_add_nr(t64 + i, sum);
-        } else if (TALLY == MCX_TALLY_F64_XCD) {
-            __hip_atomic_fetch_add(glb(t64) + i, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-            /* hi/lo: the FP32 rounding of the sum and the TwoSum error of
-             * the hi add both go to the lo half */
-            const float w = (float)sum;
-            const float lo = (float)(sum - (double)w);
-            const float a = atom_add_ret(kp.field + i, w);
-            const float s = a + w;
-            const float bb = s - a;
-            const float e = (a - (s - bb)) + (w - bb);
-            if (e != 0.f) atom_add_nr(kp.field2 + i, e);
-            if (lo != 0.f) atom_add_nr(kp.field2 + i, lo);
-        }
-    }
+    hot_box_flush<TALLY>(kp, kp.hot_off, threadIdx.x, blockDim.x);
 }
 
 /* Batched persistent launch: every wave repeatedly takes the next chunk of
@@ -1328,10 +1359,13 @@ template <int TALLY>
 __global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD)
 mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restrict__ chunk_start,
                     const uint32_t* __restrict__ order, uint32_t ninst, uint32_t total_chunks, uint32_t maxtab,
-                    uint32_t* __restrict__ queue) {
+                    uint32_t boxcells, uint32_t* __restrict__ queue) {
     extern __shared__ F4 lds[];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    F4* tab = lds + wave * maxtab;
+    /* per wave: the property table, then the hot-cell box (boxcells x 8 B) */
+    const uint32_t wstride = maxtab * 16u + boxcells * 8u;
+    F4* tab = (F4*)((char*)lds + wave * wstride);
+    const uint32_t boff = wave * wstride + maxtab * 16u;
     while (true) {
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(queue, 1u);
@@ -1351,15 +1385,28 @@ mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restric
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool hot = kp.hot_cells != 0;
+        if (hot) hot_box_zero<TALLY>(kp, boff, lane, 64);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t idx = (c - chunk_start[lo]) * 64u + lane;
         const long long total = (long long)kp.threadphoton * kp.nthread + kp.oddphotons;
         if (idx < kp.nthread && (long long)idx < total) {
             Lane<2, 1, TALLY, uint8_t> L(kp, tab, (float*)tab, 64, idx);
+            L.hoff = boff;
             L.nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
             L.run();
             L.add_counters();
         }
+        /* the chunk's lanes are done: add the wave's box to the instance's tally */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (hot) hot_box_flush<TALLY>(kp, boff, lane, 64);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -1525,16 +1572,18 @@ extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, 
     return e;
 }
 
+/* boxcells: hot-cell box cells per wave (0: none; every instance's
+ * KParam::hot_cells is at most this) */
 extern "C" hipError_t mcxk_launch_batch(int tally, const KParam* params, const uint32_t* chunk_start, const uint32_t* order,
-                                        uint32_t ninst, uint32_t total_chunks, uint32_t maxtab, uint32_t* queue,
-                                        unsigned int grid, unsigned int block, hipStream_t stream) {
-    const size_t shmem = (size_t)(block / 64) * maxtab * sizeof(F4);
+                                        uint32_t ninst, uint32_t total_chunks, uint32_t maxtab, uint32_t boxcells,
+                                        uint32_t* queue, unsigned int grid, unsigned int block, hipStream_t stream) {
+    const size_t shmem = (size_t)(block / 64) * ((size_t)maxtab * sizeof(F4) + (size_t)boxcells * 8);
     if (tally == MCX_TALLY_F32_HILO)
         hipLaunchKernelGGL(mcx_transport_batch<MCX_TALLY_F32_HILO>, dim3(grid), dim3(block), shmem, stream, params, chunk_start,
-                           order, ninst, total_chunks, maxtab, queue);
+                           order, ninst, total_chunks, maxtab, boxcells, queue);
     else if (tally == MCX_TALLY_F64)
         hipLaunchKernelGGL(mcx_transport_batch<MCX_TALLY_F64>, dim3(grid), dim3(block), shmem, stream, params, chunk_start,
-                           order, ninst, total_chunks, maxtab, queue);
+                           order, ninst, total_chunks, maxtab, boxcells, queue);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

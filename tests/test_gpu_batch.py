@@ -69,6 +69,26 @@ def test_batch_hilo_against_fp64_oracle():
         _close(r["flux"].astype(np.float64), ref, 2e-6, 1e-4)
 
 
+@pytest.mark.parametrize("tally", [1, 2])
+def test_batch_tally_cache_on_and_off_agree(tally):
+    """The per-wave LDS tally cache (hot-cell box at each instance's
+    source) changes only where deposits are summed: energies are identical
+    and the fluence agrees to the tally's summation-order error."""
+    cfgs = _small_batch(n=4, master_seed=9)
+    off, i_off = mcx_amd.run_batch(cfgs, tally=tally, block_size=128, tally_cache=-1)
+    on, i_on = mcx_amd.run_batch(cfgs, tally=tally, block_size=128, tally_cache=0)
+    assert i_off["tally_cache_cells"] == 0
+    assert i_on["tally_cache_cells"] >= 512
+    for a, b in zip(on, off):
+        assert a["stat"]["energytot"] == b["stat"]["energytot"]
+        assert a["stat"]["energyabs"] == pytest.approx(b["stat"]["energyabs"], rel=1e-12)
+        _close(a["flux"].astype(np.float64), b["flux"].astype(np.float64), 2e-6, 1e-4)
+    small, i_small = mcx_amd.run_batch(cfgs, tally=tally, block_size=128, tally_cache=300)
+    assert 0 < i_small["tally_cache_cells"] <= 300
+    for a, b in zip(small, off):
+        _close(a["flux"].astype(np.float64), b["flux"].astype(np.float64), 2e-6, 1e-4)
+
+
 def test_batch_graph_replays_match_eager():
     cfgs = _small_batch(n=4, master_seed=3)
     eager, _ = mcx_amd.run_batch(cfgs, tally=1, block_size=128)
