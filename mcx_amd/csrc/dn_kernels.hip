@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace dnk {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -38,13 +40,9 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-/* FP32 -> BF16, round to nearest even (NaN kept quiet) */
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+/* FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32; NaN
+ * stays NaN) */
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
 union Frag {
     uint4 u;
@@ -236,12 +234,33 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < NT; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    /* LDS element offsets of this lane's operands, fixed for the kernel:
+     * A of chunk kc (M tile 0; tile i adds i*HX*CS, an immediate) and B of
+     * chunk 0 (chunk kc adds 32*kc: its slice-tap pair is 4*kc + g, and a
+     * weight row holds the pairs in order, 8 channels each).  A pair past
+     * the 27 taps gets A = 0 and a B offset inside the row. */
+    int aoff[NCH];
+    const int abase = (((wv + 1) * HY + 1) * HX + r + 1) * CS;
+#pragma unroll
+    for (int kc = 0; kc < NCH; kc++) {
+        const int pr = 4 * kc + g;
+        const int t = pr / SPT, sl = pr - t * SPT;
+        int dz, dy, dx;
+        tap_offsets(t < 27 ? t : 26, dz, dy, dx);
+        aoff[kc] = t < 27 ? abase + ((dz * HY + dy) * HX + dx) * CS + 8 * sl : -1;
+    }
+    const int bbase = r * WS + 8 * g;
     for (int c0 = 0; c0 < cin; c0 += CG) {
         /* all of a lane's global loads are issued before any LDS store, so
          * their latencies overlap */
         constexpr int NH = HX * HY * HZ * SPT, NW = 16 * NT * NPAIR;
         constexpr int LH = (NH + 255) / 256, LW = (NW + 255) / 256;
         Frag hv[LH], wf[LW];
+        /* halo voxel (hx, hy, hz) is grid voxel (x0 + hx - 1, ...): element
+         * offsets are the halo origin's (64-bit, once) plus a 32-bit offset
+         * within the slab of HZ planes (the launcher checks that it fits) */
+        const long long org = (((long long)bn * D + (z0 - 1)) * H + (y0 - 1)) * W + (x0 - 1);
+        const long long xorg = org * xcs + c0, morg = org * mcs + c0;
 #pragma unroll
         for (int k = 0; k < LH; k++) {
             const int e = threadIdx.x + 256 * k;
@@ -249,12 +268,12 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
             if (e < NH) {
                 const int sl = e % SPT, vox = e / SPT;
                 const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-                if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) {
-                    const long long m = (((long long)bn * D + gz) * H + gy) * W + gx;
-                    hv[k] = load8(x + m * xcs + c0 + 8 * sl);
+                if ((unsigned)(x0 + hx - 1) < (unsigned)W && (unsigned)(y0 + hy - 1) < (unsigned)H &&
+                    (unsigned)(z0 + hz - 1) < (unsigned)D) {
+                    const int rel = (hz * H + hy) * W + hx;
+                    hv[k] = load8(x + (xorg + rel * xcs + 8 * sl));
                     if (mask) {
-                        const Frag mk = load8(mask + m * mcs + c0 + 8 * sl);
+                        const Frag mk = load8(mask + (morg + rel * mcs + 8 * sl));
 #pragma unroll
                         for (int q = 0; q < 8; q++)
                             if (!(bf2f(mk.h[q]) > 0.f)) hv[k].h[q] = 0;
@@ -287,25 +306,24 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
             }
         }
         __syncthreads();
-        for (int kc = 0; kc < NCH; kc++) {
-            const int pr = 4 * kc + g;
-            const int t = pr / SPT, sl = pr - t * SPT;
-            Frag bf[NT], af[4];
-            if (t < 27) {
-                int dz, dy, dx;
-                tap_offsets(t, dz, dy, dx);
 #pragma unroll
-                for (int j = 0; j < NT; j++) bf[j].u = *(const uint4*)(wtile + (j * 16 + r) * WS + t * CG + 8 * sl);
+        for (int kc = 0; kc < NCH; kc++) {
+            Frag bf[NT], af[4];
+            if (4 * kc + 4 <= NPAIR) {
+                /* every lane's pair is a real tap (known at compile time) */
+#pragma unroll
+                for (int j = 0; j < NT; j++) bf[j].u = *(const uint4*)(wtile + j * 16 * WS + bbase + 32 * kc);
+#pragma unroll
+                for (int i = 0; i < 4; i++) af[i].u = *(const uint4*)(tile + aoff[kc] + i * HX * CS);
+            } else {
+                const bool ok = aoff[kc] >= 0;
+#pragma unroll
+                for (int j = 0; j < NT; j++) bf[j].u = *(const uint4*)(wtile + j * 16 * WS + bbase + (ok ? 32 * kc : 0));
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const int hv = ((wv + dz + 1) * HY + (i + dy + 1)) * HX + (r + dx + 1);
-                    af[i].u = *(const uint4*)(tile + hv * CS + 8 * sl);
+                    af[i].u = *(const uint4*)(tile + (ok ? aoff[kc] : abase) + i * HX * CS);
+                    if (!ok) af[i] = zero8();
                 }
-            } else {
-#pragma unroll
-                for (int j = 0; j < NT; j++) bf[j] = zero8();
-#pragma unroll
-                for (int i = 0; i < 4; i++) af[i] = zero8();
             }
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -314,22 +332,25 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i].v, bf[j].v, acc[i][j], 0, 0, 0);
         }
     }
+    /* output: rows y0..y0+3 of z-slice z0+wv; lane (r, g) holds channel
+     * 16j + r of voxels x0 + 4g + q.  Strides between the 16 outputs are
+     * wave-uniform, and a tile inside the grid needs no bounds checks. */
     const int oz = z0 + wv;
+    const bool inside = x0 + TX <= W && y0 + TY <= H && oz < D;
+    const long long rowstride = (long long)W * ycs;
+    uint16_t* yb = y + ((((long long)bn * D + oz) * H + y0) * W + x0 + 4 * g) * ycs;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
         const int co = j * 16 + r;
         const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int oy = y0 + i;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int ox = x0 + 4 * g + q;
-                if (ox < W && oy < H && oz < D && co < cout) {
-                    const long long mr = (((long long)bn * D + oz) * H + oy) * W + ox;
+                if (inside || (x0 + 4 * g + q < W && y0 + i < H && oz < D)) {
                     float v = acc[i][j][q] + bv;
                     if (relu) v = v > 0.f ? v : 0.f;
-                    y[mr * ycs + co] = f2bf(v);
+                    yb[i * rowstride + q * ycs + co] = f2bf(v);
                 }
             }
         }
@@ -906,7 +927,8 @@ extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mc
     hipStream_t st = (hipStream_t)stream;
     const uint16_t *xp = (const uint16_t*)x, *mp = (const uint16_t*)mask, *wp = (const uint16_t*)w;
     uint16_t* yp = (uint16_t*)y;
-    if (cout <= 32 && (cin == 8 || cin % 16 == 0)) {
+    const long long slab = 6LL * h * wd * std::max(xcs, mask ? mcs : 0); /* 32-bit halo offsets */
+    if (cout <= 32 && (cin == 8 || cin % 16 == 0) && slab < 0x7fffffffLL) {
         /* halo-tiled kernel: 16 x 4 x 4 output voxels per workgroup */
         const long long nblk = (long long)((wd + 15) / 16) * ((h + 3) / 4) * ((d + 3) / 4) * nb;
         if (nblk > 0x7fffffffLL) return -22;
