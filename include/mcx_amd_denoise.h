@@ -28,6 +28,13 @@ extern "C" {
  * mask (NULL: none) has x's layout.  cout multiple of 16. */
 int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mcs, const void* w, int kpad, const float* bias,
                      void* y, int ycs, int nb, int d, int h, int wd, int cin, int cout, int relu, void* stream);
+/* y = act(bias + conv3x3x3(cat(up2(xu), x))): the first cu input channels
+ * are the 2x nearest upsampling of xu (d/2 x h/2 x wd/2 voxels, stride
+ * xucs), the other cin - cu are x; the concatenation is never written.
+ * cout 16 or 32, cu and cin - cu multiples of 16, d, h, wd even. */
+int mcx_dn_conv3_fwd_cat(const void* xu, int xucs, int cu, const void* x, int xcs, const void* w, int kpad,
+                         const float* bias, void* y, int ycs, int nb, int d, int h, int wd, int cin, int cout, int relu,
+                         void* stream);
 /* wt[ci][(26 - tap)*cout + co] = w[co][tap*cin + ci]: the weights of the
  * input-gradient convolution (run with mcx_dn_conv3_fwd) */
 int mcx_dn_weight_flip(const void* w, int kpad, void* wt, int kpadt, int cin, int cout, void* stream);

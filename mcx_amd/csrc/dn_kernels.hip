@@ -211,7 +211,8 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
                                                       const uint16_t* __restrict__ mask, int mcs,
                                                       const uint16_t* __restrict__ w, int kpad,
                                                       const float* __restrict__ bias, uint16_t* __restrict__ y, int ycs,
-                                                      int nb, int D, int H, int W, int cin, int cout, int relu) {
+                                                      int nb, int D, int H, int W, int cin, int cout, int relu,
+                                                      const uint16_t* __restrict__ xu, int xucs, int cu) {
     constexpr int TX = 16, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
     constexpr int CS = (CG / 8) % 2 ? CG + 16 : CG + 8; /* LDS channel stride per voxel: odd 16-byte units */
     constexpr int SPT = CG / 8;          /* 8-channel slices per tap */
@@ -259,8 +260,12 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
         /* halo voxel (hx, hy, hz) is grid voxel (x0 + hx - 1, ...): element
          * offsets are the halo origin's (64-bit, once) plus a 32-bit offset
          * within the slab of HZ planes (the launcher checks that it fits) */
+        /* channels [0, cu) of a concatenated input come from xu, the
+         * coarse tensor that a 2x nearest upsampling would produce them
+         * from; the rest from x (channel c - cu) */
+        const bool up = c0 < cu;
         const long long org = (((long long)bn * D + (z0 - 1)) * H + (y0 - 1)) * W + (x0 - 1);
-        const long long xorg = org * xcs + c0, morg = org * mcs + c0;
+        const long long xorg = org * xcs + (c0 - cu), morg = org * mcs + c0;
 #pragma unroll
         for (int k = 0; k < LH; k++) {
             const int e = threadIdx.x + 256 * k;
@@ -268,8 +273,13 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
             if (e < NH) {
                 const int sl = e % SPT, vox = e / SPT;
                 const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-                if ((unsigned)(x0 + hx - 1) < (unsigned)W && (unsigned)(y0 + hy - 1) < (unsigned)H &&
-                    (unsigned)(z0 + hz - 1) < (unsigned)D) {
+                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) {
+                    if (up) {
+                        const long long mu = (((long long)bn * (D / 2) + gz / 2) * (H / 2) + gy / 2) * (W / 2) + gx / 2;
+                        hv[k] = load8(xu + mu * xucs + c0 + 8 * sl);
+                        continue;
+                    }
                     const int rel = (hz * H + hy) * W + hx;
                     hv[k] = load8(x + (xorg + rel * xcs + 8 * sl));
                     if (mask) {
@@ -934,7 +944,7 @@ extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mc
         if (nblk > 0x7fffffffLL) return -22;
 #define DN_HALO(CG, NT)                                                                                            \
     hipLaunchKernelGGL((conv3_fwd_halo<CG, NT>), dim3((unsigned int)nblk), dim3(256), 0, st, xp, xcs, mp, mcs, wp, kpad, \
-                       bias, yp, ycs, nb, d, h, wd, cin, cout, relu)
+                       bias, yp, ycs, nb, d, h, wd, cin, cout, relu, (const uint16_t*)nullptr, 0, 0)
         if (cin == 8) {
             if (cout == 16) DN_HALO(8, 1);
             else DN_HALO(8, 2);
@@ -960,6 +970,36 @@ extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mc
         default: return -22;
     }
 #undef DN_FWD
+    return dn_ret();
+}
+
+/* y = act(conv3(cat(up2(xu), x))): the convolution of a channel
+ * concatenation whose first cu channels are the 2x nearest upsampling of
+ * xu (dimensions d/2, h/2, wd/2, channel stride xucs) and whose other
+ * cin - cu channels are x (dimensions d, h, wd).  Same result as
+ * mcx_dn_upsample_fwd into the first cu channels of a cin-channel buffer,
+ * x copied into the rest, then mcx_dn_conv3_fwd, without writing the
+ * concatenation.  Output channels at most 32; cu and cin - cu multiples
+ * of 16; d, h, wd even. */
+extern "C" int mcx_dn_conv3_fwd_cat(const void* xu, int xucs, int cu, const void* x, int xcs, const void* w, int kpad,
+                                    const float* bias, void* y, int ycs, int nb, int d, int h, int wd, int cin, int cout,
+                                    int relu, void* stream) {
+    if (!xu || !x || !w || !y || nb <= 0 || d <= 0 || h <= 0 || wd <= 0 || d % 2 || h % 2 || wd % 2) return -22;
+    if (cu <= 0 || cu % 16 || cin <= cu || (cin - cu) % 16 || cout <= 0 || cout > 32 || cout % 16) return -22;
+    if (xucs < cu || xucs % 8 || xcs < cin - cu || xcs % 8 || ycs < cout) return -22;
+    if (kpad < 27 * cin || kpad % 32 || !al16(x) || !al16(xu) || !al16(w)) return -22;
+    const long long slab = 6LL * h * wd * xcs;
+    const long long nblk = (long long)((wd + 15) / 16) * ((h + 3) / 4) * ((d + 3) / 4) * nb;
+    if (slab >= 0x7fffffffLL || nblk > 0x7fffffffLL) return -22;
+    hipStream_t st = (hipStream_t)stream;
+    const uint16_t *xp = (const uint16_t*)x, *up = (const uint16_t*)xu, *wp = (const uint16_t*)w;
+    uint16_t* yp = (uint16_t*)y;
+    if (cout == 16)
+        hipLaunchKernelGGL((conv3_fwd_halo<16, 1>), dim3((unsigned int)nblk), dim3(256), 0, st, xp, xcs,
+                           (const uint16_t*)nullptr, 0, wp, kpad, bias, yp, ycs, nb, d, h, wd, cin, cout, relu, up, xucs, cu);
+    else
+        hipLaunchKernelGGL((conv3_fwd_halo<16, 2>), dim3((unsigned int)nblk), dim3(256), 0, st, xp, xcs,
+                           (const uint16_t*)nullptr, 0, wp, kpad, bias, yp, ycs, nb, d, h, wd, cin, cout, relu, up, xucs, cu);
     return dn_ret();
 }
 

@@ -206,6 +206,10 @@ class UNet:
                          ("d1a", n1, 16), ("d1", n1, 16)]:
             A[nm] = torch.zeros(n * c, **bf)
             A["g_" + nm] = torch.zeros(n * c, **bf)
+        # inference: the encoder outputs in their own buffers (the decoder
+        # reads them through the concatenating convolution)
+        A["e1s"] = torch.zeros(n1 * 16, **bf)
+        A["e2s"] = torch.zeros(n2 * 32, **bf)
         A["arg1"] = torch.zeros(n2 * 16, dtype=torch.uint8, device=self.device)
         A["arg2"] = torch.zeros(n3 * 32, dtype=torch.uint8, device=self.device)
         A["arg3"] = torch.zeros(n4 * 64, dtype=torch.uint8, device=self.device)
@@ -246,22 +250,36 @@ class UNet:
         _chk(self.L.mcx_dn_conv3_fwd(_ptr(x, xoff), xcs, None, 0, _ptr(self.pb, wo), kp, _ptr(self.p, bo), _ptr(y, yoff),
                                      ycs, nb, S, S, S, cin, cout, 1, self._stream()), "conv3_fwd " + name)
 
+    def _conv_cat(self, name, xu, xucs, cu, x, xcs, y, ycs, nb):
+        """conv of cat(upsample2(xu), x) without writing the concatenation"""
+        wo, bo, cin, cout, kp = self.layout[name]
+        S = self._dims_of(name, self._acts)
+        _chk(self.L.mcx_dn_conv3_fwd_cat(_ptr(xu), xucs, cu, _ptr(x), xcs, _ptr(self.pb, wo), kp, _ptr(self.p, bo),
+                                         _ptr(y), ycs, nb, S, S, S, cin, cout, 1, self._stream()), "conv3_fwd_cat " + name)
+
     def forward(self, x0, target=None):
         """x0: torch bf16 [nb, S, S, S, 8] on the device (S multiple of 8).
         Returns the FP32 prediction [nb*S^3]; with a target (FP32 [nb*S^3])
-        also the MSE loss (a device scalar, summed on the host)."""
+        also the MSE loss (a device scalar, summed on the host).  Without a
+        target (inference) the two full-resolution skip concatenations are
+        not written: the decoder reads the encoder outputs and the upsampled
+        coarse maps through mcx_dn_conv3_fwd_cat (same result), and backward
+        needs a forward with a target."""
         nb, S = int(x0.shape[0]), int(x0.shape[1])
         A = self._alloc(nb, S)
         A["x0"].copy_(x0.reshape(-1))
         L, st = self.L, self._stream()
         s1, s2, s3, s4 = A["dims"]
         n1, n2, n3, n4 = A["n"]
+        infer = target is None
+        e1, e1o, e1c = (A["e1s"], 0, 16) if infer else (A["cat1"], 32, 48)
+        e2, e2o, e2c = (A["e2s"], 0, 32) if infer else (A["cat2"], 64, 96)
         self._conv("e1a", A["x0"], 0, NFEAT, A["e1a"], 0, 16, nb)
-        self._conv("e1", A["e1a"], 0, 16, A["cat1"], 32, 48, nb)
-        _chk(L.mcx_dn_maxpool_fwd(_ptr(A["cat1"], 32), 48, _ptr(A["p1"]), 16, _ptr(A["arg1"]), nb, s1, s1, s1, 16, st), "pool1")
+        self._conv("e1", A["e1a"], 0, 16, e1, e1o, e1c, nb)
+        _chk(L.mcx_dn_maxpool_fwd(_ptr(e1, e1o), e1c, _ptr(A["p1"]), 16, _ptr(A["arg1"]), nb, s1, s1, s1, 16, st), "pool1")
         self._conv("e2a", A["p1"], 0, 16, A["e2a"], 0, 32, nb)
-        self._conv("e2", A["e2a"], 0, 32, A["cat2"], 64, 96, nb)
-        _chk(L.mcx_dn_maxpool_fwd(_ptr(A["cat2"], 64), 96, _ptr(A["p2"]), 32, _ptr(A["arg2"]), nb, s2, s2, s2, 32, st), "pool2")
+        self._conv("e2", A["e2a"], 0, 32, e2, e2o, e2c, nb)
+        _chk(L.mcx_dn_maxpool_fwd(_ptr(e2, e2o), e2c, _ptr(A["p2"]), 32, _ptr(A["arg2"]), nb, s2, s2, s2, 32, st), "pool2")
         self._conv("e3a", A["p2"], 0, 32, A["e3a"], 0, 64, nb)
         self._conv("e3", A["e3a"], 0, 64, A["cat3"], 128, 192, nb)
         _chk(L.mcx_dn_maxpool_fwd(_ptr(A["cat3"], 128), 192, _ptr(A["p3"]), 64, _ptr(A["arg3"]), nb, s3, s3, s3, 64, st), "pool3")
@@ -270,11 +288,17 @@ class UNet:
         _chk(L.mcx_dn_upsample_fwd(_ptr(A["b2"]), 128, _ptr(A["cat3"]), 192, nb, s4, s4, s4, 128, st), "up3")
         self._conv("d3a", A["cat3"], 0, 192, A["d3a"], 0, 64, nb)
         self._conv("d3", A["d3a"], 0, 64, A["d3"], 0, 64, nb)
-        _chk(L.mcx_dn_upsample_fwd(_ptr(A["d3"]), 64, _ptr(A["cat2"]), 96, nb, s3, s3, s3, 64, st), "up2")
-        self._conv("d2a", A["cat2"], 0, 96, A["d2a"], 0, 32, nb)
+        if infer:
+            self._conv_cat("d2a", A["d3"], 64, 64, e2, e2c, A["d2a"], 32, nb)
+        else:
+            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d3"]), 64, _ptr(A["cat2"]), 96, nb, s3, s3, s3, 64, st), "up2")
+            self._conv("d2a", A["cat2"], 0, 96, A["d2a"], 0, 32, nb)
         self._conv("d2", A["d2a"], 0, 32, A["d2"], 0, 32, nb)
-        _chk(L.mcx_dn_upsample_fwd(_ptr(A["d2"]), 32, _ptr(A["cat1"]), 48, nb, s2, s2, s2, 32, st), "up1")
-        self._conv("d1a", A["cat1"], 0, 48, A["d1a"], 0, 16, nb)
+        if infer:
+            self._conv_cat("d1a", A["d2"], 32, 32, e1, e1c, A["d1a"], 16, nb)
+        else:
+            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d2"]), 32, _ptr(A["cat1"]), 48, nb, s2, s2, s2, 32, st), "up1")
+            self._conv("d1a", A["cat1"], 0, 48, A["d1a"], 0, 16, nb)
         self._conv("d1", A["d1a"], 0, 16, A["d1"], 0, 16, nb)
         ho = self.head_off
         tptr = _ptr(target) if target is not None else None

@@ -73,6 +73,57 @@ def test_conv3_forward(cin, cout, shape):
     _bf_close(_host_f32(y, ref.shape), ref)
 
 
+@pytest.mark.parametrize("cu,cx,cout,shape", [(32, 16, 16, SHAPE), (64, 32, 32, SHAPE), (32, 16, 16, (1, 8, 8, 34))])
+def test_conv3_concatenating_forward_equals_materialised(cu, cx, cout, shape):
+    """conv(cat(up2(xu), x)) read on the fly equals the upsampling written
+    into a concatenation buffer followed by the plain convolution, bit for
+    bit (the same BF16 operands in the same K order)."""
+    rng = np.random.default_rng(cu + cx)
+    n, d, h, wd = shape
+    cin = cu + cx
+    xu = dn_ref.bf16_round(rng.standard_normal((n, d // 2, h // 2, wd // 2, cu)))
+    x = dn_ref.bf16_round(rng.standard_normal(shape + (cx,)))
+    w, wpad, kp = _weights(rng, cin, cout)
+    b = rng.standard_normal(cout).astype(np.float32)
+    dxu, dx, dw = _dev_bf16(xu), _dev_bf16(x), _dev_bf16(wpad)
+    db = torch.from_numpy(b).cuda()
+    nv = int(np.prod(shape))
+    cat = torch.zeros(nv * cin, dtype=torch.int16, device="cuda")
+    y1 = torch.zeros(nv * cout, dtype=torch.int16, device="cuda")
+    y2 = torch.zeros(nv * cout, dtype=torch.int16, device="cuda")
+    assert L.mcx_dn_upsample_fwd(_p(dxu), cu, _p(cat), cin, n, d // 2, h // 2, wd // 2, cu, None) == 0
+    cat.view(nv, cin)[:, cu:] = dx.view(nv, cx)
+    assert L.mcx_dn_conv3_fwd(_p(cat), cin, None, 0, _p(dw), kp, _p(db), _p(y1), cout, n, d, h, wd, cin, cout, 1, None) == 0
+    assert L.mcx_dn_conv3_fwd_cat(_p(dxu), cu, cu, _p(dx), cx, _p(dw), kp, _p(db), _p(y2), cout, n, d, h, wd, cin, cout, 1,
+                                  None) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    ref = dn_ref.conv3_fwd(np.concatenate([xu.repeat(2, 1).repeat(2, 2).repeat(2, 3), x], axis=-1), w, b, relu=True)
+    _bf_close(_host_f32(y2, ref.shape), ref)
+    # odd dimensions and channel counts that are not multiples of 16 are refused
+    assert L.mcx_dn_conv3_fwd_cat(_p(dxu), cu, cu, _p(dx), cx, _p(dw), kp, _p(db), _p(y2), cout, n, d + 1, h, wd, cin,
+                                  cout, 1, None) == -22
+    assert L.mcx_dn_conv3_fwd_cat(_p(dxu), cu, cu - 8, _p(dx), cx, _p(dw), kp, _p(db), _p(y2), cout, n, d, h, wd,
+                                  cin - 8, cout, 1, None) == -22
+
+
+def test_unet_inference_forward_equals_training_forward():
+    """The inference forward (no concatenation buffers) predicts exactly
+    what the training forward predicts."""
+    from mcx_amd import denoise
+    net = denoise.UNet(seed=11)
+    rng = np.random.default_rng(12)
+    nb, S = 2, 16
+    x = dn_ref.bf16_round(rng.standard_normal((nb, S, S, S, denoise.NFEAT)) * 0.5)
+    xd = _dev_bf16(x).view(torch.bfloat16).reshape(nb, S, S, S, denoise.NFEAT)
+    td = torch.zeros(nb * S ** 3, dtype=torch.float32, device="cuda")
+    p_train, _ = net.forward(xd, td)
+    p_train = p_train.clone()
+    p_infer, _ = net.forward(xd)
+    torch.cuda.synchronize()
+    assert torch.equal(p_train, p_infer)
+
+
 def test_conv3_forward_channel_slices():
     """input and output as channel slices of wider buffers (skip concat)"""
     rng = np.random.default_rng(5)
