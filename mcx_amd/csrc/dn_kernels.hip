@@ -952,6 +952,15 @@ extern "C" int mcx_dn_conv3_fwd(const void* x, int xcs, const void* mask, int mc
             if (cout == 16) DN_HALO(16, 1);
             else DN_HALO(16, 2);
         }
+        return dn_ret();
+    }
+    if ((cout == 48 || cout == 64) && slab < 0x7fffffffLL) {
+        /* 48 or 64 output channels: 8-channel groups keep the weights
+         * (27.6 KB at 64 channels) and the halo within 64 KB of LDS */
+        const long long nblk = (long long)((wd + 15) / 16) * ((h + 3) / 4) * ((d + 3) / 4) * nb;
+        if (nblk > 0x7fffffffLL) return -22;
+        if (cout == 48) DN_HALO(8, 3);
+        else DN_HALO(8, 4);
 #undef DN_HALO
         return dn_ret();
     }

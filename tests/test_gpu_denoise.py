@@ -56,7 +56,8 @@ SHAPE = (2, 6, 10, 12)
 
 @pytest.mark.parametrize("cin,cout,shape", [(8, 16, SHAPE), (16, 32, SHAPE), (48, 16, SHAPE), (32, 64, SHAPE),
                                             (192, 64, SHAPE), (8, 16, (1, 9, 7, 37)), (16, 16, (1, 9, 7, 37)),
-                                            (48, 32, (2, 5, 9, 33)), (96, 32, (1, 8, 8, 16))])
+                                            (48, 32, (2, 5, 9, 33)), (96, 32, (1, 8, 8, 16)), (16, 48, SHAPE),
+                                            (64, 64, (1, 9, 7, 37)), (128, 128, SHAPE)])
 def test_conv3_forward(cin, cout, shape):
     """Both kernels (the LDS halo tile for cout <= 32, the per-wave tiles
     otherwise), on shapes that are and are not multiples of the tile."""
