@@ -266,6 +266,10 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
         const bool up = c0 < cu;
         const long long org = (((long long)bn * D + (z0 - 1)) * H + (y0 - 1)) * W + (x0 - 1);
         const long long xorg = org * xcs + (c0 - cu), morg = org * mcs + c0;
+        /* coarse voxel of halo voxel h along an axis: origin/2 + floor((h-1)/2)
+         * (tile origins are even) */
+        const int D2 = D / 2, H2 = H / 2, W2 = W / 2;
+        const long long uorg = ((((long long)bn * D2 + z0 / 2) * H2 + y0 / 2) * W2 + x0 / 2) * xucs + c0;
 #pragma unroll
         for (int k = 0; k < LH; k++) {
             const int e = threadIdx.x + 256 * k;
@@ -276,8 +280,8 @@ __global__ void __launch_bounds__(256) conv3_fwd_halo(const uint16_t* __restrict
                 const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
                 if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) {
                     if (up) {
-                        const long long mu = (((long long)bn * (D / 2) + gz / 2) * (H / 2) + gy / 2) * (W / 2) + gx / 2;
-                        hv[k] = load8(xu + mu * xucs + c0 + 8 * sl);
+                        const int ru = (((hz - 1) >> 1) * H2 + ((hy - 1) >> 1)) * W2 + ((hx - 1) >> 1);
+                        hv[k] = load8(xu + (uorg + (long long)ru * xucs + 8 * sl));
                         continue;
                     }
                     const int rel = (hz * H + hy) * W + hx;
