@@ -482,6 +482,128 @@ __global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ 
 
 /* dw[co][t*cin + ci] = sum_s partial[s][t][co][ci]; padding entries of dw
  * (k >= 27*cin) are set to zero */
+/* ---- weight gradient, 16 output channels, all taps per staging --------
+ * Same partial sums as conv3_wgrad (partial[s][tap][co][ci]) for cout = 16
+ * and cin <= 16 * NCI.  The work unit is a segment of 32 voxels along x
+ * (one x-row of one z-plane).  For a segment the workgroup stages, per
+ * 16-channel input tile, the 9 neighbouring input rows (dz, dy in -1..1)
+ * transposed to voxel-contiguous order, in three copies shifted by dx =
+ * -1, 0, 1, so that the B operand of every tap is one aligned 16-byte LDS
+ * read; the masked output gradient is staged once, transposed the same
+ * way, as the A operand.  Wave w accumulates taps w, w+4, ... (7 at most)
+ * over all segments of its workgroup: one staging feeds 27 MFMAs per input
+ * tile instead of one.  Channel rows in LDS are SV = 40 voxels apart (5
+ * 16-byte units, odd), so the 16 lanes that read 16 channels hit distinct
+ * banks.  Transposition: a thread loads 8 channels of 8 consecutive voxels
+ * and writes 8 rows of 8 voxels (v_perm pairs). */
+template <int NCI>
+__global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restrict__ x, int xcs,
+                                                        const uint16_t* __restrict__ dy, int dycs,
+                                                        const uint16_t* __restrict__ mask, int mcs,
+                                                        float* __restrict__ partial, int segs, int nb, int D, int H,
+                                                        int W, int cin) {
+    constexpr int SV = 40;
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * 9 * 16 * SV]; /* [dx][row][ci][v] */
+    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * SV];         /* [co][v] */
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nxs = (W + 31) / 32;
+    const long long nseg = (long long)nb * D * H * nxs;
+    const long long sb = (long long)blockIdx.x * segs;
+    const long long se = sb + segs < nseg ? sb + segs : nseg;
+    f32x4 acc[7][NCI];
+#pragma unroll
+    for (int j = 0; j < 7; j++)
+#pragma unroll
+        for (int c = 0; c < NCI; c++) acc[j][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    /* staging items of this thread: x: (row, 8-voxel block, channel slice,
+     * dx copy) = 9 * 4 * 2 * 3 = 216; dy: (block, slice) = 4 * 2 = 8 */
+    const int it = threadIdx.x;
+    const int xrow = it / 24, xb = (it / 6) % 4, xsl = (it / 3) % 2, xdx = it % 3;
+    for (long long sg = sb; sg < se; sg++) {
+        long long q = sg;
+        const int x0 = (int)(q % nxs) * 32; q /= nxs;
+        const int yy = (int)(q % H); q /= H;
+        const int zz = (int)(q % D);
+        const int nn = (int)(q / D);
+        for (int c = 0; c < NCI; c++) {
+            __syncthreads(); /* the previous tile's reads are done */
+            if (it < 216) {
+                const int gz = zz + xrow / 3 - 1, gy = yy + xrow % 3 - 1;
+                const int ch = 16 * c + 8 * xsl;
+                Frag in[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int gx = x0 + 8 * xb + k + xdx - 1;
+                    in[k] = zero8();
+                    if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D && ch < cin)
+                        in[k] = load8(x + ((((long long)nn * D + gz) * H + gy) * W + gx) * xcs + ch);
+                }
+#pragma unroll
+                for (int ci = 0; ci < 8; ci++) {
+                    Frag o;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) o.h[k] = in[k].h[ci];
+                    *(uint4*)(xs + ((xdx * 9 + xrow) * 16 + 8 * xsl + ci) * SV + 8 * xb) = o.u;
+                }
+            }
+            if (c == 0 && it >= 216 && it < 224) {
+                const int gb = (it - 216) / 2, gsl = (it - 216) % 2;
+                Frag in[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int gx = x0 + 8 * gb + k;
+                    in[k] = zero8();
+                    if (gx < W) {
+                        const long long v = (((long long)nn * D + zz) * H + yy) * W + gx;
+                        in[k] = load8(dy + v * dycs + 8 * gsl);
+                        if (mask) {
+                            const Frag mk = load8(mask + v * mcs + 8 * gsl);
+#pragma unroll
+                            for (int i = 0; i < 8; i++)
+                                if (!(bf2f(mk.h[i]) > 0.f)) in[k].h[i] = 0;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int co = 0; co < 8; co++) {
+                    Frag o;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) o.h[k] = in[k].h[co];
+                    *(uint4*)(gs + (8 * gsl + co) * SV + 8 * gb) = o.u;
+                }
+            }
+            __syncthreads();
+            Frag a;
+            a.u = *(const uint4*)(gs + r * SV + 8 * g);
+#pragma unroll
+            for (int j = 0; j < 7; j++) {
+                const int t = wv + 4 * j;
+                if (t < 27) {
+                    const int row = (t / 9) * 3 + (t / 3) % 3, dxi = t % 3;
+                    Frag b;
+                    b.u = *(const uint4*)(xs + ((dxi * 9 + row) * 16 + r) * SV + 8 * g);
+                    acc[j][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j][c], 0, 0, 0);
+                }
+            }
+        }
+    }
+    /* lane (r, g) holds D[co = 4g + i][ci = 16c + r] of each of its taps */
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        const int t = wv + 4 * j;
+        if (t >= 27) continue;
+        float* out = partial + ((long long)blockIdx.x * 27 + t) * 16 * cin;
+#pragma unroll
+        for (int c = 0; c < NCI; c++) {
+            const int ci = 16 * c + r;
+            if (ci >= cin) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) out[(4 * g + i) * cin + ci] = acc[j][c][i];
+        }
+    }
+}
+
 __global__ void wgrad_reduce(const float* __restrict__ partial, int nsplit, float* __restrict__ dw, int kpad, int cin, int cout) {
     const long long n = (long long)cout * kpad;
     const long long per = 27LL * cout * cin;
@@ -1039,6 +1161,19 @@ extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dy
     if (shmem > 65536) return -22;
     hipStream_t st = (hipStream_t)stream;
     const uint16_t *xp = (const uint16_t*)x, *dp = (const uint16_t*)dy, *mp = (const uint16_t*)mask;
+    if (cout == 16 && (cinp == 16 || cinp == 48)) {
+        /* all 27 taps per staging of 32-voxel x-row segments */
+        const long long nseg = (long long)nb * d * h * ((wd + 31) / 32);
+        const long long segs = (nseg + nsplit - 1) / nsplit;
+        if (segs > 0x7fffffffLL) return -22;
+        if (cinp == 16)
+            hipLaunchKernelGGL(conv3_wgrad_rows<1>, dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, mcs,
+                               partial, (int)segs, nb, d, h, wd, cin);
+        else
+            hipLaunchKernelGGL(conv3_wgrad_rows<3>, dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, mcs,
+                               partial, (int)segs, nb, d, h, wd, cin);
+        return dn_ret();
+    }
     const dim3 grid(27, (unsigned int)nsplit);
 #define DN_WG(T) hipLaunchKernelGGL(conv3_wgrad<T>, grid, dim3(256), shmem, st, xp, xcs, dp, dycs, mp, mcs, partial, (int)vps, nb, d, h, wd, cin, cout)
     if (tpw <= 1) DN_WG(1);
