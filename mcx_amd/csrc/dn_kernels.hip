@@ -482,9 +482,9 @@ __global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ 
 
 /* dw[co][t*cin + ci] = sum_s partial[s][t][co][ci]; padding entries of dw
  * (k >= 27*cin) are set to zero */
-/* ---- weight gradient, 16 output channels, all taps per staging --------
- * Same partial sums as conv3_wgrad (partial[s][tap][co][ci]) for cout = 16
- * and cin <= 16 * NCI.  The work unit is a segment of 32 voxels along x
+/* ---- weight gradient, 16 or 32 output channels, all taps per staging ---
+ * Same partial sums as conv3_wgrad (partial[s][tap][co][ci]) for cout =
+ * 16 * NCO and cin <= 16 * NCI.  The work unit is a segment of 32 voxels along x
  * (one x-row of one z-plane).  For a segment the workgroup stages, per
  * 16-channel input tile, the 9 neighbouring input rows (dz, dy in -1..1)
  * transposed to voxel-contiguous order, in three copies shifted by dx =
@@ -496,7 +496,7 @@ __global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ 
  * 16-byte units, odd), so the 16 lanes that read 16 channels hit distinct
  * banks.  Transposition: a thread loads 8 channels of 8 consecutive voxels
  * and writes 8 rows of 8 voxels (v_perm pairs). */
-template <int NCI>
+template <int NCI, int NCO>
 __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restrict__ x, int xcs,
                                                         const uint16_t* __restrict__ dy, int dycs,
                                                         const uint16_t* __restrict__ mask, int mcs,
@@ -504,20 +504,22 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
                                                         int W, int cin) {
     constexpr int SV = 40;
     __shared__ __attribute__((aligned(16))) uint16_t xs[3 * 9 * 16 * SV]; /* [dx][row][ci][v] */
-    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * SV];         /* [co][v] */
+    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * NCO * SV];   /* [co][v] */
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int nxs = (W + 31) / 32;
     const long long nseg = (long long)nb * D * H * nxs;
     const long long sb = (long long)blockIdx.x * segs;
     const long long se = sb + segs < nseg ? sb + segs : nseg;
-    f32x4 acc[7][NCI];
+    f32x4 acc[7][NCI][NCO];
 #pragma unroll
     for (int j = 0; j < 7; j++)
 #pragma unroll
-        for (int c = 0; c < NCI; c++) acc[j][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < NCI; c++)
+#pragma unroll
+            for (int o = 0; o < NCO; o++) acc[j][c][o] = f32x4{0.f, 0.f, 0.f, 0.f};
     /* staging items of this thread: x: (row, 8-voxel block, channel slice,
-     * dx copy) = 9 * 4 * 2 * 3 = 216; dy: (block, slice) = 4 * 2 = 8 */
+     * dx copy) = 9 * 4 * 2 * 3 = 216; dy: (block, slice) = 4 * 2 * NCO */
     const int it = threadIdx.x;
     const int xrow = it / 24, xb = (it / 6) % 4, xsl = (it / 3) % 2, xdx = it % 3;
     for (long long sg = sb; sg < se; sg++) {
@@ -547,8 +549,8 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
                     *(uint4*)(xs + ((xdx * 9 + xrow) * 16 + 8 * xsl + ci) * SV + 8 * xb) = o.u;
                 }
             }
-            if (c == 0 && it >= 216 && it < 224) {
-                const int gb = (it - 216) / 2, gsl = (it - 216) % 2;
+            if (c == 0 && it >= 216 && it < 216 + 8 * NCO) {
+                const int gb = (it - 216) / (2 * NCO), gsl = (it - 216) % (2 * NCO);
                 Frag in[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -574,8 +576,9 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
                 }
             }
             __syncthreads();
-            Frag a;
-            a.u = *(const uint4*)(gs + r * SV + 8 * g);
+            Frag a[NCO];
+#pragma unroll
+            for (int o = 0; o < NCO; o++) a[o].u = *(const uint4*)(gs + (16 * o + r) * SV + 8 * g);
 #pragma unroll
             for (int j = 0; j < 7; j++) {
                 const int t = wv + 4 * j;
@@ -583,39 +586,56 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
                     const int row = (t / 9) * 3 + (t / 3) % 3, dxi = t % 3;
                     Frag b;
                     b.u = *(const uint4*)(xs + ((dxi * 9 + row) * 16 + r) * SV + 8 * g);
-                    acc[j][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[j][c], 0, 0, 0);
+#pragma unroll
+                    for (int o = 0; o < NCO; o++)
+                        acc[j][c][o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[o].v, b.v, acc[j][c][o], 0, 0, 0);
                 }
             }
         }
     }
-    /* lane (r, g) holds D[co = 4g + i][ci = 16c + r] of each of its taps */
+    /* lane (r, g) holds D[co = 16o + 4g + i][ci = 16c + r] of each of its taps */
 #pragma unroll
     for (int j = 0; j < 7; j++) {
         const int t = wv + 4 * j;
         if (t >= 27) continue;
-        float* out = partial + ((long long)blockIdx.x * 27 + t) * 16 * cin;
+        float* out = partial + ((long long)blockIdx.x * 27 + t) * 16 * NCO * cin;
 #pragma unroll
         for (int c = 0; c < NCI; c++) {
             const int ci = 16 * c + r;
             if (ci >= cin) continue;
 #pragma unroll
-            for (int i = 0; i < 4; i++) out[(4 * g + i) * cin + ci] = acc[j][c][i];
+            for (int o = 0; o < NCO; o++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) out[(16 * o + 4 * g + i) * cin + ci] = acc[j][c][o][i];
         }
     }
 }
 
-__global__ void wgrad_reduce(const float* __restrict__ partial, int nsplit, float* __restrict__ dw, int kpad, int cin, int cout) {
+/* dw[co][k] = sum over the nsplit partial sums; 1024 threads per 64
+ * outputs: 16 groups each add every 16th split, then one thread adds the
+ * 16 group sums in order (deterministic) */
+__global__ void __launch_bounds__(1024) wgrad_reduce(const float* __restrict__ partial, int nsplit, float* __restrict__ dw,
+                                                     int kpad, int cin, int cout) {
+    __shared__ float red[16][64];
     const long long n = (long long)cout * kpad;
     const long long per = 27LL * cout * cin;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int el = threadIdx.x & 63, sg = threadIdx.x >> 6;
+    const long long i = (long long)blockIdx.x * 64 + el;
+    float sum = 0.f;
+    if (i < n) {
         const int co = (int)(i / kpad), k = (int)(i % kpad);
-        float sum = 0.f;
         if (k < 27 * cin) {
             const int t = k / cin, ci = k - t * cin;
             const long long off = ((long long)t * cout + co) * cin + ci;
-            for (int s = 0; s < nsplit; s++) sum += partial[s * per + off];
+            for (int s = sg; s < nsplit; s += 16) sum += partial[s * per + off];
         }
-        dw[i] = sum;
+    }
+    red[sg][el] = sum;
+    __syncthreads();
+    if (sg == 0 && i < n) {
+        float tot = 0.f;
+        for (int q = 0; q < 16; q++) tot += red[q][el];
+        dw[i] = tot;
     }
 }
 
@@ -655,12 +675,23 @@ __global__ void bias_grad(const uint16_t* __restrict__ dy, int dycs, const uint1
     }
 }
 
-__global__ void sum_rows(const float* __restrict__ part, int nrow, int ncol, float* __restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
+/* out[c] = sum over the nrow rows of part[row][c]; 1024 threads per 64
+ * columns: 16 groups each add every 16th row, then one thread adds the 16
+ * group sums in order (deterministic) */
+__global__ void __launch_bounds__(1024) sum_rows(const float* __restrict__ part, int nrow, int ncol, float* __restrict__ out) {
+    __shared__ float red[16][64];
+    const int el = threadIdx.x & 63, sg = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + el;
     float t = 0.f;
-    for (int s = 0; s < nrow; s++) t += part[(long long)s * ncol + c];
-    out[c] = t;
+    if (c < ncol)
+        for (int s = sg; s < nrow; s += 16) t += part[(long long)s * ncol + c];
+    red[sg][el] = t;
+    __syncthreads();
+    if (sg == 0 && c < ncol) {
+        float tot = 0.f;
+        for (int q = 0; q < 16; q++) tot += red[q][el];
+        out[c] = tot;
+    }
 }
 
 /* ---- 2x2x2 max pooling --------------------------------------------------- */
@@ -1161,17 +1192,23 @@ extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dy
     if (shmem > 65536) return -22;
     hipStream_t st = (hipStream_t)stream;
     const uint16_t *xp = (const uint16_t*)x, *dp = (const uint16_t*)dy, *mp = (const uint16_t*)mask;
-    if (cout == 16 && (cinp == 16 || cinp == 48)) {
+    if ((cout == 16 && (cinp == 16 || cinp == 48)) || (cout == 32 && (cinp == 16 || cinp == 32))) {
         /* all 27 taps per staging of 32-voxel x-row segments */
         const long long nseg = (long long)nb * d * h * ((wd + 31) / 32);
         const long long segs = (nseg + nsplit - 1) / nsplit;
         if (segs > 0x7fffffffLL) return -22;
-        if (cinp == 16)
-            hipLaunchKernelGGL(conv3_wgrad_rows<1>, dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, mcs,
-                               partial, (int)segs, nb, d, h, wd, cin);
+#define DN_WGR(NCI, NCO)                                                                                                 \
+    hipLaunchKernelGGL((conv3_wgrad_rows<NCI, NCO>), dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, \
+                       mcs, partial, (int)segs, nb, d, h, wd, cin)
+        if (cout == 16 && cinp == 16)
+            DN_WGR(1, 1);
+        else if (cout == 16)
+            DN_WGR(3, 1);
+        else if (cinp == 16)
+            DN_WGR(1, 2);
         else
-            hipLaunchKernelGGL(conv3_wgrad_rows<3>, dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, mcs,
-                               partial, (int)segs, nb, d, h, wd, cin);
+            DN_WGR(2, 2);
+#undef DN_WGR
         return dn_ret();
     }
     const dim3 grid(27, (unsigned int)nsplit);
@@ -1189,8 +1226,9 @@ extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dy
 
 extern "C" int mcx_dn_wgrad_reduce(const float* partial, int nsplit, float* dw, int kpad, int cin, int cout, void* stream) {
     if (!partial || !dw || nsplit <= 0 || kpad < 27 * cin || cin <= 0 || cout <= 0) return -22;
-    hipLaunchKernelGGL(wgrad_reduce, dim3(dn_grid((long long)cout * kpad)), dim3(256), 0, (hipStream_t)stream, partial, nsplit, dw,
-                       kpad, cin, cout);
+    const long long nel = (long long)cout * kpad;
+    hipLaunchKernelGGL(wgrad_reduce, dim3((unsigned int)((nel + 63) / 64)), dim3(1024), 0, (hipStream_t)stream, partial, nsplit,
+                       dw, kpad, cin, cout);
     return dn_ret();
 }
 
@@ -1204,7 +1242,7 @@ extern "C" int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int 
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(bias_grad, dim3(nsplit), dim3(256), 0, st, (const uint16_t*)dy, dycs, (const uint16_t*)mask, mcs, part, nvox,
                        vps, cout);
-    hipLaunchKernelGGL(sum_rows, dim3((cout + 255) / 256), dim3(256), 0, st, (const float*)part, nsplit, cout, db);
+    hipLaunchKernelGGL(sum_rows, dim3((cout + 63) / 64), dim3(1024), 0, st, (const float*)part, nsplit, cout, db);
     return dn_ret();
 }
 
@@ -1298,7 +1336,7 @@ extern "C" int mcx_dn_head_bwd(const void* f, int fcs, const float* w, const flo
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(head_bwd, dim3(nblocks), dim3(256), 0, st, (const uint16_t*)f, fcs, w, dpred, (uint16_t*)df, dfcs, part,
                        nvox, c);
-    hipLaunchKernelGGL(sum_rows, dim3(1), dim3(256), 0, st, (const float*)part, nblocks, c + 1, dwb);
+    hipLaunchKernelGGL(sum_rows, dim3((c + 1 + 63) / 64), dim3(1024), 0, st, (const float*)part, nblocks, c + 1, dwb);
     return dn_ret();
 }
 

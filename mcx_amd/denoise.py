@@ -224,13 +224,23 @@ class UNet:
         maxpart = 1
         for name, cin, cout in CONVS:
             nv = self._nvox_of(name, A)
-            maxpart = max(maxpart, self._nsplit(nv) * 27 * cout * cin, self._nsplit(nv) * cout)
+            ns = self._nsplit(nv, cin, cout)
+            maxpart = max(maxpart, ns * 27 * cout * cin, ns * cout)
         A["wpart"] = torch.zeros(maxpart, dtype=torch.float32, device=self.device)
         self._acts = A
         return A
 
     @staticmethod
-    def _nsplit(nvox):
+    def _nsplit(nvox, cin=None, cout=None):
+        """Voxel-range splits of a weight gradient (partial sums reduced by
+        a second kernel).  The 16/32-output-channel layers whose gradient
+        kernel works on 32-voxel row segments, one workgroup per split, get
+        about 8 segments per split (at most 2048); the others, one row of 27
+        workgroups per split, one split per 4096 voxels (at most 1024)."""
+        if cin is not None:
+            cinp = (cin + 15) // 16 * 16
+            if (cout == 16 and cinp in (16, 48)) or (cout == 32 and cinp in (16, 32)):
+                return int(min(2048, max(1, nvox // 256)))
         return int(min(1024, max(1, nvox // 4096)))
 
     def _nvox_of(self, name, A):
@@ -318,7 +328,7 @@ class UNet:
         wo, bo, cin, cout, kp = self.layout[name]
         S = self._dims_of(name, A)
         nv = self._nvox_of(name, A)
-        ns = self._nsplit(nv)
+        ns = self._nsplit(nv, cin, cout)
         _chk(L.mcx_dn_conv3_wgrad(_ptr(x, xoff), xcs, _ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(A["wpart"]), ns, nb,
                                   S, S, S, cin, cout, st), "wgrad " + name)
         _chk(L.mcx_dn_wgrad_reduce(_ptr(A["wpart"]), ns, _ptr(self.g, wo), kp, cin, cout, st), "wgrad_reduce " + name)

@@ -160,7 +160,8 @@ def test_conv3_input_gradient(cin, cout):
     _bf_close(_host_f32(gx, ref.shape), ref)
 
 
-@pytest.mark.parametrize("cin,cout,nsplit", [(8, 16, 3), (48, 16, 1), (32, 64, 5), (192, 64, 2), (128, 128, 4)])
+@pytest.mark.parametrize("cin,cout,nsplit", [(8, 16, 3), (16, 16, 7), (48, 16, 1), (16, 32, 2), (32, 32, 3), (32, 64, 5),
+                                             (192, 64, 2), (128, 128, 4)])
 def test_conv3_weight_and_bias_gradient(cin, cout, nsplit):
     rng = np.random.default_rng(cin + cout)
     x = dn_ref.bf16_round(rng.standard_normal(SHAPE + (cin,)))
