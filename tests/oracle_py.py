@@ -251,7 +251,10 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     P.outputtype = outputtype
     P.save2pt = 1
     P.bc[:] = list(bcv)
-    P.is2d = 0
+    # a domain with exactly one axis of extent 1 is simulated in 2-D
+    # (src/mcx_core.cu:2850, :2874: is2d = the singular axis, 1-based)
+    is2d = 1 if nx == 1 else (2 if ny == 1 else (3 if nz == 1 else 0))
+    P.is2d = is2d if (nx > 1) + (ny > 1) + (nz > 1) == 2 else 0
     P.savedet = issavedet
     P.savedetflag = savedetflag
     P.maxdetphoton = maxdetphoton
