@@ -314,6 +314,92 @@ def head_eval(net, args, minenergy, log):
            "denoised_photons_for_5pct": dn5, "denoised_ms_to_5pct": dms5,
            "ratio_denoised_over_plain": dms5 / ms5}
     log("head: %s" % json.dumps(out))
+    if args.batch_k > 0:
+        out["batched"] = head_batch_eval(dn, args, hc, ref, mask, ms5, log)
+    return out
+
+
+def _time_to_5pct(photons, errors, ms, sqrt_law):
+    """photons and time per volume at 5 % median error: errors fitted as
+    c/sqrt(N) (plain) or by a log-log line (denoised), time interpolated in
+    log-log between the measured counts (scaled linearly beyond them)"""
+    ph, er, t = np.asarray(photons, float), np.asarray(errors, float), np.asarray(ms, float)
+    if sqrt_law:
+        c = float(np.exp(np.mean(np.log(er) + 0.5 * np.log(ph))))
+        n5 = (c / 0.05) ** 2
+    else:
+        slope, icpt = np.polyfit(np.log(ph), np.log(er), 1)
+        n5 = float(np.exp((np.log(0.05) - icpt) / slope)) if slope < 0 else float("inf")
+    if not np.isfinite(n5):
+        return n5, float("inf")
+    o = np.argsort(ph)
+    if n5 > ph[o][-1]:
+        return n5, float(t[o][-1] * n5 / ph[o][-1])
+    return n5, float(np.exp(np.interp(np.log(n5), np.log(ph[o]), np.log(t[o]))))
+
+
+def head_batch_eval(dn, args, hc, ref, mask, single_plain_ms5, log):
+    """The batched form on the head volume: K runs with independent seeds in
+    one persistent launch of the batched kernel (mcx_run_batch), each then
+    denoised, against K plain runs in one launch; time per volume = launch
+    time / K (+ the volume's U-Net inference).  A lone low-photon launch
+    lasts as long as its longest photon history; K runs in one launch share
+    that tail."""
+    import torch
+    import mcx_amd
+    K = args.batch_k
+
+    def launch(nph, salt):
+        cfgs = []
+        for i in range(K):
+            c = dict(hc)
+            c["nphoton"] = int(nph)
+            c["seed"] = 5000 + 97 * salt + i
+            c["quiet"] = True
+            cfgs.append(c)
+        t0 = time.perf_counter()
+        _, info = mcx_amd.run_batch(cfgs, tally=1, download=False)  # autotunes the shape (cached per batch)
+        res, info = mcx_amd.run_batch(cfgs, tally=1, photons_per_lane=info["photons_per_lane"],
+                                      block_size=info["block_size"], tally_cache=info["tally_cache_cells"] or -1)
+        maps = cw_fluence(res, cfgs)
+        log("head batched: %d x %.3g photons in %.1f ms (%.0f s with tuning and download)" %
+            (K, nph, info["total_ms"], time.perf_counter() - t0))
+        return maps, float(info["total_ms"])
+
+    plain_rows = []
+    for k, v in enumerate(args.batch_plain_scale.split(",")):
+        nph = float(v) * args.head_low
+        maps, ms = launch(nph, 1 + k)
+        errs = [median_error(m, ref, mask) for m in maps]
+        plain_rows.append({"photons": nph, "ms_per_volume": ms / K, "error": float(np.median(errs))})
+    den_rows = []
+    for k, v in enumerate(args.batch_dn_scale.split(",")):
+        nph = float(v) * args.head_low
+        maps, ms = launch(nph, 50 + k)
+        errs_low, errs_dn, dn_ms = [], [], []
+        for i, m in enumerate(maps):
+            dphi = dn.upload(m)
+            pf = nph if args.photon_feature else None
+            if i == 0:
+                dn.run(dphi, pf)  # warm-up (buffer allocation)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dout = dn.run(dphi, pf)
+            torch.cuda.synchronize()
+            dn_ms.append((time.perf_counter() - t0) * 1e3)
+            errs_low.append(median_error(m, ref, mask))
+            errs_dn.append(median_error(dn.download(dout), ref, mask))
+        den_rows.append({"photons": nph, "ms_per_volume": ms / K, "denoise_ms": float(np.mean(dn_ms)),
+                         "low_error": float(np.median(errs_low)), "denoised_error": float(np.median(errs_dn))})
+    pn5, pms5 = _time_to_5pct([r["photons"] for r in plain_rows], [r["error"] for r in plain_rows],
+                              [r["ms_per_volume"] for r in plain_rows], True)
+    dn5, dms5 = _time_to_5pct([r["photons"] for r in den_rows], [r["denoised_error"] for r in den_rows],
+                              [r["ms_per_volume"] + r["denoise_ms"] for r in den_rows], False)
+    out = {"k": K, "plain": plain_rows, "denoised": den_rows, "plain_photons_for_5pct": pn5,
+           "plain_ms_per_volume_to_5pct": pms5, "denoised_photons_for_5pct": dn5,
+           "denoised_ms_per_volume_to_5pct": dms5, "ratio_denoised_over_plain": dms5 / pms5,
+           "ratio_over_single_plain": dms5 / single_plain_ms5}
+    log("head batched: %s" % json.dumps(out))
     return out
 
 
@@ -347,6 +433,11 @@ def main(argv=None):
     ap.add_argument("--head-dn-scale", default="1,2,4", help="denoised head runs: photons as multiples of --head-low")
     ap.add_argument("--head-plain-scale", default="1,2,4,10",
                     help="plain head runs (besides the references): photons as multiples of --head-low")
+    ap.add_argument("--batch-k", type=int, default=8, help="batched head evaluation: runs per launch (0: skip)")
+    ap.add_argument("--batch-dn-scale", default="0.25,0.5,1",
+                    help="batched denoised runs: photons per run as multiples of --head-low")
+    ap.add_argument("--batch-plain-scale", default="2,4",
+                    help="batched plain runs: photons per run as multiples of --head-low")
     args = ap.parse_args(argv)
     import torch
 
