@@ -322,3 +322,21 @@ def test_denoise_experiment_helpers():
     from mcx_amd import workloads
     h = workloads.head256(n=64)
     assert set(np.unique(h["vol"])) <= {0, 1, 2, 3, 4, 5} and h["unitinmm"] == 0.5
+
+
+def test_time_to_5pct_fits():
+    """The time-to-5% helper of the denoiser experiment: an exact c/sqrt(N)
+    error law gives N5 = (c/0.05)^2, a power law exp(a) N^b gives N5 from
+    the line in log-log, and the time is interpolated between the measured
+    counts in log-log (linearly in photons beyond them)."""
+    from mcx_amd import dn_experiment as dx
+    ph = np.array([1e6, 4e6])
+    n5, t5 = dx._time_to_5pct(ph, 0.1 * np.sqrt(1e6 / ph), [10.0, 40.0], True)
+    assert n5 == pytest.approx(4e6, rel=1e-9)
+    assert t5 == pytest.approx(40.0, rel=1e-9)
+    n5, t5 = dx._time_to_5pct(ph, 0.2 * (ph / 1e6) ** -0.5, [10.0, 40.0], False)
+    assert n5 == pytest.approx(1.6e7, rel=1e-6)
+    assert t5 == pytest.approx(40.0 * 4, rel=1e-6)
+    n5, t5 = dx._time_to_5pct(ph, [0.1, 0.025], [10.0, 40.0], False)
+    assert n5 == pytest.approx(2e6, rel=1e-6)
+    assert t5 == pytest.approx(20.0, rel=1e-6)
