@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # the FP64 tally and the LDS hot-cell box: (FETCH_SIZE x 2.000 + WRITE_SIZE x
 # 1.000) per launch / 1e7 photons, scale factors from the FP64->FP32 merge
 # kernel as a copy of known size (profiles/transport_sphere100_1e7_counters.md)
-TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 8403.0}
+TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 8254.0}
 ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 

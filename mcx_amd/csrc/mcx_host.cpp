@@ -161,8 +161,9 @@ size_t lds_share(int device, unsigned int groups, size_t cap) {
 }
 
 /* The hot-cell box of a single-simulation launch: as large as the LDS a
- * workgroup can take at the kernel's full occupancy allows (at most 64 KB
- * in all).  Returns the workgroup's total dynamic LDS bytes. */
+ * workgroup can take at the kernel's full occupancy allows (at most the
+ * device's per-workgroup LDS in all).  Returns the workgroup's total
+ * dynamic LDS bytes. */
 static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int numregs, size_t shmem, KParam& kp) {
     kp.hot_cells = 0;
     if (!g_hotbox || tally == MCX_TALLY_F32_SHADOW || kp.maxgate == 0 || kp.nvox == 0) return shmem;
@@ -170,7 +171,7 @@ static size_t set_hot_box(const Config* cfg, int tally, unsigned int block, int 
     if (hipGetDevice(&dev) != hipSuccess) return shmem;
     const unsigned int waves_cu = 4u * (unsigned int)waves_per_simd(numregs);
     const unsigned int wgs = std::max(1u, waves_cu / std::max(1u, block / 64u));
-    const size_t per = lds_share(dev, wgs, 65536);
+    const size_t per = lds_share(dev, wgs, SIZE_MAX);
     const size_t off = (shmem + 7) & ~(size_t)7;
     if (per <= off + 64 * sizeof(double)) return shmem;
     hot_box_geometry(cfg, (uint32_t)((per - off) / sizeof(double)), kp);
@@ -434,10 +435,20 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.replaytof = drtof.as<float>();
     kp.replaydetid = drdetid.as<int32_t>();
 
-    const unsigned int block = (unsigned int)G.autoblock;
-    const unsigned int grid = nthread / block;
     /* per lane in LDS: the partial paths, then (seed saving) the launch RNG state */
-    const size_t shmem0 = (size_t)ntable * 16 + (size_t)block * (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
+    const size_t lane_lds = (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
+    unsigned int block = (unsigned int)G.autoblock;
+    /* autopilot with the hot-cell box: one workgroup per CU holding all of
+     * the CU's waves, so the box takes the CU's whole LDS instead of an
+     * equal share of it between four workgroups */
+    if (cfg->autopilot && g_hotbox && tally != MCX_TALLY_F32_SHADOW && maxgate > 0) {
+        const unsigned int whole = 64u * 4u * (unsigned int)waves_per_simd(numregs);
+        if (whole > block && whole <= (unsigned int)maxthreads && nthread % whole == 0 &&
+            (size_t)ntable * 16 + (size_t)whole * lane_lds <= 16384)
+            block = whole;
+    }
+    const unsigned int grid = nthread / block;
+    const size_t shmem0 = (size_t)ntable * 16 + (size_t)block * lane_lds;
     if (shmem0 > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
     const size_t shmem = set_hot_box(cfg, tally, block, numregs, shmem0, kp);
 

@@ -1320,8 +1320,11 @@ __device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, ui
 #ifndef MCX_MIN_WAVES_PER_SIMD
 #define MCX_MIN_WAVES_PER_SIMD 1
 #endif
+/* Up to 1024 lanes a workgroup: with the hot-cell box the host may launch
+ * one workgroup per CU holding all of its waves, so that the box takes the
+ * CU's whole LDS. */
 template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
-__global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD) mcx_transport(const KParam* __restrict__ kpp) {
+__global__ void __launch_bounds__(1024, MCX_MIN_WAVES_PER_SIMD) mcx_transport(const KParam* __restrict__ kpp) {
     const KParam& kp = *kpp;
     extern __shared__ F4 lds[];
     for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
@@ -1553,6 +1556,10 @@ extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int
         fn = isreflect ? pick_tally<1, 1>(tally, media32) : pick_tally<1, 0>(tally, media32);
     else
         fn = isreflect ? pick_tally<0, 1>(tally, media32) : pick_tally<0, 0>(tally, media32);
+    if (shmem > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), shmem, stream, dkp);
     return hipGetLastError();
 }
