@@ -1358,8 +1358,12 @@ __global__ void __launch_bounds__(1024, MCX_MIN_WAVES_PER_SIMD) mcx_transport(co
  * chunk_start[k] the first chunk of the k-th instance in that order.  A
  * virtual thread's work is identical to the single-instance kernel's, so a
  * batched instance reproduces its plain run lane for lane. */
+/* Held to 128 VGPRs (4 waves per SIMD): the hi/lo variant otherwise takes
+ * 141 and runs 3 waves per SIMD; the 24 B per lane it then spills cost
+ * less than the fourth wave gains (batch256 at quarter scale: 5281 against
+ * 5941 ms per graph). */
 template <int TALLY>
-__global__ void __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD)
+__global__ void __attribute__((amdgpu_waves_per_eu(4))) __launch_bounds__(256, MCX_MIN_WAVES_PER_SIMD)
 mcx_transport_batch(const KParam* __restrict__ params, const uint32_t* __restrict__ chunk_start,
                     const uint32_t* __restrict__ order, uint32_t ninst, uint32_t total_chunks, uint32_t maxtab,
                     uint32_t boxcells, uint32_t* __restrict__ queue) {
