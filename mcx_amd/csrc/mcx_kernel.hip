@@ -1320,11 +1320,26 @@ __device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, ui
 #ifndef MCX_MIN_WAVES_PER_SIMD
 #define MCX_MIN_WAVES_PER_SIMD 1
 #endif
-/* Up to 1024 lanes a workgroup: with the hot-cell box the host may launch
- * one workgroup per CU holding all of its waves, so that the box takes the
- * CU's whole LDS. */
+/* Label media: up to 1024 lanes a workgroup, so that with the hot-cell box
+ * the host may launch one workgroup per CU holding all of its waves and the
+ * box takes the CU's whole LDS (these variants fit 128 VGPRs without
+ * spilling).  The property-word and split-voxel variants need 126-139 VGPRs
+ * and keep 256-lane workgroups rather than spill to fit 4 waves per SIMD. */
+template <typename MT>
+struct TransportLanes {
+    static constexpr int value = 1024;
+};
+template <>
+struct TransportLanes<mcxk::MediaWord> {
+    static constexpr int value = 256;
+};
+template <>
+struct TransportLanes<mcxk::MediaSplit> {
+    static constexpr int value = 256;
+};
 template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
-__global__ void __launch_bounds__(1024, MCX_MIN_WAVES_PER_SIMD) mcx_transport(const KParam* __restrict__ kpp) {
+__global__ void __launch_bounds__(TransportLanes<MT>::value, MCX_MIN_WAVES_PER_SIMD)
+    mcx_transport(const KParam* __restrict__ kpp) {
     const KParam& kp = *kpp;
     extern __shared__ F4 lds[];
     for (uint32_t i = threadIdx.x; i < kp.ntable; i += blockDim.x) {
