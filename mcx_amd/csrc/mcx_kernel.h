@@ -76,7 +76,23 @@ struct KParam {
     uint32_t hot_cells, hot_off, hot_g;
     uint32_t hot_x0, hot_y0, hot_z0, hot_bx, hot_by, hot_bz;
     uint32_t hot_lo, hot_span;    /* linear voxel index of the box's first voxel, last minus first */
+    /* division of a voxel index by dimxy and by nx (DivMagic) */
+    uint32_t hot_mxy, hot_shxy, hot_mx, hot_shx;
 };
+
+/* Unsigned 32-bit division by an invariant d >= 1 as a multiply-high, two
+ * shifts and two adds: Granlund and Montgomery, "Division by Invariant
+ * Integers using Multiplication", PLDI 1994, Figure 4.1.  With
+ * l = ceil(log2 d): m = floor(2^32 (2^l - d) / d) + 1, sh1 = min(l, 1),
+ * sh2 = max(l - 1, 0); then for every 32-bit n, with t = mulhi(m, n),
+ * n / d = (t + ((n - t) >> sh1)) >> sh2.  The shifts are packed as
+ * sh1 | sh2 << 8. */
+static inline void div_magic(uint32_t d, uint32_t* m, uint32_t* sh) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d) l++;
+    *m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    *sh = (l < 1 ? l : 1u) | ((l > 0 ? l - 1 : 0u) << 8);
+}
 
 /* One instance of a batched launch (its launch constants are in KParam) */
 struct BatchInst {

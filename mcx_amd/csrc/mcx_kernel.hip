@@ -901,7 +901,10 @@ struct Lane {
     __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight) {
         ndep++;
         if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
-            const uint32_t z = vox / kp.dimxy, r = vox - z * kp.dimxy, y = r / kp.nx;
+            const uint32_t tz = __umulhi(vox, kp.hot_mxy);
+            const uint32_t z = (tz + ((vox - tz) >> (kp.hot_shxy & 0xffu))) >> (kp.hot_shxy >> 8), r = vox - z * kp.dimxy;
+            const uint32_t ty = __umulhi(r, kp.hot_mx);
+            const uint32_t y = (ty + ((r - ty) >> (kp.hot_shx & 0xffu))) >> (kp.hot_shx >> 8);
             const uint32_t lx = r - y * kp.nx - kp.hot_x0, ly = y - kp.hot_y0, lz = z - kp.hot_z0;
             if (lx < kp.hot_bx && ly < kp.hot_by && lz < kp.hot_bz) {
                 extern __shared__ F4 lds[];
