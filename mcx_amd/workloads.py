@@ -67,6 +67,53 @@ def sphere100(nphoton=100_000_000, seed=12345, **kw):
     return cfg
 
 
+# Published absorbed fractions of the reference's three speed-contest
+# problems at 1e8 photons (speedcontest/mcxcontest:103-105;
+# mcxlab/examples/mcx_gpu_benchmarks.m:65, :87, :112 with seed 29012392).
+# cubesph60b is 0.2701 in the mcxlab script and 0.2702 in the contest script.
+CONTEST_ABSORBED = {"cube60": 0.1769, "cubesph60b": 0.2701, "cube60planar": 0.2551}
+
+
+def contest(name, nphoton=100_000_000, seed=29012392, **kw):
+    """The reference's speed-contest problems as the mcxlab benchmark script
+    defines them (mcxlab/examples/mcx_gpu_benchmarks.m:46-112, same as the
+    CLI bench JSON of src/mcx_bench.c apart from the seed): a 60^3 cube of
+    label 1 (0.005/mm, 1.0/mm, g 0.01, n 1.37), a third medium (0.002/mm,
+    5.0/mm, g 0.9, n 1.0), four r=1 detectors on z=0, one 0-5 ns gate.
+      cube60:       pencil at (29,29,0) along +z, no reflection;
+      cubesph60b:   plus a sphere of label 2 centred at 30 with R=15 (index
+                    1.0 inside 1.37: an internal Fresnel interface and
+                    g=0.9), reflection on;
+      cube60planar: no sphere, a 40x40 planar source at (10,10,-10) in the
+                    void above the cube, reflection on.
+    """
+    cfg = {
+        "nphoton": int(nphoton),
+        "vol": np.ones((60, 60, 60), dtype=np.uint8),
+        "prop": np.array([[0, 0, 1, 1], [0.005, 1.0, 0.01, 1.37], [0.002, 5.0, 0.9, 1.0]], dtype=np.float32),
+        "srcpos": [29, 29, 0],
+        "srcdir": [0, 0, 1],
+        "issrcfrom0": 1,
+        "tstart": 0.0,
+        "tend": 5e-9,
+        "tstep": 5e-9,
+        "seed": seed,
+        "detpos": np.array([[29, 19, 0, 1], [29, 39, 0, 1], [19, 29, 0, 1], [39, 29, 0, 1]], dtype=np.float32),
+        "issavedet": 1,
+        "isreflect": 0,
+    }
+    if name == "cubesph60b":
+        cfg["vol"] = sphere_volume(60, 15.0)
+        cfg["isreflect"] = 1
+    elif name == "cube60planar":
+        cfg.update(srctype="planar", srcparam1=[40, 0, 0, 0], srcparam2=[0, 40, 0, 0], srcpos=[10, 10, -10],
+                   isreflect=1)
+    elif name != "cube60":
+        raise ValueError("unknown contest problem %r" % name)
+    cfg.update(kw)
+    return cfg
+
+
 def batch_instance(master_seed, index, edge_range=(40, 160), photon_range=(1e5, 1e7), ngates=10, tstep=5e-10):
     """One instance of the heterogeneous batch workload.
 

@@ -31,7 +31,10 @@
 #include "mcx_oracle.h"
 #include "oracle_math.h"
 
+#include <algorithm>
 #include <atomic>
+#include <climits>
+#include <utility>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -54,6 +57,11 @@ const float ROULETTE = 10.f;
 constexpr float JUST_BELOW_ONE = 0.9998f;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
 enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
+
+/* Planted mistakes for the tests that prove the pins catch them (0 = the
+ * faithful restatement; the ids are listed in mcx_oracle.h).  Set only
+ * between runs, never during one. */
+int g_mut = 0;
 
 /* ---- glibc rand(): TYPE_3 additive feedback, r[i] = r[i-3] + r[i-31] --- */
 struct GlibcRand {
@@ -127,9 +135,10 @@ struct Ctx {
 };
 
 inline float hitgrid(const Vec4& p, const Vec4& v, const float* rv, int* id) {
-    float h0 = fabsf(((float)(id[0] + (v.x > 0.f)) - p.x) * rv[0]);
-    float h1 = fabsf(((float)(id[1] + (v.y > 0.f)) - p.y) * rv[1]);
-    float h2 = fabsf(((float)(id[2] + (v.z > 0.f)) - p.z) * rv[2]);
+    const int m = (g_mut == 4);
+    float h0 = fabsf(((float)(id[0] + (m || v.x > 0.f)) - p.x) * rv[0]);
+    float h1 = fabsf(((float)(id[1] + (m || v.y > 0.f)) - p.y) * rv[1]);
+    float h2 = fabsf(((float)(id[2] + (m || v.z > 0.f)) - p.z) * rv[2]);
     float dist = fminf(fminf(h0, h1), h2);
     id[3] = (dist == h0 ? 0 : (dist == h1 ? 1 : 2));
     return dist;
@@ -172,7 +181,7 @@ inline float comp(const Vec4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : 
 inline float sgnf(float x) { return (float)((x > 0.f) - (x < 0.f)); }
 
 inline void transmit(Vec4& v, float n1, float n2, int flipdir) {
-    float tmp0 = n1 / n2;
+    float tmp0 = (g_mut == 5) ? n2 / n1 : n1 / n2;
     v.x *= tmp0; v.y *= tmp0; v.z *= tmp0;
     if (flipdir == 0) {
         tmp0 = v.y * v.y + v.z * v.z;
@@ -189,6 +198,7 @@ inline void transmit(Vec4& v, float n1, float n2, int flipdir) {
 }
 
 inline float reflectcoeff(const Vec4& v, float n1, float n2, int flipdir) {
+    if (g_mut == 6) std::swap(n1, n2);
     float Icos = fabsf(comp(v, flipdir));
     float tmp0 = n1 * n1;
     float tmp1 = n2 * n2;
@@ -203,6 +213,50 @@ inline float reflectcoeff(const Vec4& v, float n1, float n2, int flipdir) {
         return Rtotal;
     }
     return 1.f;
+}
+
+/* src/mcx_core.cu:1949-1957: Henyey-Greenstein inversion of u, clamped */
+inline float hg_costheta(float g, float u) {
+    float tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * u);
+    if (g_mut != 2) tmp0 *= tmp0;
+    tmp0 = (1.f + g * g - tmp0) / (2.f * g);
+    if (g_mut == 1) tmp0 = -tmp0;
+    return fmaxf(-1.f, fminf(1.f, tmp0));
+}
+
+/* src/mcx_core.cu:1946-1964: scattering angle cosine from one uniform u:
+ * HG with g when the selector gsel (g, or 0 after gscatter events) is
+ * beyond EPS, isotropic otherwise */
+inline float scatter_costheta(float gsel, float g, float u) {
+    return (fabsf(gsel) > EPSF) ? hg_costheta(g, u) : 2.f * u - 1.f;
+}
+
+/* src/mcx_core.cu:2418-2432: unpolarised Fresnel reflectance at a voxel
+ * face for |cos| of the incidence angle cphi; 1 beyond the critical angle.
+ * len = cos^2 of the refraction angle, (ctheta -/+ stheta) the s and then
+ * p amplitudes' numerator/denominator. */
+inline float fresnel_face(float n1, float n2, float cphi) {
+    const float tmp0 = n1 * n1;
+    const float tmp1 = n2 * n2;
+    const float sphi = 1.f - cphi * cphi;
+    const float len = 1.f - tmp0 / tmp1 * sphi;
+    if (!(len > 0.f)) return 1.f;
+    float ctheta = tmp0 * cphi * cphi + tmp1 * len;
+    const float stheta = 2.f * n1 * n2 * cphi * sqrtf(len);
+    float Rtotal = (ctheta - stheta) / (ctheta + stheta);
+    if (g_mut != 3) ctheta = tmp1 * cphi * cphi + tmp0 * len;
+    Rtotal = (Rtotal + (ctheta - stheta) / (ctheta + stheta)) * 0.5f;
+    return Rtotal;
+}
+
+/* src/mcx_core.cu:2127-2132: move by len and, unless the step ends inside
+ * the voxel (scattering site or interface plane), enter the neighbour
+ * across face fd[3] */
+inline void advance(Vec4& p, const Vec4& v, float len, bool stay, int* fd) {
+    p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
+    if (fd[3] == 0) fd[0] += stay ? 0 : (v.x > 0.f ? 1 : -1);
+    if (fd[3] == 1) fd[1] += stay ? 0 : (v.y > 0.f ? 1 : -1);
+    if (fd[3] == 2) fd[2] += stay ? 0 : (v.z > 0.f ? 1 : -1);
 }
 
 inline float nextafter_mcx(float a, int dir, int maxvoidstep) {
@@ -568,9 +622,10 @@ struct Thread {
                             p.x = p.x + rx * s1[0];
                             p.y = p.y + ry * s1[1];
                             p.z = p.z + rz * s1[2];
+                            const float ix = (g_mut == 17) ? ry : rx, iy = (g_mut == 17) ? rx : ry;
                             p.w = Q.srcpos[3] * Q.srcpattern[(int)(rz * JUST_BELOW_ONE * s1[2]) * (int)s1[1] * (int)s1[0] +
-                                                             (int)(ry * JUST_BELOW_ONE * s1[1]) * (int)s1[0] +
-                                                             (int)(rx * JUST_BELOW_ONE * s1[0])];
+                                                             (int)(iy * JUST_BELOW_ONE * s1[1]) * (int)s1[0] +
+                                                             (int)(ix * JUST_BELOW_ONE * s1[0])];
                         } else {
                             p.x = p.x + rx * s1[0] + ry * s2[0];
                             p.y = p.y + rx * s1[1] + ry * s2[1];
@@ -603,13 +658,15 @@ struct Thread {
                         float v2x = v2w * (Q.srcdir[1] * s1[2] - Q.srcdir[2] * s1[1]);
                         float v2y = v2w * (Q.srcdir[2] * s1[0] - Q.srcdir[0] * s1[2]);
                         float v2z = v2w * (Q.srcdir[0] * s1[1] - Q.srcdir[1] * s1[0]);
+                        if (g_mut == 11) { v2x = -v2x; v2y = -v2y; v2z = -v2z; }
                         p.x = p.x + rx * s1[0] + ry * v2x;
                         p.y = p.y + rx * s1[1] + ry * v2y;
                         p.z = p.z + rx * s1[2] + ry * v2z;
                         if (Q.srctype == 10) {
                             float sa, ca, sb, cb;
-                            om_sincos2pi(s2[0] * rx + s2[2], &sa, &ca);
-                            om_sincos2pi(s2[1] * ry + s2[3], &sb, &cb);
+                            const float phx = (g_mut == 12) ? s2[3] : s2[2], phy = (g_mut == 12) ? s2[2] : s2[3];
+                            om_sincos2pi(s2[0] * rx + phx, &sa, &ca);
+                            om_sincos2pi(s2[1] * ry + phy, &sb, &cb);
                             p.w = Q.srcpos[3] * (sa * sb + 1.f) * 0.5f;
                         } else {
                             float sa, ca;
@@ -625,7 +682,7 @@ struct Thread {
                     case 11: { /* zgaussian */
                         float sphi, cphi, stheta, ctheta;
                         om_sincos2pi(t.next(), &sphi, &cphi);
-                        float ang = sqrtf(2.f * t.scatlen()) * (1.f - 2.f * t.next()) * s1[0];
+                        float ang = sqrtf((g_mut == 13 ? 1.f : 2.f) * t.scatlen()) * (1.f - 2.f * t.next()) * s1[0];
                         om_sincos2pi(ang * 0.159154943091895336f, &stheta, &ctheta);
                         rotatevector(v, stheta, ctheta, sphi, cphi);
                         canfocus = 0;
@@ -635,7 +692,7 @@ struct Thread {
                         float sphi, cphi;
                         om_sincos2pi(t.next(), &sphi, &cphi);
                         float r = sqrtf(0.5f * t.scatlen()) * s1[0];
-                        float ax = -s1[1] / s1[2];
+                        float ax = (g_mut == 16) ? s1[1] / s1[2] : -s1[1] / s1[2];
                         float ay = 1.f / sqrtf(r * r + s1[2] * s1[2]);
                         float px = r * (cphi - ax * sphi), py = r * (sphi + ax * cphi), pz = 0.f;
                         float dx = -r * sphi * ay, dy = r * cphi * ay, dz = s1[2] * ay;
@@ -676,7 +733,7 @@ struct Thread {
                                 v.x = v.x - r * ax; v.y = v.y - r * ay; v.z = v.z - r * az;
                                 r = 1.f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
                                 v.x = v.x * r; v.y = v.y * r; v.z = v.z * r;
-                                r = s2[0] * (2.f * t.next() - 1.f);
+                                r = (g_mut == 14) ? s2[0] * t.next() : s2[0] * (2.f * t.next() - 1.f);
                                 om_sincos2pi(r * 0.159154943091895336f, &sphi, &cphi);
                                 float cx = ay * v.z - az * v.y, cy = az * v.x - ax * v.z, cz = ax * v.y - ay * v.x;
                                 v.x = v.x * cphi + cx * sphi;
@@ -691,8 +748,8 @@ struct Thread {
                             float sphi, cphi;
                             om_sincos2pi(t.next(), &sphi, &cphi);
                             r = sqrtf(2.f * t.scatlen());
-                            cphi *= s2[0] * r;
-                            sphi *= s2[1] * r;
+                            cphi *= s2[g_mut == 15 ? 1 : 0] * r;
+                            sphi *= s2[g_mut == 15 ? 0 : 1] * r;
                             sphi *= 1.f / sqrtf(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
                             float cx = s1[1] * v.z - s1[2] * v.y, cy = s1[2] * v.x - s1[0] * v.z, cz = s1[0] * v.y - s1[1] * v.x;
                             cphi *= 1.f / sqrtf(cx * cx + cy * cy + cz * cz);
@@ -719,7 +776,9 @@ struct Thread {
                         } else {
                             om_sincos2pi(t.next(), &sphi, &cphi);
                         }
-                        if (Q.srctype == 8 || Q.srctype == 17) {
+                        if (Q.srctype == 17 && g_mut == 19) {
+                            r = t.next() * fabsf(s1[0] - s1[1]) + s1[1];
+                        } else if (Q.srctype == 8 || Q.srctype == 17) {
                             r = sqrtf(t.next() * fabsf(s1[0] * s1[0] - s1[1] * s1[1]) + s1[1] * s1[1]);
                         } else if (fabsf(Q.srcdir[3]) < 1e-5f || fabsf(s1[1]) < 1e-5f) {
                             r = sqrtf(0.5f * t.scatlen()) * s1[0];
@@ -760,7 +819,7 @@ struct Thread {
                             ctheta = 2.f * t.next() - 1.f;
                             stheta = om_sin_from_cos(ctheta);
                         } else {
-                            om_sincos2pi(t.next() * 0.5f, &stheta, &ctheta);
+                            om_sincos2pi(t.next() * (g_mut == 18 ? 0.25f : 0.5f), &stheta, &ctheta);
                         }
                         rotatevector(v, stheta, ctheta, sphi, cphi);
                         canfocus = 0;
@@ -863,18 +922,8 @@ struct Thread {
                     float cphi = 1.f, sphi = 0.f, stheta, ctheta, tmp0;
                     if (!Q.is2d) om_sincos2pi(t.next(), &sphi, &cphi);
                     tmp0 = (v.w > (float)Q.gscatter) ? 0.f : prop.z;
-                    if (fabsf(tmp0) > EPSF) {
-                        float g = prop.z;
-                        tmp0 = (1.f - g * g) / (1.f - g + 2.f * g * t.next());
-                        tmp0 *= tmp0;
-                        tmp0 = (1.f + g * g - tmp0) / (2.f * g);
-                        tmp0 = fmaxf(-1.f, fminf(1.f, tmp0));
-                        ctheta = tmp0;
-                        stheta = om_sin_from_cos(ctheta);
-                    } else {
-                        ctheta = 2.f * t.next() - 1.f;
-                        stheta = om_sin_from_cos(ctheta);
-                    }
+                    ctheta = scatter_costheta(tmp0, prop.z, t.next());
+                    stheta = om_sin_from_cos(ctheta);
                     if (Q.savedet) {
                         const unsigned int lab = split() ? sv_label() : (mediaid & MED_MASK);
                         if (!split() || lab > 0) {
@@ -913,11 +962,7 @@ struct Thread {
             len = ((prop.y == 0.f) ? len : (slen / (prop.y * (gs ? (1.f - prop.z) : 1.f))));
             /* src/mcx_core.cu:2112-2117: the interface plane of a split voxel */
             hitintf = split() && svsplit && testint && ray_plane(len, slen);
-            p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
-            const bool stay = (slen == f.x) || hitintf;
-            if (flipdir[3] == 0) flipdir[0] += stay ? 0 : (v.x > 0.f ? 1 : -1);
-            if (flipdir[3] == 1) flipdir[1] += stay ? 0 : (v.y > 0.f ? 1 : -1);
-            if (flipdir[3] == 2) flipdir[2] += stay ? 0 : (v.z > 0.f ? 1 : -1);
+            advance(p, v, len, (slen == f.x) || hitintf, flipdir);
             p.w *= om_expf(-prop.x * len);
             f.x -= slen;
             f.y += len * prop.w * Q.oneoverc0;
@@ -1040,21 +1085,8 @@ struct Thread {
                 if (((mediaid && Q.doreflect) ||
                      (mediaid == 0 && ((bcf == bcUnknown && Q.doreflect) || bcf == bcReflect || bcf == bcMirror))) &&
                     (bcf == bcMirror || n1 != (lt100 ? prop.w : c.gprop[mediaid > 0u ? 1 : 0].w))) {
-                    float Rtotal = 1.f;
-                    float cphi, sphi, stheta, ctheta, tmp0, tmp1;
                     if (!split()) setprop(mediaid, idx1d);
-                    tmp0 = n1 * n1;
-                    tmp1 = prop.w * prop.w;
-                    cphi = fabsf(comp(v, flipdir[3]));
-                    sphi = 1.f - cphi * cphi;
-                    len = 1.f - tmp0 / tmp1 * sphi;
-                    if (len > 0.f && bcf != bcMirror) {
-                        ctheta = tmp0 * cphi * cphi + tmp1 * len;
-                        stheta = 2.f * n1 * prop.w * cphi * sqrtf(len);
-                        Rtotal = (ctheta - stheta) / (ctheta + stheta);
-                        ctheta = tmp1 * cphi * cphi + tmp0 * len;
-                        Rtotal = (Rtotal + (ctheta - stheta) / (ctheta + stheta)) * 0.5f;
-                    }
+                    const float Rtotal = (bcf == bcMirror) ? 1.f : fresnel_face(n1, prop.w, fabsf(comp(v, flipdir[3])));
                     if (Rtotal < 1.f && (!(mediaid == 0 && bcf == bcMirror)) && t.next() > Rtotal) {
                         transmit(v, n1, prop.w, flipdir[3]);
                         if (mediaid == 0 || (split() && sv_label() == 0)) {
@@ -1191,10 +1223,15 @@ void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, uns
     }
 }
 
-int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
-    if (!P || !O || !P->media || !P->prop || !P->seeds || !O->genergy) return -1;
+}  // extern "C"
+
+namespace {
+
+/* The launch constants of src/mcx_core.cu:2853-2876 and the property +
+ * detector table (:3316-3321) for one run. */
+int init_ctx(const OracleParams* P, OracleOut* O, Ctx& c, std::vector<Vec4>& gprop) {
+    if (!P || !O || !P->media || !P->prop || !P->seeds) return -1;
     if (P->medianum < 2 || P->medianum + P->detnum > 4000) return -2;
-    Ctx c;
     c.P = P;
     c.O = O;
     c.dimlen_x = P->nx;
@@ -1207,11 +1244,24 @@ int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
     c.w0offset = c.partialdata + 4;
     c.reclen = c.partialdata + (f & 1) + 3 * (((f >> 4) & 1) + ((f >> 5) & 1)) + ((f >> 6) & 1);
     O->reclen = c.reclen;
-    std::vector<Vec4> gprop(P->medianum + P->detnum);
+    gprop.assign(P->medianum + P->detnum, Vec4{0.f, 0.f, 0.f, 0.f});
     for (unsigned int i = 0; i < P->medianum; i++) gprop[i] = {P->prop[4 * i], P->prop[4 * i + 1], P->prop[4 * i + 2], P->prop[4 * i + 3]};
     for (unsigned int i = 0; i < P->detnum; i++)
         gprop[P->medianum + i] = {P->detpos[4 * i], P->detpos[4 * i + 1], P->detpos[4 * i + 2], P->detpos[4 * i + 3]};
     c.gprop = gprop.data();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
+    if (!O || !O->genergy) return -1;
+    Ctx c;
+    std::vector<Vec4> gprop;
+    const int rc = init_ctx(P, O, c, gprop);
+    if (rc) return rc;
     std::atomic<unsigned int> detected(O->detected);
     c.detected = &detected;
     unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
@@ -1247,6 +1297,75 @@ int mcx_oracle_run(const OracleParams* P, OracleOut* O) {
     O->nstep = nstep; O->ndeposit = ndep; O->nscatter = nscat; O->nlaunch = nlaunch;
     O->errors += errors;
     return 0;
+}
+
+void mcx_oracle_set_mutation(int id) { g_mut = id; }
+
+float mcx_oracle_hg_costheta(float g, float u) { return scatter_costheta(g, g, u); }
+
+float mcx_oracle_fresnel_face(float n1, float n2, float cosi) { return fresnel_face(n1, n2, cosi); }
+
+float mcx_oracle_reflectcoeff(float n1, float n2, float cosi) {
+    const Vec4 v = {0.f, 0.f, cosi, 0.f};
+    return reflectcoeff(v, n1, n2, 2);
+}
+
+void mcx_oracle_transmit(float n1, float n2, int face, const float* vin, float* vout) {
+    Vec4 v = {vin[0], vin[1], vin[2], 0.f};
+    transmit(v, n1, n2, face);
+    vout[0] = v.x; vout[1] = v.y; vout[2] = v.z;
+}
+
+int mcx_oracle_trace(const float* p0, const float* v0, unsigned int nx, unsigned int ny, unsigned int nz,
+                     unsigned int maxn, int* ids, float* lens) {
+    Vec4 p = {p0[0], p0[1], p0[2], 1.f};
+    const Vec4 v = {v0[0], v0[1], v0[2], 0.f};
+    const float rv[3] = {1.f / v.x, 1.f / v.y, 1.f / v.z};
+    int fd[4] = {(int)floorf(p.x), (int)floorf(p.y), (int)floorf(p.z), -1};
+    unsigned int n = 0;
+    while (n < maxn) {
+        const float len = hitgrid(p, v, rv, fd);
+        advance(p, v, len, false, fd);
+        ids[3 * n] = fd[0]; ids[3 * n + 1] = fd[1]; ids[3 * n + 2] = fd[2];
+        lens[n] = len;
+        n++;
+        if ((unsigned)fd[0] >= nx || (unsigned)fd[1] >= ny || (unsigned)fd[2] >= nz) break;
+    }
+    return (int)n;
+}
+
+int mcx_oracle_launch_samples(const OracleParams* P, unsigned int n, float* out) {
+    OracleOut O;
+    memset(&O, 0, sizeof(O));
+    Ctx c;
+    std::vector<Vec4> gprop;
+    const int rc = init_ctx(P, &O, c, gprop);
+    if (rc) return rc;
+    std::atomic<unsigned int> detected(0);
+    c.detected = &detected;
+    std::vector<float> pp(c.w0offset + 1, 0.f);
+    Thread T;
+    T.C = &c;
+    T.idx = 0;
+    T.nphoton = INT_MAX;
+    T.ppath = pp.data();
+    T.p = {0.f, 0.f, 0.f, NAN};
+    T.v = {0.f, 0.f, 0.f, 0.f};
+    T.f = {0.f, 0.f, 0.f, -1.f};
+    T.flipdir[0] = T.flipdir[1] = T.flipdir[2] = 0;
+    T.flipdir[3] = -1;
+    memcpy(&T.mediaid, &P->srcparam2[3], 4);
+    T.idx1d = 0;
+    T.prop = c.gprop[1];
+    T.t.seed(P->seeds);
+    for (unsigned int i = 0; i < n; i++) {
+        if (T.launch(0)) return -3;
+        float* o = out + 8 * (size_t)i;
+        o[0] = T.p.x; o[1] = T.p.y; o[2] = T.p.z; o[3] = T.p.w;
+        o[4] = T.v.x; o[5] = T.v.y; o[6] = T.v.z; o[7] = T.v.w;
+        T.f.w = 0.f;
+    }
+    return (int)n;
 }
 
 }  // extern "C"

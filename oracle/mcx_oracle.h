@@ -103,6 +103,45 @@ float mcx_oracle_expf(float x);
 float mcx_oracle_logf(float x);
 void  mcx_oracle_sincos2pi(float u, float* s, float* c);
 
+/* The transport loop's own pieces, exposed so tests can pin each one
+ * against closed forms (the loop calls exactly these functions):
+ * the scattering cos(theta) for g and a uniform u: HG inversion, or
+ * isotropic when |g| <= FLT_EPSILON (src/mcx_core.cu:1946-1964);
+ * Fresnel reflectance at a voxel face for |cos| of the incidence angle
+ * (:2418-2432); the void-entry reflectance used with isspecular
+ * (reflectcoeff, :552-570); the refraction of a unit vector across `face`
+ * (transmit, :523-537). */
+float mcx_oracle_hg_costheta(float g, float u);
+float mcx_oracle_fresnel_face(float n1, float n2, float cosi);
+float mcx_oracle_reflectcoeff(float n1, float n2, float cosi);
+void  mcx_oracle_transmit(float n1, float n2, int face, const float* vin, float* vout);
+
+/* Straight-line voxel stepping of the loop (hitgrid + face crossing,
+ * src/mcx_core.cu:493-507, :2127-2132) from p0 along unit v0: writes the
+ * voxel entered (3 ints) and the step length of each crossing until the ray
+ * leaves the nx*ny*nz grid or maxn steps.  Returns the step count. */
+int mcx_oracle_trace(const float* p0, const float* v0, unsigned int nx, unsigned int ny, unsigned int nz,
+                     unsigned int maxn, int* ids, float* lens);
+
+/* n successive photon launches of virtual thread 0 (launchnewphoton,
+ * src/mcx_core.cu:1089-1717) for the source in P, its RNG seeded from
+ * P->seeds[0..3]: out[8i..8i+7] = position x,y,z, weight, direction
+ * x,y,z, w.  Returns n, or < 0 when a launch fails. */
+int mcx_oracle_launch_samples(const OracleParams* P, unsigned int n, float* out);
+
+/* Planted mistakes, for the tests that prove the pins catch a mistake:
+ *  0 none;  1 HG cos(theta) with the wrong sign;  2 HG square dropped;
+ *  3 face Fresnel p-term equal to the s-term (n1, n2 transposed);
+ *  4 hitgrid always takes the upper face;  5 Snell ratio inverted in
+ *  transmit;  6 reflectcoeff with n1, n2 swapped;  11 fourierx second
+ *  edge reversed;  12 fourierx2d phases swapped;  13 zgaussian angle
+ *  scale sqrt(E) instead of sqrt(2E);  14 line fan one-sided;  15 slit
+ *  broadening widths swapped;  16 hyperboloid focus on the wrong side;
+ *  17 pattern3d x/y indices transposed;  18 arcsine zenith over [0, pi/2];
+ *  19 ring radius uniform instead of area-uniform.
+ * Global to the library: set between runs only. */
+void  mcx_oracle_set_mutation(int id);
+
 #ifdef __cplusplus
 }
 #endif

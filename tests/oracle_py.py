@@ -103,6 +103,21 @@ def lib():
         L.mcx_oracle_logf.argtypes = [ctypes.c_float]
         L.mcx_oracle_logf.restype = ctypes.c_float
         L.mcx_oracle_sincos2pi.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        for name in ("mcx_oracle_hg_costheta",):
+            getattr(L, name).argtypes = [ctypes.c_float, ctypes.c_float]
+            getattr(L, name).restype = ctypes.c_float
+        for name in ("mcx_oracle_fresnel_face", "mcx_oracle_reflectcoeff"):
+            getattr(L, name).argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+            getattr(L, name).restype = ctypes.c_float
+        L.mcx_oracle_transmit.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float)]
+        L.mcx_oracle_trace.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_uint,
+                                       ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_float)]
+        L.mcx_oracle_trace.restype = ctypes.c_int
+        L.mcx_oracle_launch_samples.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_uint, ctypes.POINTER(ctypes.c_float)]
+        L.mcx_oracle_launch_samples.restype = ctypes.c_int
+        L.mcx_oracle_set_mutation.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -139,6 +154,113 @@ def maskdet(vol, detpos, svmc=0):
     nx, ny, nz = v.shape
     lib().mcx_oracle_maskdet(_ptr(flat, ctypes.c_uint), nx, ny, nz, _ptr(d, ctypes.c_float), d.shape[0], int(svmc))
     return flat.reshape((nx, ny, nz), order="F")
+
+
+class mutation:
+    """Context manager that plants mistake `mid` in the oracle (ids in
+    oracle/mcx_oracle.h) and removes it on exit."""
+
+    def __init__(self, mid):
+        self.mid = int(mid)
+
+    def __enter__(self):
+        lib().mcx_oracle_set_mutation(self.mid)
+        return self
+
+    def __exit__(self, *exc):
+        lib().mcx_oracle_set_mutation(0)
+        return False
+
+
+def hg_costheta(g, u):
+    L = lib()
+    return np.array([L.mcx_oracle_hg_costheta(float(g), float(x)) for x in np.asarray(u, dtype=np.float32)],
+                    dtype=np.float32)
+
+
+def fresnel_face(n1, n2, cosi):
+    return lib().mcx_oracle_fresnel_face(float(n1), float(n2), float(cosi))
+
+
+def reflectcoeff(n1, n2, cosi):
+    return lib().mcx_oracle_reflectcoeff(float(n1), float(n2), float(cosi))
+
+
+def transmit(n1, n2, face, v):
+    vin = np.ascontiguousarray(v, dtype=np.float32)
+    vout = np.zeros(3, dtype=np.float32)
+    lib().mcx_oracle_transmit(float(n1), float(n2), int(face), _ptr(vin, ctypes.c_float), _ptr(vout, ctypes.c_float))
+    return vout
+
+
+def trace(p0, v0, dims, maxn=4096):
+    """Voxels entered and step lengths of a straight ray (the loop's stepping)."""
+    p = np.ascontiguousarray(p0, dtype=np.float32)
+    v = np.ascontiguousarray(v0, dtype=np.float32)
+    ids = np.zeros(3 * maxn, dtype=np.int32)
+    lens = np.zeros(maxn, dtype=np.float32)
+    n = lib().mcx_oracle_trace(_ptr(p, ctypes.c_float), _ptr(v, ctypes.c_float), dims[0], dims[1], dims[2], maxn,
+                               _ptr(ids, ctypes.c_int), _ptr(lens, ctypes.c_float))
+    return ids[: 3 * n].reshape(n, 3), lens[:n]
+
+
+SRCTYPES = ["pencil", "isotropic", "cone", "gaussian", "planar", "pattern", "fourier", "arcsine", "disk", "fourierx",
+            "fourierx2d", "zgaussian", "line", "slit", "pencilarray", "pattern3d", "hyperboloid", "ring"]
+
+
+def launch_samples(srctype, srcpos, srcdir, n, srcparam1=(0, 0, 0, 0), srcparam2=(0, 0, 0, 0), seed=1234,
+                   srcpattern=None, dims=(64, 64, 64)):
+    """n launches of one virtual thread in a uniform label-1 volume (so no
+    void skipping moves them): returns positions (n,3), weights (n,),
+    directions (n,3).  srcdir is normalised as the host does; its 4th
+    element is the focal length."""
+    st = SRCTYPES.index(srctype) if isinstance(srctype, str) else int(srctype)
+    nx, ny, nz = dims
+    media = np.ones(nx * ny * nz, dtype=np.uint32)
+    prop = np.array([[0, 0, 1, 1], [0.01, 1.0, 0.9, 1.37]], dtype=np.float32)
+    dp = np.zeros((1, 4), dtype=np.float32)
+    sd = np.array(srcdir, dtype=np.float64)
+    nrm = 1.0 / np.sqrt((sd[:3] ** 2).sum())
+    srcdir4 = [sd[0] * np.float32(nrm), sd[1] * np.float32(nrm), sd[2] * np.float32(nrm),
+               srcdir[3] if len(srcdir) > 3 else 0.0]
+    sp = list(srcpos)[:3] + [srcpos[3] if len(srcpos) > 3 else 1.0]
+    p2 = np.array(srcparam2, dtype=np.float32).copy()
+    if st in (0, 1, 2, 7, 11):
+        i1 = int(np.floor(sp[2])) * nx * ny + int(np.floor(sp[1])) * nx + int(np.floor(sp[0]))
+        p2[2:4] = np.array([i1, 1], dtype=np.uint32).view(np.float32)
+    seeds = seedwords(seed, 4)
+    P = OracleParams()
+    P.nx, P.ny, P.nz = nx, ny, nz
+    P.media = _ptr(media, ctypes.c_uint)
+    P.prop = _ptr(prop, ctypes.c_float)
+    P.medianum = 2
+    P.detpos = _ptr(dp, ctypes.c_float)
+    P.detnum = 0
+    P.srctype = st
+    P.srcpos[:] = list(np.array(sp, dtype=np.float32))
+    P.srcdir[:] = list(np.array(srcdir4, dtype=np.float32))
+    P.srcparam1[:] = list(np.array(srcparam1, dtype=np.float32))
+    P.srcparam2[:] = list(p2)
+    P.twin0, P.twin1, P.Rtstep = 0.0, 5e-9, 2e8
+    P.oneoverc0 = P.minaccumtime = R_C0
+    P.maxgate = 1
+    P.maxvoidstep = 1000
+    P.gscatter = 1000000000
+    P.save2pt = 1
+    P.nthread = 1
+    P.threadphoton = n
+    P.seeds = _ptr(seeds, ctypes.c_uint)
+    keep = [media, prop, dp, seeds]
+    if srcpattern is not None:
+        pat = np.ascontiguousarray(np.asarray(srcpattern, dtype=np.float32).ravel(order="F"))
+        keep.append(pat)
+        P.srcpattern = _ptr(pat, ctypes.c_float)
+    out = np.zeros(8 * n, dtype=np.float32)
+    rc = lib().mcx_oracle_launch_samples(ctypes.byref(P), n, _ptr(out, ctypes.c_float))
+    if rc != n:
+        raise RuntimeError("launch sampling failed (%d)" % rc)
+    o = out.reshape(n, 8)
+    return o[:, 0:3].astype(np.float64), o[:, 3].astype(np.float64), o[:, 4:7].astype(np.float64)
 
 
 def savedet_reclen(savedetflag, medianum):

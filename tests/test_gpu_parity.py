@@ -119,6 +119,9 @@ def test_sphere100_gates_detectors_bitexact():
     dict(srctype="line", srcpos=[10, 29.5, 29.5], srcparam1=[40, 0, 0, 0], srcparam2=[1.0, 0, 0, 0]),
     dict(srctype="slit", srcpos=[10, 29.5, 0], srcparam1=[40, 0, 0, 0], srcparam2=[0.1, 0.05, 0, 0]),
     dict(srctype="hyperboloid", srcpos=[29.5, 29.5, 0], srcparam1=[1.0, 20.0, 5.0, 0]),
+    dict(srctype="arcsine", srcpos=[29.5, 29.5, 29.5]),
+    dict(srctype="ring", srcparam1=[8, 3, 0, 0]),
+    dict(srctype="ring", srcparam1=[8, 3, 0.5, 2.0]),
     dict(srctype="pattern", srcpos=[10, 10, 0], srcparam1=[40, 0, 0, 8], srcparam2=[0, 40, 0, 8],
          srcpattern=np.random.default_rng(1).uniform(size=(8, 8)).astype(np.float32)),
     dict(srctype="pattern3d", srcpos=[20, 20, 0], srcparam1=[8, 8, 4, 0],
@@ -154,11 +157,14 @@ def test_autopilot_cube60b_known_answer():
 
 
 @pytest.mark.parametrize("bc,nphoton,band", [("aarraa", 100000, (0.27, 0.28)), ("cccccc", 1000, (0.99, 1.0)),
-                                             ("rrrrrr", 20000, (0.0, 1.0)), ("mmaamm", 20000, (0.0, 1.0))])
+                                             ("rrrrrr", 20000, (0.26, 0.29)), ("mmaamm", 20000, None)])
 def test_boundary_conditions_bitexact(bc, nphoton, band):
     """Per-face boundary strings (absorb / reflect / mirror / cyclic):
     bit-exact against the oracle, and the reference's known answers
-    (test/testmcx.sh:73, :77)."""
+    (test/testmcx.sh:73, :77; 'rrrrrr' forces the Fresnel faces of cube60b,
+    27.x %, with a 5-sigma band at 2e4 photons).  Mirror faces have no
+    published answer: the oracle's mirror is pinned by the unfolded-domain
+    equivalence (tests/test_oracle_pins.py)."""
     cfg = workloads.cube60(nphoton, reflect=0, bc=bc)
     res = _run(_layout(cfg, nthread=2048))
     res["_tstep"] = cfg["tstep"]
@@ -167,7 +173,7 @@ def test_boundary_conditions_bitexact(bc, nphoton, band):
     _check_energies(res, ora)
     _check_flux(res, ora, rtol=2e-4, floor=1e-2)
     a = res["stat"]["energyabs"] / res["stat"]["energytot"]
-    assert band[0] <= a < band[1], a
+    assert band is None or band[0] <= a < band[1], a
 
 
 def test_boundary_detector_faces_bitexact():
