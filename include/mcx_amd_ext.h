@@ -74,7 +74,11 @@ typedef struct McxBatchOptions {
     int   block_size;        /* workgroup size, multiple of 64; 0: autotuned */
     int   use_graph;         /* capture clear + transport + energy sums + merge in a hipGraph */
     int   repeat;            /* executions of the sequence (results are the last one's); <= 1: once */
-    int   download;          /* copy each normalised fluence to cfg->exportfield (allocated when NULL) */
+    int   download;          /* 1: copy each normalised fluence (nvox x gates) to cfg->exportfield;
+                                2: copy the time-integrated map instead, tstep x the sum over gates,
+                                   reduced on the device (nvox floats);
+                                0: none.  exportfield is allocated when NULL (refused when the
+                                buffers would exceed 80 % of the available host memory) */
     float tune_fraction;     /* photon fraction used by the autotuner's trial runs (0: 1/16) */
     int   chosen_photons_per_lane;  /* out */
     int   chosen_block_size;        /* out */
@@ -96,8 +100,8 @@ typedef struct McxBatchOptions {
  * each Config, e.g. after mcx_validatecfg) to many instances: each gets its
  * own volume, media, source, time gates, photon count and seed, and on
  * return its energytot/energyesc/energyabs/normalizer/runtime and, with
- * `download`, its normalised fluence in exportfield (overwritten, not
- * accumulated).  An instance with autopilot off and the same nthread as a
+ * `download`, its normalised fluence (or time-integrated map) in
+ * exportfield (overwritten, not accumulated).  An instance with autopilot off and the same nthread as a
  * single-simulation run reproduces that run's per-thread trajectories.
  * Not batched: detected photons, respin != 1, labels >= 128.
  * `gpu` may be NULL (device from cfgs[0]->deviceid).  Returns 0, or a

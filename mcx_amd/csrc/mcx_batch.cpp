@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <climits>
@@ -39,6 +40,7 @@ extern "C" hipError_t mcxk_launch_batch(int tally, const KParam* params, const u
 extern "C" hipError_t mcxk_batch_attrs(int tally, int* numregs);
 extern "C" hipError_t mcxk_launch_batch_post(int tally, const KParam* params, BatchInst* bi, uint32_t ninst, hipStream_t stream);
 extern "C" hipError_t mcxk_fill_zero16(void* p, size_t bytes, hipStream_t stream);
+extern "C" hipError_t mcxk_launch_batch_cw(const KParam* params, const BatchInst* bi, uint32_t ninst, hipStream_t stream);
 
 namespace mcx {
 
@@ -67,7 +69,7 @@ struct InstPlan {
     unsigned int gates, nthread;
     int threadphoton, oddphotons;
     double cost;
-    size_t off_media, off_prop, off_seed, off_tally, off_tally2, off_energy, off_genergy, off_out;
+    size_t off_media, off_prop, off_seed, off_tally, off_tally2, off_tallyk, off_energy, off_genergy, off_out;
 };
 
 class BatchRun {
@@ -80,9 +82,14 @@ class BatchRun {
     }
 
     void setup();
+    /* a new workgroup size and tally cache for the next runs, on the same
+     * arenas (the autotuner's trials) */
+    void reshape(int block, int boxcells);
     void enqueue(hipStream_t s);
     float run(hipStream_t s, bool graph, int repeat, float* kernel_ms);
-    void download(hipStream_t s, bool fields);
+    /* energies always; fields 1: every gate's fluence, 2: the
+     * time-integrated map (nvox floats), 0: none */
+    void download(hipStream_t s, int fields);
     void read_counters(hipStream_t s, double* out);
 
     double photons = 0.0;
@@ -98,9 +105,11 @@ class BatchRun {
     int device_;
     std::vector<InstPlan> plan_;
     std::vector<uint32_t> order_, chunk_start_;
+    std::vector<KParam> kps_;
+    std::vector<BatchInst> bis_;
     uint32_t total_chunks_ = 0, maxtab_ = 0, boxcells_ = 0;
     unsigned int grid_ = 0;
-    DevMem dmedia_, dprop_, dseed_, dzero_, dout_, dkp_, dchunk_, dorder_, dbi_;
+    DevMem dmedia_, dprop_, dseed_, dzero_, dout_, dkp_, dchunk_, dorder_, dbi_, dcw_;
     size_t queue_off_ = 0, counters_off_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
@@ -195,6 +204,8 @@ void BatchRun::setup() {
         zero_b += align256(P.fieldlen * (tally_ == MCX_TALLY_F64 ? 8 : 4));
         P.off_tally2 = zero_b;
         if (tally_ != MCX_TALLY_F64) zero_b += align256(P.fieldlen * 4);
+        P.off_tallyk = zero_b;
+        if (tally_ == MCX_TALLY_F32_HILO) zero_b += align256(P.fieldlen * 4);
         P.off_genergy = zero_b;
         zero_b += align256((size_t)8 * P.nthread);
         P.off_energy = zero_b;
@@ -220,16 +231,6 @@ void BatchRun::setup() {
     }
 
     HIP_CHECK(mcxk_batch_attrs(tally_, &vgprs));
-    hipDeviceProp_t dp;
-    HIP_CHECK(hipGetDeviceProperties(&dp, device_));
-    const unsigned int block = (unsigned int)shape_.block;
-    if (block < 64 || block > 1024 || block % 64) MCX_THROW(-1, "batched launch: block size must be a multiple of 64 in [64, 1024]");
-    if ((size_t)(block / 64) * maxtab_ * 16 > 65536) MCX_THROW(-1, "batched launch: property tables exceed the LDS budget");
-    if (shape_.boxcells < 0) MCX_THROW(-1, "batched launch: negative tally cache size");
-    boxcells_ = (uint32_t)std::min(shape_.boxcells, batch_box_budget(device_, vgprs, block, maxtab_));
-    if (boxcells_ < 64) boxcells_ = 0;
-    const unsigned long long resident = (unsigned long long)dp.multiProcessorCount * 4 * waves_per_simd(vgprs) * 64;
-    grid_ = (unsigned int)std::max<unsigned long long>(1, resident / block);
 
     /* host staging */
     Stream st;
@@ -264,8 +265,10 @@ void BatchRun::setup() {
     dbi_.alloc(sizeof(BatchInst) * n_);
 
     char* zb = dzero_.as<char>();
-    std::vector<KParam> kps(n_);
-    std::vector<BatchInst> bis(n_);
+    kps_.assign(n_, KParam{});
+    bis_.assign(n_, BatchInst{});
+    std::vector<KParam>& kps = kps_;
+    std::vector<BatchInst>& bis = bis_;
     for (int i = 0; i < n_; i++) {
         const InstPlan& P = plan_[i];
         const Config* cfg = P.cfg;
@@ -285,16 +288,13 @@ void BatchRun::setup() {
         } else {
             kp.field = (float*)(zb + P.off_tally);
             kp.field2 = (float*)(zb + P.off_tally2);
+            kp.fieldk = (int32_t*)(zb + P.off_tallyk);
         }
         kp.fieldlen = P.fieldlen;
         kp.genergy = (float*)(zb + P.off_genergy);
         kp.seeds = (const uint32_t*)(dseed_.as<char>() + P.off_seed);
         kp.detcount = (uint32_t*)(zb + P.off_energy + 64); /* error-exit counter ([1]) */
         kp.counters = counters_enabled() ? (uint64_t*)(zb + counters_off_) : nullptr;
-        /* each instance's hot-cell box lives in the wave's LDS slice; the
-         * kernel passes the slice's offset */
-        hot_box_geometry(cfg, boxcells_, kp);
-        kp.hot_off = 0;
         BatchInst& b = bis[i];
         memset(&b, 0, sizeof(b));
         b.out = (float*)(dout_.as<char>() + P.off_out);
@@ -308,11 +308,50 @@ void BatchRun::setup() {
     HIP_CHECK(hipMemcpyAsync(dmedia_.p, hmedia.data(), media_b, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipMemcpyAsync(dprop_.p, hprop.data(), prop_b, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipMemcpyAsync(dseed_.p, hseed.data(), seed_b, hipMemcpyHostToDevice, st.s));
-    HIP_CHECK(hipMemcpyAsync(dkp_.p, kps.data(), sizeof(KParam) * n_, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipMemcpyAsync(dchunk_.p, chunk_start_.data(), 4 * n_, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipMemcpyAsync(dorder_.p, order_.data(), 4 * n_, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipMemcpyAsync(dbi_.p, bis.data(), sizeof(BatchInst) * n_, hipMemcpyHostToDevice, st.s));
     HIP_CHECK(hipStreamSynchronize(st.s));
+    reshape(shape_.block, shape_.boxcells);
+}
+
+void BatchRun::reshape(int block_size, int boxcells) {
+    hipDeviceProp_t dp;
+    HIP_CHECK(hipGetDeviceProperties(&dp, device_));
+    const unsigned int block = (unsigned int)block_size;
+    if (block < 64 || block > 1024 || block % 64) MCX_THROW(-1, "batched launch: block size must be a multiple of 64 in [64, 1024]");
+    if ((size_t)(block / 64) * maxtab_ * 16 > 65536) MCX_THROW(-1, "batched launch: property tables exceed the LDS budget");
+    if (boxcells < 0) MCX_THROW(-1, "batched launch: negative tally cache size");
+    shape_.block = block_size;
+    shape_.boxcells = boxcells;
+    boxcells_ = (uint32_t)std::min(boxcells, batch_box_budget(device_, vgprs, block, maxtab_));
+    if (boxcells_ < 64) boxcells_ = 0;
+    const unsigned long long resident = (unsigned long long)dp.multiProcessorCount * 4 * waves_per_simd(vgprs) * 64;
+    grid_ = (unsigned int)std::max<unsigned long long>(1, resident / block);
+    /* each instance's hot-cell box lives in the wave's LDS slice; the
+     * kernel passes the slice's offset */
+    for (int i = 0; i < n_; i++) {
+        hot_box_geometry(plan_[i].cfg, boxcells_, kps_[i]);
+        kps_[i].hot_off = 0;
+    }
+    Stream st;
+    HIP_CHECK(hipMemcpyAsync(dkp_.p, kps_.data(), sizeof(KParam) * n_, hipMemcpyHostToDevice, st.s));
+    HIP_CHECK(hipStreamSynchronize(st.s));
+    if (gexec_) {
+        (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+    }
+    if (graph_) {
+        (void)hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+/* Physical memory currently available to this process, in bytes (0 when
+ * unknown) */
+static size_t host_bytes_available() {
+    const long pages = sysconf(_SC_AVPHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
+    return (pages > 0 && psz > 0) ? (size_t)pages * (size_t)psz : 0;
 }
 
 void BatchRun::enqueue(hipStream_t s) {
@@ -367,7 +406,35 @@ float BatchRun::run(hipStream_t s, bool graph, int repeat, float* kernel_ms) {
     return total;
 }
 
-void BatchRun::download(hipStream_t s, bool fields) {
+void BatchRun::download(hipStream_t s, int fields) {
+    if (fields == 2) {
+        /* time-integrated maps reduced on the device: nvox floats per
+         * instance instead of nvox x gates */
+        size_t cw_b = 0;
+        std::vector<size_t> off(n_);
+        for (int i = 0; i < n_; i++) {
+            off[i] = cw_b;
+            cw_b += align256(plan_[i].nvox * 4);
+        }
+        if (!dcw_.p) dcw_.alloc(cw_b);
+        for (int i = 0; i < n_; i++) bis_[i].cw = (float*)(dcw_.as<char>() + off[i]);
+        HIP_CHECK(hipMemcpyAsync(dbi_.p, bis_.data(), sizeof(BatchInst) * n_, hipMemcpyHostToDevice, s));
+        HIP_CHECK(mcxk_launch_batch_cw(dkp_.as<KParam>(), dbi_.as<BatchInst>(), (uint32_t)n_, s));
+    }
+    /* host buffers the library has to allocate: refuse before allocating
+     * when they would not fit in the memory available */
+    if (fields) {
+        size_t need = 0;
+        for (int i = 0; i < n_; i++)
+            if (!plan_[i].cfg->exportfield) need += (fields == 2 ? plan_[i].nvox : plan_[i].fieldlen) * 4;
+        const size_t avail = host_bytes_available();
+        if (need && avail && need > avail / 10 * 8) {
+            char msg[256];
+            snprintf(msg, sizeof(msg), "batched launch: the fluence of %d instances needs %.1f GB of host memory, %.1f GB available",
+                     n_, need / 1e9, avail / 1e9);
+            MCX_THROW(-1, msg);
+        }
+    }
     std::vector<double> en((size_t)n_ * 32);
     std::vector<uint32_t> errs(n_);
     for (int i = 0; i < n_; i++) {
@@ -389,9 +456,11 @@ void BatchRun::download(hipStream_t s, bool fields) {
         cfg->his.normalizer = (float)e[2];
         cfg->maxgate = P.gates;
         if (fields) {
-            if (!cfg->exportfield) cfg->exportfield = (float*)calloc(P.fieldlen, sizeof(float));
+            const size_t len = fields == 2 ? P.nvox : P.fieldlen;
+            if (!cfg->exportfield) cfg->exportfield = (float*)calloc(len, sizeof(float));
             if (!cfg->exportfield) MCX_THROW(-1, "batched launch: out of host memory for the fluence");
-            HIP_CHECK(hipMemcpyAsync(cfg->exportfield, dout_.as<char>() + P.off_out, P.fieldlen * 4, hipMemcpyDeviceToHost, s));
+            const char* src = fields == 2 ? (const char*)bis_[i].cw : dout_.as<char>() + P.off_out;
+            HIP_CHECK(hipMemcpyAsync(cfg->exportfield, src, len * 4, hipMemcpyDeviceToHost, s));
         }
     }
     HIP_CHECK(hipStreamSynchronize(s));
@@ -449,14 +518,16 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
         if (chunks >= kChunksPerWave * waves) best.photons_per_lane = ppl;
     }
     double best_rate = -1.0;
-    const BatchShape base = best;
+    /* one reduced copy of the batch (arenas at full size, photon counts
+     * scaled), relaunched with each workgroup size and tally cache */
+    BatchShape sh = best;
+    sh.block = blk_c[0];
+    sh.boxcells = 0;
+    BatchRun r(cfgs, n, tally, sh, fraction, device);
+    r.setup();
     for (int b : blk_c) {
         for (int box : {0, INT_MAX}) {
-            BatchShape sh = base;
-            sh.block = b;
-            sh.boxcells = box;
-            BatchRun r(cfgs, n, tally, sh, fraction, device);
-            r.setup();
+            r.reshape(b, box);
             float kms = 0.f;
             r.run(s, false, 1, &kms);
             const double rate = r.photons / std::max(1e-3, (double)kms);
@@ -507,7 +578,8 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     r.setup();
     float kms = 0.f;
     const float tms = r.run(st.s, opt->use_graph != 0, opt->repeat, &kms);
-    r.download(st.s, opt->download != 0);
+    if (opt->download < 0 || opt->download > 2) MCX_THROW(-1, "mcx_run_batch: download must be 0, 1 or 2");
+    r.download(st.s, opt->download);
     r.read_counters(st.s, opt->counters);
     for (int i = 0; i < n; i++) cfgs[i]->runtime = (unsigned int)llround(tms);
     opt->kernel_ms = kms;

@@ -35,7 +35,7 @@ extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int
                                             unsigned int grid, unsigned int block, size_t shmem, hipStream_t stream);
 extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads);
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream);
-extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, size_t n, hipStream_t stream);
+extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, const void* c, size_t n, hipStream_t stream);
 
 namespace mcx {
 
@@ -353,7 +353,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
 
     Stream st;
-    DevMem dkp, dmedia, dprop, dfield, dfield2, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
+    DevMem dkp, dmedia, dprop, dfield, dfield2, dfieldk, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
     DevMem dsaved, drseed, drweight, drtof, drdetid, dpattern;
     if (pattern_len(cfg)) {
         dpattern.alloc(pattern_len(cfg) * sizeof(float));
@@ -386,6 +386,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     } else {
         dfield.alloc(fieldlen * sizeof(float));
         dfield2.alloc(fieldlen * sizeof(float));
+        if (tally == MCX_TALLY_F32_HILO) dfieldk.alloc(fieldlen * sizeof(int32_t));
     }
     dmerged.alloc(fieldlen * sizeof(float));
     dgenergy.alloc((size_t)nthread * 2 * sizeof(float));
@@ -419,6 +420,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.gprop = dprop.as<float>();
     kp.field = dfield.as<float>();
     kp.field2 = dfield2.as<float>();
+    kp.fieldk = dfieldk.as<int32_t>();
     kp.field64 = dfield64.as<double>();
     kp.fieldlen = fieldlen;
     kp.genergy = dgenergy.as<float>();
@@ -480,6 +482,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             } else {
                 HIP_CHECK(hipMemsetAsync(dfield.p, 0, dfield.bytes, st.s));
                 HIP_CHECK(hipMemsetAsync(dfield2.p, 0, dfield2.bytes, st.s));
+                if (dfieldk.p) HIP_CHECK(hipMemsetAsync(dfieldk.p, 0, dfieldk.bytes, st.s));
             }
             if (savedet) HIP_CHECK(hipMemsetAsync(ddet.p, 0, ddet.bytes, st.s));
             HIP_CHECK(hipMemsetAsync(dcount.p, 0, dcount.bytes, st.s));
@@ -535,7 +538,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             }
             if (cfg->issave2pt) {
                 const void* a = (tally == MCX_TALLY_F64 || tally == MCX_TALLY_F64_XCD) ? dfield64.p : dfield.p;
-                HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, fieldlen, st.s));
+                HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, dfieldk.p, fieldlen, st.s));
                 HIP_CHECK(hipMemcpyAsync(field.data(), dmerged.p, fieldlen * sizeof(float), hipMemcpyDeviceToHost, st.s));
                 HIP_CHECK(hipStreamSynchronize(st.s));
                 if (abs(cfg->respin) > 1) {

@@ -22,6 +22,11 @@ enum {
 
 #define MCX_NUM_XCD 8
 
+/* Hi/lo tally (mode 1): the FP32 hi half is kept within about one chunk of
+ * zero by moving whole chunks to an integer count per cell; a power of two,
+ * so the moves and the count are exact */
+#define MCX_HILO_CHUNK 1024.0f
+
 struct KParam {
     /* grid */
     uint32_t nx, ny, nz, dimxy, nvox;
@@ -48,6 +53,7 @@ struct KParam {
     const float* gprop;           /* ntable x 4 floats */
     float* field;                 /* tally (FP32 modes) */
     float* field2;                /* shadow (mode 0) or lo (mode 1) */
+    int32_t* fieldk;              /* mode 1: chunks of MCX_HILO_CHUNK moved out of the hi half */
     double* field64;              /* modes 2, 3 (mode 3: MCX_NUM_XCD copies of fieldlen) */
     uint64_t fieldlen;            /* voxels x gates of one tally copy */
     float* genergy;               /* 2*nthread */
@@ -104,6 +110,7 @@ struct BatchInst {
     int32_t outputtype;
     int32_t normalize;   /* 0: raw tally */
     int32_t pad;
+    float* cw;           /* time-integrated map (nvox), filled by mcx_batch_cw */
 };
 
 #endif

@@ -65,6 +65,50 @@ __device__ __forceinline__ float atom_add_ret(float* p, float v) {
 __device__ __forceinline__ uint32_t atom_add_u32(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void atom_add_nr(int32_t* p, int32_t v) {
+    __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Completes an FP32 hi/lo deposit of w whose returning add found `a` in
+ * the hi half of a cell (the add stored s = fl(a + w)):
+ *  - the exact rounding error of that add (TwoSum) goes to the lo half;
+ *  - when a -> s crossed a multiple of MCX_HILO_CHUNK, the lane moves the
+ *    chunks it crossed from the hi half to the cell's chunk count, by a
+ *    returning add whose rounding error also goes to the lo half.
+ * Every move is claimed by the one add that crossed the boundary, so the
+ * hi half stays within about a chunk of zero and its rounding step far
+ * below the deposits.  The same lane then takes the lo half (exchange with
+ * zero) and adds it to the hi half, again a returning add whose error goes
+ * back to lo: the lo half is emptied every chunk, so it stays small even
+ * when its errors are one-signed (equal deposits, e.g. every unscattered
+ * photon crossing the source voxel, round the same way each time).
+ * Without both bounds the hi half grows until deposits fall below half its
+ * rounding step, and the lo half, grown large, loses what it gathers.
+ * The merged value is chunks x MCX_HILO_CHUNK + hi + lo, in FP64. */
+__device__ __forceinline__ void hilo_complete(float* hi, float* lo, int32_t* chunks, float a, float w) {
+    const float s = a + w;
+    const float bb = s - a;
+    const float e = (a - (s - bb)) + (w - bb);
+    if (e != 0.f) atom_add_nr(lo, e);
+    const float m = floorf(s * (1.f / MCX_HILO_CHUNK)) - floorf(a * (1.f / MCX_HILO_CHUNK));
+    if (m != 0.f) {
+        const float d = -m * MCX_HILO_CHUNK;
+        const float a2 = atom_add_ret(hi, d);
+        const float s2 = a2 + d;
+        const float b2 = s2 - a2;
+        const float e2 = (a2 - (s2 - b2)) + (d - b2);
+        if (e2 != 0.f) atom_add_nr(lo, e2);
+        atom_add_nr(chunks, (int32_t)m);
+        const float x = __hip_atomic_exchange(glb(lo), 0.f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (x != 0.f) {
+            const float a3 = atom_add_ret(hi, x);
+            const float s3 = a3 + x;
+            const float b3 = s3 - a3;
+            const float e3 = (a3 - (s3 - b3)) + (x - b3);
+            if (e3 != 0.f) atom_add_nr(lo, e3);
+        }
+    }
+}
 
 template <typename MT>
 __device__ __forceinline__ uint32_t load_media(const void* m, uint32_t i);
@@ -878,8 +922,9 @@ struct Lane {
      * 2246-2253): once the returned old value exceeds MAX_ACCUM, move
      * MAX_ACCUM from the value half to the shadow half.  Hi/lo mode: the
      * atomic stored s = fl(old + w); TwoSum gives the exact rounding error,
-     * which goes to the lo half.  Both moves preserve the merged sum, so
-     * doing them one deposit later changes only when, not what. */
+     * which goes to the lo half, and whole chunks leave the hi half
+     * (hilo_complete).  The moves preserve the merged sum, so doing them
+     * one deposit later changes only when, not what. */
     __device__ __forceinline__ void flush_pending() {
         if (TALLY == MCX_TALLY_F64 || TALLY == MCX_TALLY_F64_XCD || pend_i == ~(size_t)0) return;
         const size_t i = pend_i;
@@ -891,10 +936,7 @@ struct Lane {
                 atom_add_nr(kp.field2 + i, (a > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
             }
         } else {
-            const float s = a + w;
-            const float bb = s - a;
-            const float e = (a - (s - bb)) + (w - bb);
-            if (e != 0.f) atom_add_nr(kp.field2 + i, e);
+            hilo_complete(kp.field + i, kp.field2 + i, kp.fieldk + i, a, w);
         }
     }
 
@@ -911,13 +953,33 @@ struct Lane {
                 const uint32_t c = (((uint32_t)tshift * kp.hot_bz + lz) * kp.hot_by + ly) * kp.hot_bx + lx;
                 if (TALLY == MCX_TALLY_F32_HILO) {
                     /* FP32 hi/lo cells: the returned old value gives the
-                     * TwoSum error of the hi add, which goes to the lo half */
+                     * TwoSum error of the hi add, which goes to the lo half;
+                     * chunks crossed leave the hi cell for the global chunk
+                     * count, as in hilo_complete */
                     float* box = (float*)((char*)lds + hoff);
                     const float a = __hip_atomic_fetch_add(box + c, weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     const float s = a + weight;
                     const float bb = s - a;
                     const float e = (a - (s - bb)) + (weight - bb);
                     if (e != 0.f) __hip_atomic_fetch_add(box + kp.hot_cells + c, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const float m = floorf(s * (1.f / MCX_HILO_CHUNK)) - floorf(a * (1.f / MCX_HILO_CHUNK));
+                    if (m != 0.f) {
+                        const float d = -m * MCX_HILO_CHUNK;
+                        const float a2 = __hip_atomic_fetch_add(box + c, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        const float s2 = a2 + d;
+                        const float b2 = s2 - a2;
+                        const float e2 = (a2 - (s2 - b2)) + (d - b2);
+                        if (e2 != 0.f) __hip_atomic_fetch_add(box + kp.hot_cells + c, e2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        atom_add_nr(kp.fieldk + (size_t)vox + (size_t)tshift * kp.nvox, (int32_t)m);
+                        const float x = __hip_atomic_exchange(box + kp.hot_cells + c, 0.f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (x != 0.f) {
+                            const float a3 = __hip_atomic_fetch_add(box + c, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            const float s3 = a3 + x;
+                            const float b3 = s3 - a3;
+                            const float e3 = (a3 - (s3 - b3)) + (x - b3);
+                            if (e3 != 0.f) __hip_atomic_fetch_add(box + kp.hot_cells + c, e3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
                 } else {
                     double* box = (double*)((char*)lds + hoff);
                     __hip_atomic_fetch_add(box + c, (double)weight, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1283,7 +1345,7 @@ __device__ __forceinline__ void hot_box_zero(const KParam& kp, uint32_t off, uin
 }
 
 /* adds the box to the tally: one global atomic per non-zero cell (hi/lo:
- * the hi half by a returning add whose TwoSum error joins the lo half) */
+ * the hi half by a returning add completed by hilo_complete) */
 template <int TALLY>
 __device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, uint32_t t, uint32_t n) {
     extern __shared__ F4 lds[];
@@ -1298,13 +1360,7 @@ __device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, ui
         if (TALLY == MCX_TALLY_F32_HILO) {
             const float w = ((const float*)base)[c];
             const float lo = ((const float*)base)[kp.hot_cells + c];
-            if (w != 0.f) {
-                const float a = atom_add_ret(kp.field + i, w);
-                const float s = a + w;
-                const float bb = s - a;
-                const float e = (a - (s - bb)) + (w - bb);
-                if (e != 0.f) atom_add_nr(kp.field2 + i, e);
-            }
+            if (w != 0.f) hilo_complete(kp.field + i, kp.field2 + i, kp.fieldk + i, atom_add_ret(kp.field + i, w), w);
             if (lo != 0.f) atom_add_nr(kp.field2 + i, lo);
         } else {
             const double sum = ((const double*)base)[c];
@@ -1486,8 +1542,26 @@ __global__ void mcx_batch_merge(const KParam* __restrict__ params, const BatchIn
         if (TALLY == MCX_TALLY_F64)
             v = (float)glb(kp.field64)[i];
         else
-            v = (float)((double)glb(kp.field)[i] + (double)glb(kp.field2)[i]);
+            v = (float)((double)glb(kp.fieldk)[i] * (double)MCX_HILO_CHUNK + (double)glb(kp.field)[i] + (double)glb(kp.field2)[i]);
         out[i] = v * scale;
+    }
+}
+
+/* time-integrated (CW) map of each instance's normalised output, one row
+ * of blocks per instance: cw[v] = tstep * sum over gates of out[g][v],
+ * summed in FP64 in gate order (one thread per voxel, gate planes read
+ * coalesced) */
+__global__ void mcx_batch_cw(const KParam* __restrict__ params, const BatchInst* __restrict__ bi) {
+    const KParam& kp = params[blockIdx.y];
+    const BatchInst& b = bi[blockIdx.y];
+    const size_t nvox = kp.nvox;
+    const uint32_t gates = kp.maxgate;
+    const MCX_GLB float* out = glb(b.out);
+    MCX_GLB float* cw = glb(b.cw);
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (size_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (uint32_t g = 0; g < gates; g++) acc += (double)out[(size_t)g * nvox + v];
+        cw[v] = (float)(acc * (double)b.tstep);
     }
 }
 
@@ -1526,10 +1600,11 @@ __global__ void mcx_merge_f32(float* __restrict__ out, const float* __restrict__
     for (; i < n; i += stride) out[i] = a[i] + b[i];
 }
 
-__global__ void mcx_merge_hilo(float* __restrict__ out, const float* __restrict__ hi, const float* __restrict__ lo, size_t n) {
+__global__ void mcx_merge_hilo(float* __restrict__ out, const float* __restrict__ hi, const float* __restrict__ lo,
+                               const int32_t* __restrict__ chunks, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) out[i] = (float)((double)hi[i] + (double)lo[i]);
+    for (; i < n; i += stride) out[i] = (float)((double)chunks[i] * (double)MCX_HILO_CHUNK + (double)hi[i] + (double)lo[i]);
 }
 
 __global__ void mcx_merge_f64x(float* __restrict__ out, const double* __restrict__ d, size_t n) {
@@ -1635,6 +1710,11 @@ extern "C" hipError_t mcxk_launch_batch_post(int tally, const KParam* params, Ba
     return hipGetLastError();
 }
 
+extern "C" hipError_t mcxk_launch_batch_cw(const KParam* params, const BatchInst* bi, uint32_t ninst, hipStream_t stream) {
+    hipLaunchKernelGGL(mcx_batch_cw, dim3(256, ninst), dim3(256), 0, stream, params, bi);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t mcxk_fill_zero16(void* p, size_t bytes, hipStream_t stream) {
     const size_t n = bytes / 16;
     size_t g = (n + 255) / 256;
@@ -1653,12 +1733,13 @@ extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uin
     return hipGetLastError();
 }
 
-extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, size_t n, hipStream_t stream) {
+extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, const void* c, size_t n, hipStream_t stream) {
     const unsigned int block = 256;
     size_t g = (n + block - 1) / block;
     unsigned int grid = (unsigned int)(g > 65536 ? 65536 : (g ? g : 1));
     if (tally == MCX_TALLY_F32_HILO)
-        hipLaunchKernelGGL(mcx_merge_hilo, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b, n);
+        hipLaunchKernelGGL(mcx_merge_hilo, dim3(grid), dim3(block), 0, stream, out, (const float*)a, (const float*)b,
+                           (const int32_t*)c, n);
     else if (tally == MCX_TALLY_F64)
         hipLaunchKernelGGL(mcx_merge_f64, dim3(grid), dim3(block), 0, stream, out, (const double*)a, n);
     else if (tally == MCX_TALLY_F64_XCD)

@@ -68,7 +68,10 @@ def cw_fluence(results, cfgs):
     out = []
     for r, c in zip(results, cfgs):
         u = float(c.get("unitinmm", 1.0))
-        out.append(r["flux"].sum(axis=3) * float(c["tstep"]) * u * u)
+        if "cw" in r:  # reduced on the device: tstep x the sum over gates
+            out.append(r["cw"] * (u * u))
+        else:
+            out.append(r["flux"].sum(axis=3) * float(c["tstep"]) * u * u)
     return out
 
 
@@ -86,7 +89,7 @@ def simulate(cfgs, nphoton, seed_salt, tally=2, minenergy=0.0):
         d["seed"] = int((c["seed"] * 1000003 + seed_salt * 7919 + i) % (2 ** 31 - 2)) + 1
         d["quiet"] = True
         runs.append(d)
-    res, info = mcx_amd.run_batch(runs, tally=tally, photons_per_lane=32, block_size=256)
+    res, info = mcx_amd.run_batch(runs, tally=tally, photons_per_lane=32, block_size=256, download="cw")
     return cw_fluence(res, runs), info["kernel_ms"], info["photons"]
 
 
@@ -358,9 +361,13 @@ def head_batch_eval(dn, args, hc, ref, mask, single_plain_ms5, log):
             c["quiet"] = True
             cfgs.append(c)
         t0 = time.perf_counter()
+        log("head batched: launch of %d x %.3g photons: autotuning" % (K, nph))
         _, info = mcx_amd.run_batch(cfgs, tally=1, download=False)  # autotunes the shape (cached per batch)
+        log("head batched: tuned (%d photons/lane, block %d, cache %d) in %.1f s; timed run" %
+            (info["photons_per_lane"], info["block_size"], info["tally_cache_cells"], time.perf_counter() - t0))
         res, info = mcx_amd.run_batch(cfgs, tally=1, photons_per_lane=info["photons_per_lane"],
-                                      block_size=info["block_size"], tally_cache=info["tally_cache_cells"] or -1)
+                                      block_size=info["block_size"], tally_cache=info["tally_cache_cells"] or -1,
+                                      download="cw")
         maps = cw_fluence(res, cfgs)
         log("head batched: %d x %.3g photons in %.1f ms (%.0f s with tuning and download)" %
             (K, nph, info["total_ms"], time.perf_counter() - t0))
@@ -377,6 +384,7 @@ def head_batch_eval(dn, args, hc, ref, mask, single_plain_ms5, log):
         nph = float(v) * args.head_low
         maps, ms = launch(nph, 50 + k)
         errs_low, errs_dn, dn_ms = [], [], []
+        log("head batched: denoising %d maps" % len(maps))
         for i, m in enumerate(maps):
             dphi = dn.upload(m)
             pf = nph if args.photon_feature else None

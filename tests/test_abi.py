@@ -340,3 +340,14 @@ def test_time_to_5pct_fits():
     n5, t5 = dx._time_to_5pct(ph, [0.1, 0.025], [10.0, 40.0], False)
     assert n5 == pytest.approx(2e6, rel=1e-6)
     assert t5 == pytest.approx(20.0, rel=1e-6)
+
+
+def test_batch_download_refused_beyond_host_memory():
+    """A batch whose downloaded fluence would exceed the host memory is
+    refused before anything is allocated or launched."""
+    from mcx_amd import batch, workloads
+    huge = dict(workloads.cube60(1000), vol=np.lib.stride_tricks.as_strided(np.zeros(1, np.uint8), (4096, 4096, 4096), (0, 0, 0)),
+                tend=5e-9, tstep=5e-11)
+    with pytest.raises(RuntimeError, match="host memory"):
+        batch._check_host_memory([huge], 1)
+    batch._check_host_memory([workloads.cube60(1000)], 1)

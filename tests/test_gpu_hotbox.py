@@ -129,3 +129,15 @@ def test_box_on_a_2d_domain_against_oracle(shape, src, dirv):
     assert res["stat"]["energyabs"] == tot - seq_sum(ora["genergy"][0::2])
     ref, _ = normalize_like_engine(ora["field"], tot, cfg["tstep"])
     _same_field(res["flux"], ref, 1e-6)
+
+
+@pytest.mark.parametrize("hot", [False, True])
+def test_hilo_exact_with_millions_of_deposits_per_cell(hot):
+    """The single-simulation hi/lo tally (global cells, or the LDS box cells
+    and their flush) against the FP64 tally of the same trajectories when
+    the source voxel takes ~2e7 deposits."""
+    cfg = _layout(workloads.cube60(20_000_000, tend=1e-9, tstep=1e-9), nthread=65536, block=256)
+    hl, _ = _run(cfg, 1, hot)
+    f6, _ = _run(cfg, 2, hot)
+    assert hl["stat"]["energyabs"] == f6["stat"]["energyabs"]
+    _same_field(hl["flux"], f6["flux"], 2.5e-7)
