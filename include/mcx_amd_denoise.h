@@ -47,6 +47,16 @@ int mcx_dn_wgrad_reduce(const float* partial, int nsplit, float* dw, int kpad, i
 /* db[co] = sum_v dy[v][co] * (mask[v][co] > 0); part: nsplit*cout floats */
 int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int mcs, float* part, int nsplit, float* db,
                      long long nvox, int cout, void* stream);
+/* mcx_dn_bias_grad without a workspace argument and without allocating:
+ * the nsplit x cout partial sums live in a fixed device array of 65536
+ * floats (256 KB on each GPU the library runs on, held until the process
+ * ends).  No counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving or training loops).
+ * Returns -7 when nsplit * cout exceeds 65536.  Calls that overlap on one
+ * GPU share the array and race; each call writes every partial sum it
+ * reads, so nothing carries over from the previous call. */
+int mcx_dn_bias_grad_noalloc(const void* dy, int dycs, const void* mask, int mcs, int nsplit, float* db, long long nvox,
+                             int cout, void* stream);
 /* 2x2x2 max pooling (d, h, wd even: input dims); arg: window index of the
  * maximum per output element */
 int mcx_dn_maxpool_fwd(const void* x, int xcs, void* y, int ycs, unsigned char* arg, int nb, int d, int h, int wd, int c,

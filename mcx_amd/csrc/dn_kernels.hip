@@ -32,6 +32,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 namespace dnk {
 
@@ -1240,6 +1241,49 @@ extern "C" int mcx_dn_bias_grad(const void* dy, int dycs, const void* mask, int 
         return -22;
     const int vps = (int)((nvox + nsplit - 1) / nsplit);
     hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(bias_grad, dim3(nsplit), dim3(256), 0, st, (const uint16_t*)dy, dycs, (const uint16_t*)mask, mcs, part, nvox,
+                       vps, cout);
+    hipLaunchKernelGGL(sum_rows, dim3((cout + 63) / 64), dim3(1024), 0, st, (const float*)part, nsplit, cout, db);
+    return dn_ret();
+}
+
+/* ---- bias gradient without allocation ------------------------------------
+ * The per-range partial sums of mcx_dn_bias_grad live in this fixed device
+ * array instead of a caller workspace, for callers that cannot allocate.
+ * Each GPU holds its own copy until the process ends; host code reaches it
+ * only through the address hipGetSymbolAddress returns for the current
+ * device (the array's name is a host-side stand-in). */
+#define MCX_DN_SCRATCH_FLOATS 65536
+#define MCX_DN_MAX_DEVICES 64
+static __device__ __align__(256) float scratch[MCX_DN_SCRATCH_FLOATS];
+static float* g_scratch_dev[MCX_DN_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* this device's copy of `scratch`, looked up once per device; NULL on error */
+static float* scratch_ptr(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_dev[dev] = (float*)p;
+    }
+    return g_scratch_dev[dev];
+}
+
+extern "C" int mcx_dn_bias_grad_noalloc(const void* dy, int dycs, const void* mask, int mcs, int nsplit, float* db, long long nvox,
+                                        int cout, void* stream) {
+    if (!dy || !db || nsplit <= 0 || nsplit > 65535 || cout <= 0 || cout > 256 || cout % 8 || 256 % (cout / 8) || nvox <= 0 ||
+        dycs % 8 || !al16(dy) || (mask && (mcs % 8 || !al16(mask))))
+        return -22;
+    if ((long long)nsplit * cout > MCX_DN_SCRATCH_FLOATS) return -7; /* E2BIG: beyond the fixed scratch array */
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MCX_DN_MAX_DEVICES) return -22;
+    float* part = scratch_ptr(dev);
+    if (!part) return -(int)hipErrorInvalidSymbol;
+    const int vps = (int)((nvox + nsplit - 1) / nsplit);
+    hipStream_t st = (hipStream_t)stream;
+    /* bias_grad writes every one of the nsplit x cout partial sums that
+     * sum_rows reads: nothing to clear */
     hipLaunchKernelGGL(bias_grad, dim3(nsplit), dim3(256), 0, st, (const uint16_t*)dy, dycs, (const uint16_t*)mask, mcs, part, nvox,
                        vps, cout);
     hipLaunchKernelGGL(sum_rows, dim3((cout + 63) / 64), dim3(1024), 0, st, (const float*)part, nsplit, cout, db);
