@@ -12,6 +12,11 @@ launch+sync time of the transport kernel (src/mcx_core.cu:3392, :3572,
 :4009-4012); inputs are resident in HBM when it starts.  ms_per_step is the
 whole call (uploads, tally merge, D2H, normalisation).
 
+The second half of the metric, time to 5 % median voxel error on the 256^3
+head volume for the plain kernel and for batched low-photon runs with the
+learned denoiser (mcx_amd.time_to_error), is measured after the photons/ms
+leg and reported in config.time_to_5pct (--no-tt5 skips it).
+
 Prints one JSON line.
 """
 import argparse
@@ -57,11 +62,13 @@ def atomic_roof_table():
     """Microbenchmark of the device's scattered-atomic rate: every one of
     4096 x 256 lanes adds 1 at 256 uniformly random indices.  Modes: 0 FP64
     agent scope no return, 1 FP32 no return, 2 FP32 returning, 3 FP64
-    workgroup scope into a private copy per XCD."""
+    workgroup scope into a private copy per XCD; coalesced forms: 4 FP64 with
+    8 lanes per random 64-byte block, 5 FP64 with a wave on 64 consecutive
+    elements, 6 FP32 with 16 lanes per 64-byte block."""
     from mcx_amd import _lib
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4, 5, 6):
         for mb in ATOMIC_SIZES_MB:
-            es = 8 if mode in (0, 3) else 4
+            es = 4 if mode in (1, 2, 6) else 8
             r = _lib.atomic_bench(mode, (mb << 20) // es, 256, 4096)
             r.update(mode=mode, mbytes=mb)
             print(json.dumps(r), flush=True)
@@ -172,6 +179,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atomic-roof", action="store_true", help="print the scattered-atomic microbenchmark and exit")
     ap.add_argument("--dn-timing", action="store_true", help="time the denoiser U-Net (train step, 256^3 inference)")
+    ap.add_argument("--no-tt5", action="store_true", help="skip the time-to-5%% comparison on the head volume")
+    ap.add_argument("--tt5-k", type=int, default=8, help="head volumes per batched launch in the time-to-5%% leg")
     args = ap.parse_args()
     if args.atomic_roof:
         atomic_roof_table()
@@ -232,11 +241,12 @@ def main():
     flush_pp = stats_w["hot_cells"] * (stats_w["nthread"] / stats_w["nblock"]) / nph
     glob_pp = dep_pp - hot_pp + flush_pp
     reclen = 3 if args.workload == "sphere100" else 0
-    # algorithmic bytes: 1 B label load per step; per global tally atomic
-    # the tally word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8,
-    # FP64: 8+8); detected-photon records
+    # algorithmic bytes: 1 B label load per step; per tally deposit the
+    # tally word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8, FP64:
+    # 8+8), counted for every deposit whether the LDS box or a global
+    # atomic takes it; detected-photon records
     dep_bytes = {0: 8.0, 1: 16.0, 2: 16.0, 3: 16.0}[args.tally]
-    bytes_pp = steps_pp * 1.0 + glob_pp * dep_bytes + reclen * 4.0 * detected_w / nph
+    bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
     tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally))
@@ -284,6 +294,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": round(tpp * nph) if tpp else None,
+            "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per photon from the committed 1e7-photon "
+                              "counter profile (profiles/transport_sphere100_1e7_counters.md) x photons per launch"
+                              if tpp else None,
             "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3), "unit": "G atomics/s",
                        "frac": round(arate / aroof, 4),
                        "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},
@@ -292,7 +305,28 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = _cpu_baseline(mk, args.cpu_budget)
+    if not args.no_tt5 and args.workload == "sphere100":
+        line["config"]["time_to_5pct"] = _time_to_5pct(args.tt5_k)
     print(json.dumps(line), flush=True)
+
+
+def _time_to_5pct(k):
+    from mcx_amd import time_to_error
+    r = time_to_error.measure(K=k, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if "skipped" in r:
+        return r
+    p, d = r["plain"], r["batched_denoised"]
+    return {
+        "workload": r["workload"], "reliable_voxels": r["reliable_voxels"],
+        "plain_photons": round(p["photons_for_5pct"]), "plain_ms": round(p["ms_to_5pct"], 2),
+        "plain_error": round(p["error_at"], 4),
+        "batched_denoised_k": d["k"], "batched_denoised_photons": round(d["photons_for_5pct"]),
+        "batched_denoised_ms_per_volume": round(d["ms_per_volume_to_5pct"], 2),
+        "batched_denoised_error": round(d["error_at"], 4), "denoise_ms_per_volume": round(d["denoise_ms"], 2),
+        "host_ms_per_volume": round(d["host_ms_per_volume"], 2),
+        "ratio": round(r["ratio"], 4), "target_ratio": r["target_ratio"], "seconds": round(r["seconds"], 1),
+        "points": {"plain": p["points"], "batched_denoised": d["points"]},
+    }
 
 
 if __name__ == "__main__":

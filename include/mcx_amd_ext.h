@@ -59,9 +59,11 @@ int  mcx_amd_last_stats(double* out, int n);
  * nblocks x 256 lanes each add 1 at `nper` uniformly random indices of an
  * `nelem`-element array.  mode 0: FP64 agent scope, no return; 1: FP32 no
  * return; 2: FP32 returning; 3: FP64 workgroup scope into one private copy
- * per XCD.  out[0] atomics/s, out[1] ms, out[2] array sum / atomics issued
- * (1 = no lost update), out[3..10] workgroups per XCC id.  Returns 0 on
- * success. */
+ * per XCD; coalesced: 4 FP64 with 8 lanes per random aligned 64-byte
+ * block, 5 FP64 with a wave's 64 lanes on 64 consecutive elements, 6 FP32
+ * no return with 16 lanes per 64-byte block (nelem >= 64).  out[0]
+ * atomics/s, out[1] ms, out[2] array sum / atomics issued (1 = no lost
+ * update), out[3..10] workgroups per XCC id.  Returns 0 on success. */
 int  mcx_amd_atomic_bench(int mode, size_t nelem, unsigned int nper, unsigned int nblocks, double* out);
 
 /* Options and results of mcx_run_batch.  Inputs: zero selects the default
@@ -84,7 +86,8 @@ typedef struct McxBatchOptions {
     int   chosen_block_size;        /* out */
     int   vgprs;                    /* out: vector registers per lane of the batched kernel */
     float kernel_ms;         /* out: transport kernel time of the last execution (graph: whole graph) */
-    float total_ms;          /* out: last execution, clear to merge */
+    float total_ms;          /* out: last execution, clear to merge (and, with download 2, the
+                                time-integrated reduction) */
     double photons;          /* out: photons simulated per execution */
     double lanes;            /* out: virtual threads over all instances */
     double counters[4];      /* out, with mcx_amd_set_counters(1): steps, deposits, scatters, launches of the

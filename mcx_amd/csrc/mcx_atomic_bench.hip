@@ -10,6 +10,11 @@
  *   3  FP64 add, workgroup scope, no return, into one private copy of the
  *      array per XCD (chosen by the XCC_ID hardware register), so the add
  *      can complete in that XCD's L2
+ * and coalesced forms of mode 0/1, where groups of lanes share a random
+ * aligned block and each lane adds to its own element of it:
+ *   4  FP64, groups of 8 lanes per 64-byte block
+ *   5  FP64, the whole wave on 64 consecutive elements (512 bytes)
+ *   6  FP32 no return, groups of 16 lanes per 64-byte block
  * The array is zeroed first and its sum checked afterwards (every add adds
  * exactly 1), so a lost update is reported.
  */
@@ -33,15 +38,18 @@ __device__ __forceinline__ uint32_t xs32(uint32_t& s) {
 template <int MODE>
 __global__ void __launch_bounds__(256) atomic_bench(void* buf, size_t nelem, uint32_t nper, float* sink, uint32_t* xcchist) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t s = gid * 2654435761u + 12345u;
+    /* lanes of one group draw the same block index from a shared stream */
+    constexpr uint32_t G = MODE == 4 ? 8u : (MODE == 5 ? 64u : (MODE == 6 ? 16u : 1u));
+    uint32_t s = (gid / G) * 2654435761u + 12345u;
+    const size_t nblk = nelem / G;
     const uint32_t x = xcc_id();
     if (threadIdx.x == 0) atomicAdd(xcchist + x, 1u);
     float acc = 0.f;
     for (uint32_t k = 0; k < nper; k++) {
-        const size_t i = (size_t)(((uint64_t)xs32(s) * (uint64_t)nelem) >> 32);
-        if (MODE == 0) {
+        const size_t i = (size_t)(((uint64_t)xs32(s) * (uint64_t)nblk) >> 32) * G + gid % G;
+        if (MODE == 0 || MODE == 4 || MODE == 5) {
             unsafeAtomicAdd((double*)buf + i, 1.0);
-        } else if (MODE == 1) {
+        } else if (MODE == 1 || MODE == 6) {
             unsafeAtomicAdd((float*)buf + i, 1.0f);
         } else if (MODE == 2) {
             acc += unsafeAtomicAdd((float*)buf + i, 1.0f);
@@ -71,8 +79,8 @@ __global__ void sum_f32(const float* a, size_t n, double* out) {
  * out[2] = (sum of the array) / (atomics issued), 1 when no add was lost,
  * out[3..10] = workgroups seen per XCC id. */
 extern "C" int mcx_amd_atomic_bench(int mode, size_t nelem, unsigned int nper, unsigned int nblocks, double* out) {
-    if (mode < 0 || mode > 3 || nelem == 0 || nper == 0 || nblocks == 0) return -1;
-    const size_t esize = (mode == 0 || mode == 3) ? 8 : 4;
+    if (mode < 0 || mode > 6 || nelem < 64 || nper == 0 || nblocks == 0) return -1;
+    const size_t esize = (mode == 1 || mode == 2 || mode == 6) ? 4 : 8;
     const size_t copies = (mode == 3) ? 8 : 1;
     void* buf = nullptr;
     float* sink = nullptr;
@@ -92,7 +100,10 @@ extern "C" int mcx_amd_atomic_bench(int mode, size_t nelem, unsigned int nper, u
         case 0: hipLaunchKernelGGL(atomic_bench<0>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
         case 1: hipLaunchKernelGGL(atomic_bench<1>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
         case 2: hipLaunchKernelGGL(atomic_bench<2>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
-        default: hipLaunchKernelGGL(atomic_bench<3>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
+        case 3: hipLaunchKernelGGL(atomic_bench<3>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
+        case 4: hipLaunchKernelGGL(atomic_bench<4>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
+        case 5: hipLaunchKernelGGL(atomic_bench<5>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
+        default: hipLaunchKernelGGL(atomic_bench<6>, dim3(nblocks), dim3(256), 0, 0, buf, nelem, nper, sink, hist); break;
     }
     (void)hipEventRecord(b, 0);
     if (hipEventSynchronize(b) == hipSuccess) {

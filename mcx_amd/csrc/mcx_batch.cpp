@@ -25,6 +25,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <map>
 #include <mutex>
@@ -45,6 +46,17 @@ extern "C" hipError_t mcxk_launch_batch_cw(const KParam* params, const BatchInst
 namespace mcx {
 
 static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+/* With MCX_AMD_VERBOSE set, each phase of a batched call is reported on
+ * stderr with the seconds since the call started, so a long call shows
+ * where its time goes. */
+static void phase(const char* what, const std::chrono::steady_clock::time_point& t0) {
+    static const bool on = getenv("MCX_AMD_VERBOSE") != nullptr;
+    if (!on) return;
+    fprintf(stderr, "[mcx_run_batch] %s at %.2f s\n", what,
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    fflush(stderr);
+}
 
 struct BatchShape {
     int photons_per_lane = 0;  /* 0: each Config's own nthread (autopilot off) */
@@ -88,8 +100,9 @@ class BatchRun {
     void enqueue(hipStream_t s);
     float run(hipStream_t s, bool graph, int repeat, float* kernel_ms);
     /* energies always; fields 1: every gate's fluence, 2: the
-     * time-integrated map (nvox floats), 0: none */
-    void download(hipStream_t s, int fields);
+     * time-integrated map (nvox floats), 0: none.  Returns the device time
+     * of the time-integrated reduction in ms (0 without it). */
+    float download(hipStream_t s, int fields);
     void read_counters(hipStream_t s, double* out);
 
     double photons = 0.0;
@@ -406,7 +419,8 @@ float BatchRun::run(hipStream_t s, bool graph, int repeat, float* kernel_ms) {
     return total;
 }
 
-void BatchRun::download(hipStream_t s, int fields) {
+float BatchRun::download(hipStream_t s, int fields) {
+    float cw_ms = 0.f;
     if (fields == 2) {
         /* time-integrated maps reduced on the device: nvox floats per
          * instance instead of nvox x gates */
@@ -419,7 +433,12 @@ void BatchRun::download(hipStream_t s, int fields) {
         if (!dcw_.p) dcw_.alloc(cw_b);
         for (int i = 0; i < n_; i++) bis_[i].cw = (float*)(dcw_.as<char>() + off[i]);
         HIP_CHECK(hipMemcpyAsync(dbi_.p, bis_.data(), sizeof(BatchInst) * n_, hipMemcpyHostToDevice, s));
+        EventPair ev;
+        HIP_CHECK(hipEventRecord(ev.a, s));
         HIP_CHECK(mcxk_launch_batch_cw(dkp_.as<KParam>(), dbi_.as<BatchInst>(), (uint32_t)n_, s));
+        HIP_CHECK(hipEventRecord(ev.b, s));
+        HIP_CHECK(hipEventSynchronize(ev.b));
+        HIP_CHECK(hipEventElapsedTime(&cw_ms, ev.a, ev.b));
     }
     /* host buffers the library has to allocate: refuse before allocating
      * when they would not fit in the memory available */
@@ -448,7 +467,9 @@ void BatchRun::download(hipStream_t s, int fields) {
         const double* e = &en[(size_t)i * 32];
         uint32_t err;
         memcpy(&err, (const char*)e + 64 + 4, 4);
-        if (err) MCX_THROW(-2, "batched transport kernel: a photon left the volume without being relaunched (internal error)");
+        if (err)
+            MCX_THROW(-2, "batched transport kernel: a lane stopped on an internal error (a photon outside the volume without "
+                          "a relaunch, or a history beyond the step limit)");
         cfg->energyesc = e[0];
         cfg->energytot = e[1];
         cfg->energyabs = e[1] - e[0];
@@ -464,6 +485,7 @@ void BatchRun::download(hipStream_t s, int fields) {
         }
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    return cw_ms;
 }
 
 void BatchRun::read_counters(hipStream_t s, double* out) {
@@ -481,7 +503,12 @@ void BatchRun::read_counters(hipStream_t s, double* out) {
  * rule takes the largest candidate that leaves at least kChunksPerWave
  * chunks per resident wave at the batch's full photon counts (a reduced
  * copy of the batch cannot show this trade-off: it has fewer chunks, and
- * trials on one favour few photons per lane).  Measured on the
+ * trials on one favour few photons per lane).  When no candidate leaves
+ * that many the batch is small and bound by the longest chain of
+ * histories one lane runs: {1, 2, 4, 8, 16} photons per lane are then
+ * timed on the full batch (8 head-volume runs of 1.25e5 photons: 143 ms
+ * at 16 photons per lane, 94 ms at 1; 8 of 5e5: 154 ms at 16, 227 ms at
+ * 2, where the per-chunk costs dominate).  Measured on the
  * 256-instance workload: at 5.5e8 photons 8/16/32/64/128/256 photons per
  * lane give 13.2/16.4/19.7/22.8/25.4/25.8 k photons/ms (256: 11 chunks per
  * wave), at 5.5e7 photons 8/16/32/64 give 13.0/15.9/17.7/18.6 k (64: 4.4
@@ -503,6 +530,7 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
         if (it != g_tune_cache.end()) return it->second;
     }
     static const int ppl_c[] = {16, 32, 64, 128, 256};
+    static const int ppl_small[] = {1, 2, 4, 8, 16};
     static const int blk_c[] = {64, 256};
     int vgprs = 0;
     HIP_CHECK(mcxk_batch_attrs(tally, &vgprs));
@@ -510,7 +538,7 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
     HIP_CHECK(hipGetDeviceProperties(&dp, device));
     const double waves = (double)dp.multiProcessorCount * 4 * waves_per_simd(vgprs);
     BatchShape best;
-    best.photons_per_lane = ppl_c[0];
+    best.photons_per_lane = 0;
     best.block = 256;
     for (int ppl : ppl_c) {
         double chunks = 0.0;
@@ -518,6 +546,29 @@ static BatchShape autotune(Config** cfgs, int n, int tally, int device, double f
         if (chunks >= kChunksPerWave * waves) best.photons_per_lane = ppl;
     }
     double best_rate = -1.0;
+    if (best.photons_per_lane == 0) {
+        /* a small batch: its launch is bound by the longest chain of
+         * histories a lane runs, which fewer photons per lane shorten,
+         * against the fixed cost of each chunk, which more of them
+         * amortise; such a batch runs in a fraction of a second, so each
+         * candidate is timed on the full batch */
+        fraction = 1.0;
+        for (int ppl : ppl_small) {
+            BatchShape t;
+            t.photons_per_lane = ppl;
+            t.block = 256;
+            BatchRun tr(cfgs, n, tally, t, 1.0, device);
+            tr.setup();
+            float kms = 0.f;
+            tr.run(s, false, 1, &kms);
+            const double rate = tr.photons / std::max(1e-3, (double)kms);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best.photons_per_lane = ppl;
+            }
+        }
+        best_rate = -1.0;
+    }
     /* one reduced copy of the batch (arenas at full size, photon counts
      * scaled), relaunched with each workgroup size and tally cache */
     BatchShape sh = best;
@@ -554,6 +605,8 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     int device = 0;
     if (cfgs[0]->deviceid[0] > 0) device = cfgs[0]->deviceid[0] - 1;
     (void)gpu;
+    const auto t0 = std::chrono::steady_clock::now();
+    phase("start", t0);
     HIP_CHECK(hipSetDevice(device));
     Stream st;
     BatchShape shape;
@@ -568,7 +621,9 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     if (shape.photons_per_lane < 0) MCX_THROW(-1, "mcx_run_batch: photons_per_lane must be >= 0");
     if ((need_ppl && shape.photons_per_lane == 0) || shape.block == 0) {
         const double frac = opt->tune_fraction > 0.f ? (double)opt->tune_fraction : 1.0 / 16.0;
+        phase("autotune", t0);
         BatchShape t = autotune(cfgs, n, tally, device, frac, st.s);
+        phase("autotune done", t0);
         if (shape.photons_per_lane == 0 && need_ppl) shape.photons_per_lane = t.photons_per_lane;
         if (shape.block == 0) shape.block = t.block;
         if (opt->tally_cache_cells == 0) shape.boxcells = t.boxcells;
@@ -576,14 +631,17 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     if (!need_ppl && opt->photons_per_lane == 0) shape.photons_per_lane = 0;
     BatchRun r(cfgs, n, tally, shape, 1.0, device);
     r.setup();
+    phase("setup done", t0);
     float kms = 0.f;
     const float tms = r.run(st.s, opt->use_graph != 0, opt->repeat, &kms);
+    phase("run done", t0);
     if (opt->download < 0 || opt->download > 2) MCX_THROW(-1, "mcx_run_batch: download must be 0, 1 or 2");
-    r.download(st.s, opt->download);
+    const float cw_ms = r.download(st.s, opt->download);
     r.read_counters(st.s, opt->counters);
-    for (int i = 0; i < n; i++) cfgs[i]->runtime = (unsigned int)llround(tms);
+    phase("download done", t0);
+    for (int i = 0; i < n; i++) cfgs[i]->runtime = (unsigned int)llround(tms + cw_ms);
     opt->kernel_ms = kms;
-    opt->total_ms = tms;
+    opt->total_ms = tms + cw_ms;
     opt->photons = r.photons;
     opt->chosen_photons_per_lane = shape.photons_per_lane;
     opt->chosen_block_size = shape.block;

@@ -655,7 +655,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         fflush(cfg->flog);
     }
     cfg->energyabs = cfg->energytot - cfg->energyesc;
-    if (errlanes) MCX_THROW(-2, "transport kernel: a photon left the volume without being relaunched (internal error)");
+    if (errlanes)
+        MCX_THROW(-2, "transport kernel: a lane stopped on an internal error (a photon outside the volume without a relaunch, "
+                      "or a history beyond the step limit)");
 }
 
 static int list_gpu_impl(Config* cfg, GPUInfo** info) {

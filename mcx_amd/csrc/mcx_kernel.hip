@@ -39,6 +39,12 @@ constexpr uint32_t OUT_MAX = 0x7FFFFFFFu;
 constexpr float MAX_ACCUM = 1000.f;
 constexpr float ROULETTE = 10.f;
 constexpr float JUST_BELOW_ONE = 0.9998f;
+/* Steps one photon may take before its lane stops on an internal error
+ * (counted like a photon left outside the volume).  With time windows of
+ * nanoseconds a history takes at most ~1e5 steps; the limit turns a
+ * history that stops advancing into a reported error, not a launch that
+ * never ends. */
+constexpr uint32_t MAX_PHOTON_STEPS = 1u << 26;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
 enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
@@ -192,6 +198,7 @@ struct Lane {
     double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t gid;         /* global index of the current photon (replay) */
     uint32_t nstep, ndep, nscat, nlaunch, nhot;
+    uint32_t pstep;       /* steps of the current photon */
     uint32_t hoff;        /* byte offset of this lane's hot-cell box in the dynamic LDS */
     /* split-voxel state (src/mcx_core.h:65-83): interface point and
      * normal of the current voxel, its two labels, which side the photon
@@ -915,6 +922,7 @@ struct Lane {
         v.w = EPSF;
         f.z = 0.f;
         nlaunch++;
+        pstep = 0;
         return 0;
     }
 
@@ -1064,6 +1072,11 @@ struct Lane {
             }
             if (!(f.w < (float)nphoton)) break;
             nstep++;
+            if (++pstep > MAX_PHOTON_STEPS) {
+                atom_add_u32(kp.detcount + 1, 1u);
+                flush_pending();
+                return;
+            }
             if (f.x <= 0.f) {
                 f.x = scatlen();
                 if (v.w != EPSF) {

@@ -185,18 +185,31 @@ class HeadDenoiser:
         return dout.cpu().numpy().reshape(self.shape, order="F")
 
 
-def train(net, X, T, steps, batch=8, lr=1e-3, seed=2024, log=None):
-    """X: bf16 device tensor [N, S, S, S, 8]; T: fp32 device [N, S^3]."""
+def train(net, X, T, steps, batch=8, lr=1e-3, seed=2024, log=None, decay=True, augment=True):
+    """X: bf16 device tensor [N, S, S, S, 8] ([x, y, z] voxel order); T: fp32
+    device [N, S^3].  decay: cosine learning-rate decay to 5 % of lr;
+    augment: each batch mirrored in x and/or y and/or with x and y swapped
+    (symmetries of a source entering along +z; z is kept)."""
     import torch
     g = np.random.default_rng(seed)
     n = X.shape[0]
+    S = X.shape[1]
     losses = []
     t0 = time.perf_counter()
     for s in range(steps):
         idx = torch.from_numpy(g.choice(n, size=batch, replace=False)).to(X.device)
         xb = X.index_select(0, idx)
-        tb = T.index_select(0, idx).reshape(-1)
-        loss = net.train_step(xb, tb, lr=lr)
+        tb = T.index_select(0, idx).reshape(-1, S, S, S)
+        if augment:
+            k = int(g.integers(0, 8))
+            dims = [d for d, bit in ((1, 1), (2, 2)) if k & bit]
+            if dims:
+                xb, tb = torch.flip(xb, dims), torch.flip(tb, dims)
+            if k & 4:
+                xb, tb = xb.transpose(1, 2).contiguous(), tb.transpose(1, 2).contiguous()
+        tb = tb.reshape(-1)
+        lr_s = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * s / max(1, steps)))) if decay else lr
+        loss = net.train_step(xb, tb, lr=lr_s)
         if s % 50 == 0 or s == steps - 1:
             losses.append((s, float(loss)))
             if log:
@@ -473,7 +486,13 @@ def main(argv=None):
     lows = np.exp(gl.uniform(np.log(args.low / sa), np.log(args.low * sb), size=len(tr)))
     lo, lo_ms, _ = simulate(tr, lows, 1, minenergy=args.minenergy)
     log("train inputs: %d x %.0e-%.0e photons, %.0f ms" % (len(tr), args.low / sa, args.low * sb, lo_ms))
-    hi, hi_ms, hi_ph = simulate(tr, args.high, 2, minenergy=args.minenergy)
+    hi, hi_ms, hi_ph = [], 0.0, 0.0
+    for c0 in range(0, len(tr), 64):
+        m, ms, ph = simulate(tr[c0:c0 + 64], args.high, 2 + c0, minenergy=args.minenergy)
+        hi += m
+        hi_ms += ms
+        hi_ph += ph
+        log("train targets: %d of %d volumes (%.0f ms)" % (min(c0 + 64, len(tr)), len(tr), hi_ms))
     log("train targets: %d x %.0e photons, %.0f ms (%.0f photons/ms)" % (len(tr), args.high, hi_ms, hi_ph / hi_ms))
     X = to_device_bf16(make_features(lo, tr, lows if args.photon_feature else None))
     T = torch.from_numpy(make_targets(hi, tr)).to("cuda")

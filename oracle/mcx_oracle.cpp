@@ -55,6 +55,9 @@ const float TWO_PI = 6.28318530717959f;
 const float MAX_ACCUM = 1000.f;
 const float ROULETTE = 10.f;
 constexpr float JUST_BELOW_ONE = 0.9998f;
+/* the engine's per-photon step limit (an internal-error exit, never taken
+ * by a physical history with nanosecond time windows) */
+constexpr unsigned long long MAX_PHOTON_STEPS = 1ull << 26;
 enum { bcUnknown, bcReflect, bcAbsorb, bcMirror, bcCyclic };
 enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus, otWLTOF, otWPTOF };
 
@@ -291,7 +294,7 @@ struct Thread {
     float* ppath;
     unsigned int gid = 0;        /* global index of the current photon */
     unsigned int seedsave[4];    /* launch RNG state of the current photon */
-    unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0;
+    unsigned long long nstep = 0, ndep = 0, nscat = 0, nlaunch = 0, pstep = 0;
     int err = 0;
     /* split-voxel state (src/mcx_core.h:65-83) */
     float rp[3] = {0.f, 0.f, 0.f}, nv[3] = {0.f, 0.f, 0.f};
@@ -874,6 +877,7 @@ struct Thread {
         v.w = EPSF;
         f.z = 0.f;
         nlaunch++;
+        pstep = 0;
         return 0;
     }
 
@@ -916,6 +920,10 @@ struct Thread {
         mediaid &= MED_MASK;
         while (f.w < (float)nphoton) {
             nstep++;
+            if (++pstep > MAX_PHOTON_STEPS) {
+                err++;
+                return;
+            }
             if (f.x <= 0.f) {
                 f.x = t.scatlen();
                 if (v.w != EPSF) {

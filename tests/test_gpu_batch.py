@@ -105,8 +105,9 @@ def test_batch_autotuned_shape_and_energy():
     for c in cfgs:
         c["quiet"] = True
     res, info = mcx_amd.run_batch(cfgs, tally=1, tune_fraction=0.25)
-    # few photons: the smallest photons-per-lane candidate keeps the most chunks per wave
-    assert info["photons_per_lane"] == 16
+    # few photons: no throughput candidate leaves 4 chunks per resident
+    # wave, so photons per lane were timed among the small candidates
+    assert info["photons_per_lane"] in (1, 2, 4, 8, 16)
     assert info["block_size"] in (64, 256)
     assert info["photons"] == sum(c["nphoton"] for c in cfgs)
     for cfg, r in zip(cfgs, res):
