@@ -32,8 +32,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per photon of the transport kernel on the sphere100 workload with
 # the FP64 tally and the LDS hot-cell box: (FETCH_SIZE x 2.000 + WRITE_SIZE x
 # 1.000) per launch / 1e7 photons, scale factors from the FP64->FP32 merge
-# kernel as a copy of known size (profiles/transport_sphere100_1e7_counters.md)
-TRAFFIC_B_PER_PHOTON = {("sphere100", 2): 8254.0}
+# kernel as a copy of known size; keyed by (workload, tally mode, blocked
+# tally) (profiles/transport_sphere100_1e7_counters.md,
+# profiles/transport_sphere100_blocked_tally.md)
+TRAFFIC_B_PER_PHOTON = {("sphere100", 2, False): 8254.0, ("sphere100", 2, True): 5544.0}
 ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 
@@ -175,6 +177,7 @@ def main():
     ap.add_argument("--workload", default="sphere100", choices=["sphere100", "cube60", "batch256"])
     ap.add_argument("--batch-scale", type=float, default=1.0, help="photon scale of the batch256 workload")
     ap.add_argument("--tally", type=int, default=2, help="0 FP32+shadow, 1 FP32 hi/lo, 2 FP64, 3 FP64 per-XCD copies")
+    ap.add_argument("--tally-blocks", type=int, default=1, help="FP64 tally in 2x2x2 voxel blocks summed per lane in LDS")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atomic-roof", action="store_true", help="print the scattered-atomic microbenchmark and exit")
@@ -211,6 +214,7 @@ def main():
 
     cfg = mk(nph)
     mcx_amd.set_tally_mode(args.tally)
+    mcx_amd.set_tally_blocks(bool(args.tally_blocks))
     # warmup; the first one also counts work (steps, deposits) per photon
     stats_w = None
     for i in range(max(args.warmup, 1)):
@@ -239,7 +243,11 @@ def main():
     # global tally atomics per photon: the deposits outside the LDS hot-cell
     # box, plus the box's flush (at most one atomic per cell per workgroup)
     flush_pp = stats_w["hot_cells"] * (stats_w["nthread"] / stats_w["nblock"]) / nph
-    glob_pp = dep_pp - hot_pp + flush_pp
+    # requests the lanes send to the memory-side atomic unit: one per
+    # global atomic, one per 2x2x2 block the blocked tally adds (its cells
+    # share a 64-byte line), plus the box's flush
+    req = stats_w.get("tally_requests", 0.0)
+    glob_pp = (req / nph if req > 0 else dep_pp - hot_pp) + flush_pp
     reclen = 3 if args.workload == "sphere100" else 0
     # algorithmic bytes: 1 B label load per step; per tally deposit the
     # tally word(s) read and written (FP32: 4+4, FP32 hi/lo: 8+8, FP64:
@@ -249,7 +257,7 @@ def main():
     bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
-    tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally))
+    tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally, bool(stats_w.get("tally_blocks", 0))))
     # the tally's atomic instruction: FP64 no-return (modes 2, 3), FP32
     # returning (1), FP32 no-return + rare returning shadow moves (0)
     nvox = int(cfg["vol"].size)
@@ -285,6 +293,7 @@ def main():
             "lds_box_deposits_per_photon": round(hot_pp, 2),
             "lds_box_cells": int(stats_w["hot_cells"]),
             "global_atomics_per_photon": round(glob_pp, 2),
+            "tally_blocks": bool(stats_w.get("tally_blocks", 0)),
             "absorbed_fraction": round(absorbed, 6),
         },
         "roofline": {
@@ -295,8 +304,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": round(tpp * nph) if tpp else None,
             "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per photon from the committed 1e7-photon "
-                              "counter profile (profiles/transport_sphere100_1e7_counters.md) x photons per launch"
-                              if tpp else None,
+                              "counter profile of this tally layout (profiles/transport_sphere100_blocked_tally.md) x "
+                              "photons per launch; a constant, not measured in this run" if tpp else None,
             "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3), "unit": "G atomics/s",
                        "frac": round(arate / aroof, 4),
                        "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},

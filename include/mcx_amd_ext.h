@@ -42,6 +42,27 @@ void mcx_amd_set_counters(int enable);
  * left at full occupancy allows.  Returns the previous setting. */
 int  mcx_amd_set_hot_box(int enable);
 
+/* Blocked FP64 tally (default on): in subsequent mcx_run_simulation calls
+ * with tally mode 2, the tally is stored in 2 x 2 x 2 voxel blocks (the 8
+ * cells of a block in one 64-byte line), each lane sums its deposits into
+ * the block it is in, in LDS, and when a lane moves to another block the
+ * wave adds the blocks left by its lanes to the tally with 8 lanes to a
+ * block; the device's atomic unit takes one line per request.  The output
+ * is the same volume (merged back to the reference's order).  Returns the
+ * previous setting. */
+int  mcx_amd_set_tally_blocks(int enable);
+
+/* Numbering of replayed photons (seed = SEED_FROM_FILE) in subsequent
+ * mcx_run_simulation calls.  0 (default): photon k of virtual thread idx is
+ * saved photon idx*threadphoton + min(idx, oddphotons) + k, which visits
+ * every saved photon once.  1: the reference's numbering
+ * idx*threadphoton + min(idx, oddphotons - 1) + k (src/mcx_core.cu:1206,
+ * :1701, :2021): identical while threads outnumber photons; otherwise one
+ * photon is replayed twice and the last one never, and a run in which every
+ * thread takes the same count is rejected (the reference reads the entry
+ * before the first saved photon).  Returns the previous setting. */
+int  mcx_amd_set_replay_index(int reference);
+
 /* Statistics of the last mcx_run_simulation call on this process:
  *  out[0] kernel time in ms (HIP events, sum over launches)
  *  out[1] launch+sync wall time in ms (the reference's runtime definition)
@@ -52,6 +73,9 @@ int  mcx_amd_set_hot_box(int enable);
  *  out[11] tally cells in the hot-cell box (0: off)
  *  out[12] dynamic LDS bytes per workgroup
  *  out[13] deposits summed in the hot-cell box (of out[3]; with counters)
+ *  out[14] 1 when the tally was blocked (mcx_amd_set_tally_blocks)
+ *  out[15] global tally atomic requests of the lanes (with counters; a
+ *          blocked tally's block flush counts once)
  * Returns the number of values written (<= n). */
 int  mcx_amd_last_stats(double* out, int n);
 

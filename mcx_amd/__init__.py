@@ -30,5 +30,18 @@ def set_hot_box(enable):
     return bool(_lib.lib().mcx_amd_set_hot_box(1 if enable else 0))
 
 
+def set_tally_blocks(enable):
+    """FP64 tally in 2x2x2 voxel blocks, summed per lane in LDS and added
+    to the tally 8 lanes to a block (default on).  Returns the previous
+    setting."""
+    return bool(_lib.lib().mcx_amd_set_tally_blocks(1 if enable else 0))
+
+
+def set_replay_index(reference):
+    """Number replayed photons as the reference does (True) or contiguously
+    (False, default).  Returns the previous setting."""
+    return bool(_lib.lib().mcx_amd_set_replay_index(1 if reference else 0))
+
+
 def last_stats():
     return _lib.last_stats()

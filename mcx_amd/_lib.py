@@ -113,7 +113,9 @@ ABI_SYMBOLS = [
     "mcx_run_simulation", "mcx_list_gpu", "mcx_initcfg", "mcx_clearcfg", "mcx_preprocess", "mcx_maskdet",
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
-    "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_last_stats",
+    "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_set_replay_index",
+    "mcx_amd_set_tally_blocks",
+    "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
     "mcx_dn_conv3_fwd", "mcx_dn_conv3_fwd_cat", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
     "mcx_dn_bias_grad_noalloc",
@@ -154,6 +156,10 @@ def _setup(L):
     L.mcx_amd_set_counters.argtypes = [ctypes.c_int]
     L.mcx_amd_set_hot_box.argtypes = [ctypes.c_int]
     L.mcx_amd_set_hot_box.restype = ctypes.c_int
+    L.mcx_amd_set_tally_blocks.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_tally_blocks.restype = ctypes.c_int
+    L.mcx_amd_set_replay_index.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_replay_index.restype = ctypes.c_int
     L.mcx_amd_last_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
     L.mcx_amd_last_stats.restype = ctypes.c_int
     L.mcx_amd_abi_layout.argtypes = [P(ctypes.c_size_t), ctypes.c_int]
@@ -261,8 +267,9 @@ def atomic_bench(mode, nelem, nper, nblocks):
 
 
 def last_stats():
-    out = (ctypes.c_double * 14)()
-    n = lib().mcx_amd_last_stats(out, 14)
+    out = (ctypes.c_double * 16)()
+    n = lib().mcx_amd_last_stats(out, 16)
     keys = ["kernel_ms", "wall_ms", "nstep", "ndeposit", "nscatter", "nlaunch", "vgprs", "nthread", "nblock",
-            "nlaunches", "errlanes", "hot_cells", "lds_bytes", "ndeposit_hot"]
+            "nlaunches", "errlanes", "hot_cells", "lds_bytes", "ndeposit_hot", "tally_blocks",
+            "tally_requests"]
     return dict(zip(keys, list(out[:n])))

@@ -36,14 +36,20 @@ extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int
 extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads);
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream);
 extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, const void* c, size_t n, hipStream_t stream);
+extern "C" hipError_t mcxk_launch_merge_blocked(float* out, const double* a, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bnx,
+                                                 uint32_t bny, uint32_t bnz, size_t n, hipStream_t stream);
 
 namespace mcx {
 
 static std::mutex g_statmu;
-static double g_stats[14];
+static double g_stats[16];
 static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
 static int g_hotbox = 1;
+/* replayed-photon numbering: 0 contiguous, 1 the reference's */
+static int g_replayindex = 0;
+/* blocked FP64 tally with per-lane LDS blocks (KParam::tblock) */
+static int g_tblock = 1;
 
 static bool supported_source(int t) { return t >= MCX_SRC_PENCIL && t <= MCX_SRC_RING; }
 
@@ -83,7 +89,8 @@ void check_supported(const Config* cfg) {
     if ((cfg->srctype == MCX_SRC_PATTERN || cfg->srctype == MCX_SRC_PATTERN3D) && (!cfg->srcpattern || pattern_len(cfg) == 0))
         MCX_THROW(-4, "the 'srcpattern' field can not be empty when your 'srctype' is 'pattern'");
     const bool replaytype = cfg->outputtype == otJacobian || cfg->outputtype == otWP || cfg->outputtype == otDCS ||
-                            cfg->outputtype == otWLTOF || cfg->outputtype == otWPTOF;
+                            cfg->outputtype == otRF || cfg->outputtype == otRFmus || cfg->outputtype == otWLTOF ||
+                            cfg->outputtype == otWPTOF;
     if (replaytype && !replay) MCX_THROW(-6, "Jacobian output is only valid in the reply mode. Please define cfg.seed");
     if (cfg->outputtype != otFlux && cfg->outputtype != otFluence && cfg->outputtype != otEnergy && cfg->outputtype != otL &&
         !replaytype)
@@ -330,7 +337,10 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const bool replay = cfg->seed == MCX_SEED_FROM_FILE;
     const unsigned int blocks = (replay && cfg->replaydet == -1) ? std::max(1u, cfg->detnum) : 1u;
     if (replay && totalgates > maxgate) MCX_THROW(-1, "photon replay needs all time gates in one pass (reduce the gate count)");
-    const size_t fieldlen = nvox * maxgate * blocks;
+    /* RF replay: an imaginary plane follows the real one (src/mcx_core.cu:2909-2911) */
+    const bool rf = replay && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
+    const size_t planelen = nvox * maxgate * blocks;
+    const size_t fieldlen = planelen * (rf ? 2 : 1);
     if (cfg->exportfield == NULL && cfg->issave2pt) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
     if (cfg->exportdetected == NULL) cfg->exportdetected = (float*)malloc(std::max<size_t>(1, (size_t)hostdetreclen * cfg->maxdetphoton) * sizeof(float));
 
@@ -351,6 +361,11 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         threadphoton = (int)(gpuphoton / nthread / (size_t)(-cfg->respin));
         oddphotons = (int)(gpuphoton / (size_t)(-cfg->respin) - (size_t)threadphoton * nthread);
     }
+    /* the reference's replay numbering (src/mcx_core.cu:1206) starts thread
+     * 0 at photon -1 when every thread takes the same count */
+    if (replay && g_replayindex && oddphotons == 0 && threadphoton > 0)
+        MCX_THROW(-6, "the reference's replay photon numbering reads before the first saved photon when the photon count "
+                      "is a multiple of the thread count; change the thread count or use the contiguous numbering");
 
     Stream st;
     DevMem dkp, dmedia, dprop, dfield, dfield2, dfieldk, dfield64, dgenergy, dseeds, ddet, dcount, dcounters, dmerged;
@@ -379,8 +394,12 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     HIP_CHECK(hipMemcpyAsync(dprop.p, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, st.s));
 
     const bool savedet = cfg->issavedet != 0;
+    /* blocked FP64 tally: 2 x 2 x 2 voxel blocks of every gate plane */
+    const uint32_t bnx = (cfg->dim.x + 1) / 2, bny = (cfg->dim.y + 1) / 2, bnz = (cfg->dim.z + 1) / 2;
+    const uint64_t nblk = (uint64_t)(fieldlen / nvox) * bnz * bny * bnx;
+    bool tblock = g_tblock && tally == MCX_TALLY_F64 && nblk < 0xFFFFFFFFull;
     if (tally == MCX_TALLY_F64) {
-        dfield64.alloc(fieldlen * sizeof(double));
+        dfield64.alloc((tblock ? std::max<size_t>(fieldlen, (size_t)nblk * 8) : fieldlen) * sizeof(double));
     } else if (tally == MCX_TALLY_F64_XCD) {
         dfield64.alloc(fieldlen * MCX_NUM_XCD * sizeof(double));
     } else {
@@ -394,7 +413,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     dseeds.alloc((size_t)nthread * 4 * sizeof(uint32_t));
     if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
     dcount.alloc(4 * sizeof(uint32_t));
-    dcounters.alloc(5 * sizeof(uint64_t));
+    dcounters.alloc(6 * sizeof(uint64_t));
     const bool saveseed = savedet && cfg->issaveseed;
     if (saveseed) dsaved.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton) * 16);
     if (replay) {
@@ -438,6 +457,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.replayweight = drweight.as<float>();
     kp.replaytof = drtof.as<float>();
     kp.replaydetid = drdetid.as<int32_t>();
+    kp.replayindex = g_replayindex;
+    kp.omega = cfg->omega;
+    kp.rfshift = rf ? (int32_t)(maxgate * blocks) : 0;
 
     /* per lane in LDS: the partial paths, then (seed saving) the launch RNG state */
     const size_t lane_lds = (partialdata + (saveseed ? 4 : 0)) * sizeof(float);
@@ -445,16 +467,37 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     /* autopilot with the hot-cell box: one workgroup per CU holding all of
      * the CU's waves, so the box takes the CU's whole LDS instead of an
      * equal share of it between four workgroups */
+    /* blocked tally: per lane 9 words of block and one queue word */
+    const size_t lane_blk = tblock ? 40 : 0;
     if (cfg->autopilot && g_hotbox && tally != MCX_TALLY_F32_SHADOW && maxgate > 0) {
         const unsigned int whole = 64u * 4u * (unsigned int)waves_per_simd(numregs);
         if (whole > block && whole <= (unsigned int)maxthreads && nthread % whole == 0 &&
-            (size_t)ntable * 16 + (size_t)whole * lane_lds <= 16384)
+            (size_t)ntable * 16 + (size_t)whole * lane_lds <= 16384 &&
+            (size_t)ntable * 16 + (size_t)whole * (lane_lds + lane_blk) <= 65536)
             block = whole;
     }
     const unsigned int grid = nthread / block;
     const size_t shmem0 = (size_t)ntable * 16 + (size_t)block * lane_lds;
     if (shmem0 > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
-    const size_t shmem = set_hot_box(cfg, tally, block, numregs, shmem0, kp);
+    size_t shmemb = shmem0;
+    if (tblock) {
+        const size_t blk_off = (shmem0 + 15) & ~(size_t)15;
+        const size_t bq_off = (blk_off + (size_t)block * 36 + 15) & ~(size_t)15;
+        if (bq_off + (size_t)block * 4 <= 65536) {
+            kp.tblock = 1;
+            kp.bnx = bnx;
+            kp.bny = bny;
+            kp.bnz = bnz;
+            kp.blk_off = (uint32_t)blk_off;
+            kp.bq_off = (uint32_t)bq_off;
+            div_magic(kp.dimxy, &kp.hot_mxy, &kp.hot_shxy);
+            div_magic(kp.nx, &kp.hot_mx, &kp.hot_shx);
+            shmemb = bq_off + (size_t)block * 4;
+        } else {
+            tblock = false;
+        }
+    }
+    const size_t shmem = set_hot_box(cfg, tally, block, numregs, shmemb, kp);
 
     if (cfg->flog && cfg->printnum >= 0) {
         fprintf(cfg->flog, "GPU=%d (%s) threadph=%d extra=%d np=%.0f nthread=%u maxgate=%u repetition=%d\n", gpuid + 1, G.name,
@@ -538,7 +581,11 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             }
             if (cfg->issave2pt) {
                 const void* a = (tally == MCX_TALLY_F64 || tally == MCX_TALLY_F64_XCD) ? dfield64.p : dfield.p;
-                HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, dfieldk.p, fieldlen, st.s));
+                if (tblock)
+                    HIP_CHECK(mcxk_launch_merge_blocked(dmerged.as<float>(), dfield64.as<double>(), cfg->dim.x, cfg->dim.y,
+                                                        cfg->dim.z, bnx, bny, bnz, fieldlen, st.s));
+                else
+                    HIP_CHECK(mcxk_launch_merge(tally, dmerged.as<float>(), a, dfield2.p, dfieldk.p, fieldlen, st.s));
                 HIP_CHECK(hipMemcpyAsync(field.data(), dmerged.p, fieldlen * sizeof(float), hipMemcpyDeviceToHost, st.s));
                 HIP_CHECK(hipStreamSynchronize(st.s));
                 if (abs(cfg->respin) > 1) {
@@ -566,7 +613,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
     cfg->runtime = (unsigned int)llround(wall_ms);
 
-    uint64_t ctr[5] = {0, 0, 0, 0, 0};
+    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     if (g_counters) {
         HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
     }
@@ -586,6 +633,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         g_stats[11] = kp.hot_cells;
         g_stats[12] = (double)shmem;
         g_stats[13] = (double)ctr[4];
+        g_stats[14] = tblock ? 1.0 : 0.0;
+        g_stats[15] = (double)ctr[5];
     }
 
     /* normalisation (src/mcx_core.cu:3823-3925) */
@@ -605,7 +654,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
              * summed detected weight (per detector with replaydet -1); path
              * tallies (Jacobian, WL x TOF) are in grid units and also scaled
              * by unitinmm */
-            const bool pathtype = cfg->outputtype == otJacobian || cfg->outputtype == otWLTOF;
+            const bool pathtype = cfg->outputtype == otJacobian || cfg->outputtype == otRF || cfg->outputtype == otWLTOF;
             if (cfg->replaydet == -1) {
                 for (unsigned int det = 1; det <= cfg->detnum; det++) {
                     if (cfg->isnormalized != 2) {
@@ -621,6 +670,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
                     }
                     if (cfg->flog) fprintf(cfg->flog, "normalization factor for detector %u alpha=%f\n", det, scale);
                     mcx_normalize(cfg->exportfield + (size_t)(det - 1) * nvox * maxgate, scale, nvox * maxgate, cfg->isnormalized, 0, 1);
+                    if (rf)
+                        mcx_normalize(cfg->exportfield + planelen + (size_t)(det - 1) * nvox * maxgate, scale, nvox * maxgate,
+                                      cfg->isnormalized, 0, 1);
                 }
                 done = true;
             } else {
@@ -637,11 +689,12 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         }
     }
     if (cfg->parentid == mpStandalone) {
-        if (cfg->issave2pt) mcx_savedata(cfg->exportfield, fieldlen, cfg);
+        if (cfg->issave2pt) mcx_savedata(cfg->exportfield, planelen, cfg);
         if (cfg->issavedet && cfg->exportdetected) {
             cfg->his.unitinmm = cfg->unitinmm;
             cfg->his.savedphoton = (unsigned int)cfg->detectedcount;
             cfg->his.totalphoton = (unsigned int)cfg->nphoton;
+            if (cfg->issaveseed) cfg->his.seedbyte = 16;
             cfg->his.detected = (unsigned int)cfg->detectedcount;
             mcx_savedetphoton(cfg->exportdetected, cfg->seeddata, (int)cfg->detectedcount, saveseed ? 16 : 0, cfg);
         }
@@ -782,6 +835,18 @@ int mcx_amd_set_tally_mode(int mode) {
 
 void mcx_amd_set_counters(int enable) { g_counters = enable ? 1 : 0; }
 
+int mcx_amd_set_replay_index(int reference) {
+    int old = g_replayindex;
+    g_replayindex = reference ? 1 : 0;
+    return old;
+}
+
+int mcx_amd_set_tally_blocks(int enable) {
+    int old = g_tblock;
+    g_tblock = enable ? 1 : 0;
+    return old;
+}
+
 int mcx_amd_set_hot_box(int enable) {
     int old = g_hotbox;
     g_hotbox = enable ? 1 : 0;
@@ -790,7 +855,7 @@ int mcx_amd_set_hot_box(int enable) {
 
 int mcx_amd_last_stats(double* out, int n) {
     std::lock_guard<std::mutex> lk(g_statmu);
-    int k = std::min(n, 14);
+    int k = std::min(n, 16);
     for (int i = 0; i < k; i++) out[i] = g_stats[i];
     return k;
 }

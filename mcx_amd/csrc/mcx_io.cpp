@@ -517,6 +517,12 @@ static void loadjson(const Json& root, Config* cfg, std::string* volfile) {
             throw Error(-1, "JData-encoded volumes in Shapes are not supported by this engine");
         rasterize_shapes(*Shapes, g);
     }
+    /* src/mcx_utils.c:3203-3207: the detected-photon file header */
+    cfg->his.maxmedia = cfg->medianum - 1;
+    cfg->his.detnum = cfg->detnum;
+    cfg->his.srcnum = cfg->srcnum;
+    cfg->his.savedetflag = cfg->savedetflag;
+    cfg->his.totalsource = 1;
 }
 
 static std::string read_text(const char* fname) {
@@ -756,12 +762,24 @@ void mcx_savedata(float* dat, size_t len, Config* cfg) {
     EntryGuard g;
     try {
         std::string name = out_name(cfg);
+        /* src/mcx_utils.c:929-951: per-detector replay blocks and the RF
+         * imaginary plane form a sixth dimension; RF data is twice len */
+        const bool replay = cfg->seed == MCX_SEED_FROM_FILE;
+        const bool rf = replay && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
+        unsigned int d5 = 1;
+        if (replay && cfg->replaydet == -1 && cfg->detnum > 1) d5 *= cfg->detnum;
+        if (rf) d5 *= 2;
         if (cfg->outputformat == ofJNifti || cfg->outputformat == ofBJNifti) {
             std::vector<unsigned int> dims = {cfg->dim.x, cfg->dim.y, cfg->dim.z, cfg->maxgate, cfg->srcnum};
             std::vector<float> vs = {cfg->steps.x, cfg->steps.y, cfg->steps.z, cfg->tstep, 1.f};
+            if (d5 > 1) {
+                dims.push_back(d5);
+                vs.push_back(1.f);
+            }
             save_jnii(dat, dims, vs, name, cfg);
             return;
         }
+        if (rf) len *= 2;
         if (cfg->outputformat == ofNifti || cfg->outputformat == ofAnalyze || cfg->outputformat == ofUBJSON || cfg->outputformat == ofTX3)
             throw Error(-2, "this output format is not supported by this engine; use jnii or mc2");
         std::string fname = name + ".mc2";

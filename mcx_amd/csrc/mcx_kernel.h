@@ -72,6 +72,10 @@ struct KParam {
     const float* replayweight;    /* detected weight of each replayed photon */
     const float* replaytof;       /* time of flight of each replayed photon */
     const int32_t* replaydetid;   /* detector id of each replayed photon */
+    int32_t replayindex;          /* 0: photon k of thread idx is idx*threadphoton + min(idx, oddphotons) + k;
+                                     1: idx*threadphoton + min(idx, oddphotons - 1) + k (the reference's) */
+    float omega;                  /* RF replay: modulation angular frequency (rad/s) */
+    int32_t rfshift;              /* RF replay: gates from the real plane to the imaginary plane; 0 otherwise */
     uint32_t mediaformat;         /* 1, 2, 4: labels; 96-104: per-voxel property words */
     /* Hot-cell box: the tally cells of gates [0, hot_g) over voxels
      * [hot_x0, hot_x0 + hot_bx) x [hot_y0, ..) x [hot_z0, ..) are summed in
@@ -84,7 +88,20 @@ struct KParam {
     uint32_t hot_lo, hot_span;    /* linear voxel index of the box's first voxel, last minus first */
     /* division of a voxel index by dimxy and by nx (DivMagic) */
     uint32_t hot_mxy, hot_shxy, hot_mx, hot_shx;
+    /* Blocked FP64 tally (tblock = 1; single-simulation FP64 tally): the
+     * cell of voxel (x, y, z) in gate plane g is element
+     *   8 * (((g * bnz + z/2) * bny + y/2) * bnx + x/2) + (x&1) + 2 (y&1) + 4 (z&1),
+     * so the 8 cells of a 2 x 2 x 2 voxel block share one 64-byte line.
+     * Each lane sums the deposits into its current block in LDS (9 words
+     * per lane at byte offset blk_off: 8 FP32 cells and the block index);
+     * when a lane moves to another block the wave adds the blocks left by
+     * its lanes to the tally together, 8 lanes to a block, one atomic
+     * instruction per 8 blocks (lane-id queue of 64 words per wave at
+     * bq_off).  The device's atomic unit takes one 64-byte line per
+     * request, so a block's cells cost one request instead of one each. */
+    uint32_t tblock, bnx, bny, bnz, blk_off, bq_off;
 };
+
 
 /* Unsigned 32-bit division by an invariant d >= 1 as a multiply-high, two
  * shifts and two adds: Granlund and Montgomery, "Division by Invariant

@@ -174,6 +174,30 @@ __device__ __forceinline__ float nextafter_mcx(float a, int dir, uint32_t maxvoi
     return mcx_bits2f(i) - (float)maxvoidstep;
 }
 
+/* element of the blocked FP64 tally (KParam::tblock) for voxel (x, y, z)
+ * of gate plane g */
+__device__ __forceinline__ size_t block_cell(const KParam& kp, uint32_t x, uint32_t y, uint32_t z, uint32_t g) {
+    const uint32_t b = ((g * kp.bnz + (z >> 1)) * kp.bny + (y >> 1)) * kp.bnx + (x >> 1);
+    return ((size_t)b << 3) + (x & 1u) + ((y & 1u) << 1) + ((z & 1u) << 2);
+}
+
+/* voxel index -> x, y, z by the multiply-high divisions of KParam */
+__device__ __forceinline__ void vox_xyz(const KParam& kp, uint32_t vox, uint32_t& x, uint32_t& y, uint32_t& z) {
+    const uint32_t tz = __umulhi(vox, kp.hot_mxy);
+    z = (tz + ((vox - tz) >> (kp.hot_shxy & 0xffu))) >> (kp.hot_shxy >> 8);
+    const uint32_t r = vox - z * kp.dimxy;
+    const uint32_t ty = __umulhi(r, kp.hot_mx);
+    y = (ty + ((r - ty) >> (kp.hot_shx & 0xffu))) >> (kp.hot_shx >> 8);
+    x = r - y * kp.nx;
+}
+
+/* number of set bits of m below this lane */
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+#define MCX_NO_BLOCK 0xFFFFFFFFu
+
 template <int ISPENCIL, int ISREFLECT, int TALLY, typename MT>
 struct Lane {
     const KParam& kp;
@@ -198,8 +222,15 @@ struct Lane {
     double* tally64;      /* FP64 modes: this lane's tally copy */
     uint32_t gid;         /* global index of the current photon (replay) */
     uint32_t nstep, ndep, nscat, nlaunch, nhot;
+    uint32_t nreq;        /* global tally atomic requests (a block flush is one) */
     uint32_t pstep;       /* steps of the current photon */
     uint32_t hoff;        /* byte offset of this lane's hot-cell box in the dynamic LDS */
+    /* blocked FP64 tally (single-simulation kernels): this lane's LDS block
+     * (8 cells + block index), the index of the block it holds, the
+     * workgroup's block area and this wave's lane queue */
+    static constexpr bool BLOCKED = TALLY == MCX_TALLY_F64 && ISPENCIL != 2;
+    float* blk;
+    uint32_t bopen;
     /* split-voxel state (src/mcx_core.h:65-83): interface point and
      * normal of the current voxel, its two labels, which side the photon
      * is on, whether the voxel is split; whether the current path segment
@@ -877,7 +908,9 @@ struct Lane {
         /* src/mcx_core.cu:1205-1215: a replayed photon restarts from its
          * saved RNG state; with seed saving the launch state is kept (in
          * this lane's LDS column after the partial paths) until detection */
-        gid = idx * (uint32_t)kp.threadphoton + min(idx, (uint32_t)kp.oddphotons) + (uint32_t)((int)f.w + 1);
+        gid = idx * (uint32_t)kp.threadphoton +
+              (kp.replayindex ? (uint32_t)min((int)idx, kp.oddphotons - 1) : min(idx, (uint32_t)kp.oddphotons)) +
+              (uint32_t)((int)f.w + 1);
         if (kp.replay) {
             const MCX_GLB uint32_t* rs = glb(kp.replayseed) + (size_t)gid * 4;
             t0 = ((uint64_t)rs[1] << 32) | rs[0];
@@ -934,6 +967,7 @@ struct Lane {
      * (hilo_complete).  The moves preserve the merged sum, so doing them
      * one deposit later changes only when, not what. */
     __device__ __forceinline__ void flush_pending() {
+        if (BLOCKED) block_finish();
         if (TALLY == MCX_TALLY_F64 || TALLY == MCX_TALLY_F64_XCD || pend_i == ~(size_t)0) return;
         const size_t i = pend_i;
         const float a = pend_old, w = pend_w;
@@ -948,7 +982,66 @@ struct Lane {
         }
     }
 
-    __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight) {
+    /* the wave adds the blocks its lanes are leaving (pend_i: the cell of
+     * the deposit that did not fit a lane's block) to the tally, 8 lanes to
+     * a block; then those lanes open the block of their new deposit.
+     * Called where every lane still stepping is active. */
+    __device__ __forceinline__ void block_flush() {
+        if (!BLOCKED || !kp.tblock) return;
+        const bool mine = pend_i != ~(size_t)0;
+        const uint64_t m = __ballot(mine);
+        if (m == 0) return;
+        extern __shared__ F4 lds[];
+        const float* all = (const float*)((const char*)lds + kp.blk_off);
+        uint32_t* q = (uint32_t*)((char*)lds + kp.bq_off) + (threadIdx.x & ~63u);
+        const uint64_t act = __ballot(true);
+        const uint32_t na = (uint32_t)__popcll(act), rank = lanes_below(act);
+        const uint32_t work = 8u * (uint32_t)__popcll(m);
+        /* whole blocks per instruction: a multiple of 8 workers */
+        const uint32_t nw = na >= 8u ? (na & ~7u) : na;
+        if (mine) q[lanes_below(m)] = threadIdx.x;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (rank < nw) {
+            for (uint32_t item = rank; item < work; item += nw) {
+                const float* b = all + 9u * q[item >> 3];
+                const float w = b[item & 7u];
+                if (w != 0.f) atom_add_nr(tally64 + (((size_t)mcx_f2bits(b[8]) << 3) + (item & 7u)), (double)w);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (mine) {
+            nreq++;
+            for (uint32_t c = 0; c < 8; c++) blk[c] = 0.f;
+            blk[pend_i & 7u] = pend_w;
+            bopen = (uint32_t)(pend_i >> 3);
+            blk[8] = mcx_bits2f(bopen);
+            pend_i = ~(size_t)0;
+        }
+    }
+
+    /* a lane leaving the kernel adds its block (and a deposit still
+     * waiting for one) to the tally by itself */
+    __device__ __forceinline__ void block_finish() {
+        if (!BLOCKED || !kp.tblock) return;
+        if (bopen != MCX_NO_BLOCK)
+            for (uint32_t c = 0; c < 8; c++)
+                if (blk[c] != 0.f) {
+                    atom_add_nr(tally64 + (((size_t)bopen << 3) + c), (double)blk[c]);
+                    nreq++;
+                }
+        if (pend_i != ~(size_t)0) {
+            atom_add_nr(tally64 + pend_i, (double)pend_w);
+            nreq++;
+        }
+        bopen = MCX_NO_BLOCK;
+        pend_i = ~(size_t)0;
+    }
+
+    /* `main`: the voxel-exit deposit of the step loop (at most one per
+     * loop iteration), which may wait in the lane's block */
+    __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight, bool main = false) {
         ndep++;
         if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
             const uint32_t tz = __umulhi(vox, kp.hot_mxy);
@@ -996,6 +1089,26 @@ struct Lane {
                 return;
             }
         }
+        if (BLOCKED && kp.tblock) {
+            uint32_t x, y, z;
+            vox_xyz(kp, vox, x, y, z);
+            const size_t c = block_cell(kp, x, y, z, (uint32_t)tshift);
+            if (main) {
+                if ((uint32_t)(c >> 3) == bopen) {
+                    blk[c & 7u] += weight;
+                    return;
+                }
+                if (pend_i == ~(size_t)0) {
+                    pend_i = c;
+                    pend_w = weight;
+                    return;
+                }
+            }
+            atom_add_nr(tally64 + c, (double)weight);
+            nreq++;
+            return;
+        }
+        nreq++;
         const size_t i = (size_t)vox + (size_t)tshift * kp.nvox;
         if (TALLY == MCX_TALLY_F64) {
             atom_add_nr(tally64 + i, (double)weight);  /* no return: fire and forget */
@@ -1019,6 +1132,7 @@ struct Lane {
             atomicAdd((unsigned long long*)kp.counters + 2, (unsigned long long)nscat);
             atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)nlaunch);
             if (kp.hot_cells) atomicAdd((unsigned long long*)kp.counters + 4, (unsigned long long)nhot);
+            atomicAdd((unsigned long long*)kp.counters + 5, (unsigned long long)nreq);
         }
     }
 
@@ -1039,7 +1153,14 @@ struct Lane {
         pend_i = ~(size_t)0;
         tally64 = kp.field64;
         if (TALLY == MCX_TALLY_F64_XCD) tally64 += (size_t)xcc_id() * kp.fieldlen;
-        nstep = ndep = nscat = nlaunch = nhot = 0;
+        bopen = MCX_NO_BLOCK;
+        if (BLOCKED && kp.tblock) {
+            extern __shared__ F4 lds[];
+            blk = (float*)((char*)lds + kp.blk_off) + 9u * threadIdx.x;
+            for (uint32_t c = 0; c < 8; c++) blk[c] = 0.f;
+            blk[8] = mcx_bits2f(MCX_NO_BLOCK);
+        }
+        nstep = ndep = nscat = nlaunch = nhot = nreq = 0;
         svsplit = svlo = svup = svupper = 0;
         rpx = rpy = rpz = nvx = nvy = nvz = 0.f;
         testint = hitintf = false;
@@ -1056,6 +1177,7 @@ struct Lane {
         bool first = true, relaunch = true;
         uint32_t detarg = 0;
         while (true) {
+            block_flush();
             if (relaunch) {
                 if (launch(detarg)) {
                     if (first) return;
@@ -1109,17 +1231,28 @@ struct Lane {
                     v.w += 1.f;
                     calc_rv();
                     nscat++;
-                    /* src/mcx_core.cu:2019-2047: replayed photons tally their
-                     * detected weight at every scattering event */
-                    if (kp.replay && (kp.outputtype == otWP || kp.outputtype == otDCS || kp.outputtype == otWPTOF)) {
+                    /* src/mcx_core.cu:2019-2062: replayed photons tally their
+                     * detected weight at every scattering event (otRFmus:
+                     * times the modulation phasor of their time of flight,
+                     * real and imaginary planes) */
+                    if (kp.replay && (kp.outputtype == otWP || kp.outputtype == otDCS || kp.outputtype == otWPTOF ||
+                                      kp.outputtype == otRFmus)) {
                         const float tof = glb(kp.replaytof)[gid];
                         int ts = (int)floorf((tof - kp.twin0) * kp.Rtstep);
                         ts = max(0, min((int)kp.maxgate - 1, ts));
                         if (kp.replaydet == -1) ts += ((glb(kp.replaydetid)[gid] & 0xFFFF) - 1) * (int)kp.maxgate;
-                        float w = (kp.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
-                        w = (kp.outputtype == otWPTOF) ? tof : w;
-                        w *= glb(kp.replayweight)[gid];
-                        deposit(idx1d, ts, w);
+                        if (kp.outputtype == otRFmus) {
+                            float rs, rc;
+                            mcx_sincosf(kp.omega * tof, &rs, &rc);
+                            const float rw = glb(kp.replayweight)[gid];
+                            deposit(idx1d, ts, rw * rc);
+                            if (kp.omega > 0.f) deposit(idx1d, ts + kp.rfshift, rw * rs);
+                        } else {
+                            float w = (kp.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
+                            w = (kp.outputtype == otWPTOF) ? tof : w;
+                            w *= glb(kp.replayweight)[gid];
+                            deposit(idx1d, ts, w);
+                        }
                     }
                 }
                 v.w = (float)(int)v.w;
@@ -1160,25 +1293,37 @@ struct Lane {
             const bool moved = ((idx1d != idx1dold) || (SPLIT && hitintf)) && mediaidold;
             if (moved) {
                 if (kp.save2pt && f.y >= kp.twin0 && f.y < kp.twin1) {
-                    float weight = 0.f;
+                    float weight = 0.f, imag = 0.f;
                     int tshift = (int)floorf((f.y - kp.twin0) * kp.Rtstep);
                     if (tshift > (int)kp.maxgate - 1) tshift = (int)kp.maxgate - 1;
                     if (kp.outputtype == otEnergy)
                         weight = w0 - p.w;
                     else if (kp.outputtype == otFluence || kp.outputtype == otFlux)
                         weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
-                    else if (kp.replay && (kp.outputtype == otJacobian || kp.outputtype == otWLTOF)) {
+                    else if (kp.replay && (kp.outputtype == otJacobian || kp.outputtype == otRF || kp.outputtype == otWLTOF)) {
                         /* src/mcx_core.cu:2208-2220 (the gate index is clamped
                          * to the gate range, as the scattering tally clamps it
                          * to the last gate) */
                         const float tof = glb(kp.replaytof)[gid];
                         weight = glb(kp.replayweight)[gid] * f.z;
-                        if (kp.outputtype == otWLTOF) weight = weight * tof;
+                        if (kp.outputtype == otRF) {
+                            /* src/mcx_core.cu:2211-2213, :2254-2257, :2271-2273:
+                             * real part -w L cos(omega tof), imaginary part
+                             * -w L sin(omega tof), or p.w L sin(omega tof)
+                             * for pattern sources */
+                            float rs, rc;
+                            mcx_sincosf(kp.omega * tof, &rs, &rc);
+                            imag = (kp.srctype == 5 || kp.srctype == 15) ? p.w * f.z * rs : -weight * rs;
+                            weight = -weight * rc;
+                        } else if (kp.outputtype == otWLTOF) {
+                            weight = weight * tof;
+                        }
                         tshift = max(0, min((int)kp.maxgate - 1, (int)floorf((tof - kp.twin0) * kp.Rtstep)));
                         if (kp.replaydet == -1) tshift += ((glb(kp.replaydetid)[gid] & 0xFFFF) - 1) * (int)kp.maxgate;
                     } else if (kp.outputtype == otL)
                         weight = w0 * f.z;
-                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
+                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight, true);
+                    if (kp.rfshift && kp.omega > 0.f && fabsf(imag) > 0.f) deposit(idx1dold, tshift + kp.rfshift, imag);
                 }
                 w0 = p.w;
                 f.z = 0.f;
@@ -1369,7 +1514,9 @@ __device__ __forceinline__ void hot_box_flush(const KParam& kp, uint32_t off, ui
     for (uint32_t c = t; c < kp.hot_cells; c += n) {
         const uint32_t g = c / vol, r = c - g * vol, lz = r / plane, r2 = r - lz * plane, ly = r2 / kp.hot_bx;
         const uint32_t lx = r2 - ly * kp.hot_bx;
-        const size_t i = (size_t)((kp.hot_z0 + lz) * kp.dimxy + (kp.hot_y0 + ly) * kp.nx + kp.hot_x0 + lx) + (size_t)g * kp.nvox;
+        const size_t i = (TALLY == MCX_TALLY_F64 && kp.tblock)
+                             ? block_cell(kp, kp.hot_x0 + lx, kp.hot_y0 + ly, kp.hot_z0 + lz, g)
+                             : (size_t)((kp.hot_z0 + lz) * kp.dimxy + (kp.hot_y0 + ly) * kp.nx + kp.hot_x0 + lx) + (size_t)g * kp.nvox;
         if (TALLY == MCX_TALLY_F32_HILO) {
             const float w = ((const float*)base)[c];
             const float lo = ((const float*)base)[kp.hot_cells + c];
@@ -1636,6 +1783,22 @@ __global__ void mcx_merge_f64(float* __restrict__ out, const double* __restrict_
     for (; i < n; i += stride) out[i] = (float)d[i];
 }
 
+/* blocked FP64 tally (KParam::tblock) -> FP32 volume in the reference's
+ * [gate][z][y][x] order; one lane per output cell */
+__global__ void mcx_merge_f64_blocked(float* __restrict__ out, const double* __restrict__ d, uint32_t nx, uint32_t ny,
+                                      uint32_t nz, uint32_t bnx, uint32_t bny, uint32_t bnz, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t nvox = (size_t)nx * ny * nz;
+    for (; i < n; i += stride) {
+        const uint32_t g = (uint32_t)(i / nvox);
+        const uint32_t r = (uint32_t)(i - (size_t)g * nvox);
+        const uint32_t z = r / (nx * ny), r2 = r - z * nx * ny, y = r2 / nx, x = r2 - y * nx;
+        const size_t b = (((size_t)g * bnz + (z >> 1)) * bny + (y >> 1)) * bnx + (x >> 1);
+        out[i] = (float)d[(b << 3) + (x & 1u) + ((y & 1u) << 1) + ((z & 1u) << 2)];
+    }
+}
+
 /* ---- host-side dispatch (called from mcx_host.cpp) ------------------- */
 
 typedef void (*mcx_kernel_fn)(const KParam*);
@@ -1743,6 +1906,15 @@ extern "C" hipError_t mcxk_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream
 
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream) {
     hipLaunchKernelGGL(mcx_test_rng, dim3(1), dim3(64), 0, stream, out, seeds, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t mcxk_launch_merge_blocked(float* out, const double* a, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                 uint32_t bnx, uint32_t bny, uint32_t bnz, size_t n, hipStream_t stream) {
+    const unsigned int block = 256;
+    size_t g = (n + block - 1) / block;
+    unsigned int grid = (unsigned int)(g > 65536 ? 65536 : (g ? g : 1));
+    hipLaunchKernelGGL(mcx_merge_f64_blocked, dim3(grid), dim3(block), 0, stream, out, a, nx, ny, nz, bnx, bny, bnz, n);
     return hipGetLastError();
 }
 

@@ -14,9 +14,11 @@
  *         s = (m-1)/(m+1), odd series to s^9.
  *   sincos(2 pi u): quadrant split of 4u, Taylor series to a^9 / a^10 on
  *         |a| <= pi/4.
+ *   sincos(x): Cody-Waite reduction by pi/2 (three-part split), the same
+ *         quadrant series.
  *
  * These stand in for CUDA's fast-math expf/logf/sincosf used by the
- * reference (src/mcx_core.cu:2140, :1900, :1932, :1958-1963).
+ * reference (src/mcx_core.cu:2140, :1900, :1932, :1958-1963, :1702).
  */
 #ifndef MCX_AMD_MATH_H
 #define MCX_AMD_MATH_H
@@ -79,12 +81,8 @@ MCX_HD float mcx_logf(float x) {
     return __builtin_fmaf(fe, MCX_LN2_HI, __builtin_fmaf(fe, MCX_LN2_LO, r));
 }
 
-MCX_HD void mcx_sincos2pi(float u, float* s, float* c) {
-    float x = u * 4.0f;
-    float q = __builtin_rintf(x);
-    float f = x - q;
-    int qi = ((int)q) & 3;
-    float a = f * 1.57079632679489662f;
+/* sin and cos of a, |a| <= pi/4, rotated by the quadrant qi (0..3) */
+MCX_HD void mcx_sincos_quadrant(float a, int qi, float* s, float* c) {
     float a2 = a * a;
     float ps = 1.0f / 362880.0f;
     ps = __builtin_fmaf(ps, a2, -1.0f / 5040.0f);
@@ -101,6 +99,29 @@ MCX_HD void mcx_sincos2pi(float u, float* s, float* c) {
     float cc = (qi & 1) ? sn : cs;
     *s = (qi & 2) ? -ss : ss;
     *c = ((qi + 1) & 2) ? -cc : cc;
+}
+
+MCX_HD void mcx_sincos2pi(float u, float* s, float* c) {
+    float x = u * 4.0f;
+    float q = __builtin_rintf(x);
+    float f = x - q;
+    mcx_sincos_quadrant(f * 1.57079632679489662f, ((int)q) & 3, s, c);
+}
+
+/* sin and cos of any float x with |x| < 2^12 pi/2 (larger arguments keep
+ * |s|, |c| <= 1 but lose accuracy): Cody-Waite reduction by pi/2 split in
+ * three parts (8, 12 and 24 significant bits; q * hi and q * mid are exact
+ * for |q| < 2^12), then the quadrant series. */
+#define MCX_PIO2_HI 1.5703125f
+#define MCX_PIO2_MID 4.837512969970703125e-04f
+#define MCX_PIO2_LO 7.549790126404332e-08f
+
+MCX_HD void mcx_sincosf(float x, float* s, float* c) {
+    float q = __builtin_rintf(x * 0.636619772367581343f);
+    float r = __builtin_fmaf(-q, MCX_PIO2_HI, x);
+    r = __builtin_fmaf(-q, MCX_PIO2_MID, r);
+    r = __builtin_fmaf(-q, MCX_PIO2_LO, r);
+    mcx_sincos_quadrant(r, ((int)q) & 3, s, c);
 }
 
 /* sin(theta) from cos(theta) for theta in [0, pi] */

@@ -31,6 +31,7 @@ OUTPUTTYPES = ["flux", "fluence", "energy", "jacobian", "nscat", "wl", "wp", "wm
 SAVEFLAGS = "DSPMXVWI"
 SEEDBYTE = 16               # bytes of one saved xorshift128+ state
 MCX_SEED_FROM_FILE = -999
+OT_RF, OT_RFMUS = 6, 8  # TOutputType values of the RF replay outputs
 DEBUGFLAGS = "RMP"
 
 _INT_FIELDS = ["nphoton", "nblocksize", "nthread", "maxdetphoton", "maxgate", "respin", "isreflect", "isref3",
@@ -259,10 +260,14 @@ def run(cfg=None, **kwargs):
         ngates = int((c.tend - c.tstart) / c.tstep + 0.5)
         nvox = c.dim.x * c.dim.y * c.dim.z
         reclen = _reclen(c)
-        # per-detector replay tallies carry one block of gates per detector
-        blocks = c.detnum if (c.seed == MCX_SEED_FROM_FILE and c.replaydet == -1 and c.detnum > 1) else 1
+        # per-detector replay tallies carry one block of gates per detector;
+        # RF replay adds an imaginary plane (src/pmcx.cpp:1115-1121)
+        replay = c.seed == MCX_SEED_FROM_FILE
+        blocks = c.detnum if (replay and c.replaydet == -1 and c.detnum > 1) else 1
+        planes = 2 if (replay and c.outputtype in (OT_RF, OT_RFMUS)) else 1
         if c.issave2pt:
-            c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates * c.srcnum * blocks, 4), ctypes.POINTER(ctypes.c_float))
+            c.exportfield = ctypes.cast(_libc.calloc(nvox * ngates * c.srcnum * blocks * planes, 4),
+                                        ctypes.POINTER(ctypes.c_float))
         if c.issavedet:
             c.exportdetected = ctypes.cast(_cmalloc(reclen * c.maxdetphoton * 4), ctypes.POINTER(ctypes.c_float))
         if user_cfg.get("dumpmask", False):
@@ -283,8 +288,13 @@ def run(cfg=None, **kwargs):
             else:
                 output["seeds"] = np.zeros((SEEDBYTE, 0), dtype=np.uint8, order="F")
         if c.issave2pt:
-            f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum * blocks,)).copy()
-            shape = (c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates) + ((blocks,) if blocks > 1 else ())
+            f = np.ctypeslib.as_array(c.exportfield, shape=(nvox * ngates * c.srcnum * blocks * planes,)).copy()
+            # src/pmcx.cpp:1243-1262: [x, y, z, gates(, detectors)(, re/im)]
+            d4 = c.detnum if (replay and c.replaydet == -1) else 1
+            if planes > 1:
+                shape = (c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates, d4, planes)
+            else:
+                shape = (c.srcnum * c.dim.x, c.dim.y, c.dim.z, ngates) + ((d4,) if d4 > 1 else ())
             if c.issaveref:
                 # src/pmcx.cpp:1268-1289: void voxels hold the (negated)
                 # escaping weight; split them out as 'dref'

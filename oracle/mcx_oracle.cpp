@@ -66,6 +66,20 @@ enum { otFlux, otFluence, otEnergy, otJacobian, otWP, otDCS, otRF, otL, otRFmus,
  * between runs, never during one. */
 int g_mut = 0;
 
+/* The RF replay's modulation phasor sin, cos of x = omega * tof: the loop
+ * and mcx_oracle_sincosf both call this.  Mutation 21 drops the last
+ * part of the pi/2 reduction. */
+inline void rf_sincos(float x, float* s, float* c) {
+    if (g_mut != 21) {
+        om_sincosf(x, s, c);
+        return;
+    }
+    const float q = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-q, 1.5703125f, x);
+    r = fmaf(-q, 4.837512969970703125e-04f, r);
+    om_sincos_series(r, ((int)q) & 3, s, c);
+}
+
 /* ---- glibc rand(): TYPE_3 additive feedback, r[i] = r[i-3] + r[i-31] --- */
 struct GlibcRand {
     int32_t r[34];
@@ -133,6 +147,8 @@ struct Ctx {
     unsigned int dimlen_x, dimlen_y, dimlen_z;
     float maxidx_x, maxidx_y, maxidx_z;
     unsigned int maxmedia, partialdata, w0offset, reclen;
+    size_t shadowoff;     /* FP32 tally: offset of the MAX_ACCUM shadow half */
+    int rfshift;          /* RF replay: gates from the real plane to the imaginary one; 0 otherwise */
     std::atomic<unsigned int>* detected;
     const Vec4* gprop;   /* media then detectors */
 };
@@ -595,8 +611,11 @@ struct Thread {
             for (unsigned int i = 0; i < C->partialdata; i++) ppath[i] = 0.f;
         if ((int)f.w >= nphoton - 1) return 1;
         /* src/mcx_core.cu:1205-1215, with the photon index counted from the
-         * thread's first photon idx*threadphoton + min(idx, oddphotons) */
-        gid = idx * (unsigned)Q.threadphoton + std::min(idx, (unsigned)Q.oddphotons) + (unsigned)((int)f.w + 1);
+         * thread's first photon idx*threadphoton + min(idx, oddphotons), or
+         * (replayindex 1) from idx*threadphoton + min(idx, oddphotons - 1) */
+        gid = idx * (unsigned)Q.threadphoton + (Q.replayindex ? (unsigned)std::min((int)idx, Q.oddphotons - 1)
+                                                              : std::min(idx, (unsigned)Q.oddphotons)) +
+              (unsigned)((int)f.w + 1);
         if (Q.replayseed) t.setstate(Q.replayseed + (size_t)gid * 4);
         if (Q.saveseed) t.getstate(seedsave);
         float* pp = ppath + C->partialdata;
@@ -882,7 +901,6 @@ struct Thread {
     }
 
     void deposit(unsigned int vox, int tshift, float weight) {
-        const OracleParams& Q = P();
         size_t nvox = (size_t)C->dimlen_z;
         size_t i = (size_t)vox + (size_t)tshift * nvox;
         ndep++;
@@ -892,7 +910,7 @@ struct Thread {
             float oldval = atomic_add_float_ret(fld + i, weight);
             if (fabsf(oldval) > MAX_ACCUM) {
                 atomic_add_float_ret(fld + i, (oldval > 0.f) ? -MAX_ACCUM : MAX_ACCUM);
-                atomic_add_float_ret(fld + i + nvox * Q.maxgate * (Q.replayseed && Q.replaydet == -1 ? Q.detnum : 1),
+                atomic_add_float_ret(fld + i + C->shadowoff,
                                      (oldval > 0.f) ? MAX_ACCUM : -MAX_ACCUM);
             }
         }
@@ -946,16 +964,28 @@ struct Thread {
                     v.w += 1.f;
                     rv[0] = 1.f / v.x; rv[1] = 1.f / v.y; rv[2] = 1.f / v.z;
                     nscat++;
-                    /* src/mcx_core.cu:2019-2047 */
-                    if (Q.replayseed && (Q.outputtype == otWP || Q.outputtype == otDCS || Q.outputtype == otWPTOF)) {
+                    /* src/mcx_core.cu:2019-2062 */
+                    if (Q.replayseed && (Q.outputtype == otWP || Q.outputtype == otDCS || Q.outputtype == otWPTOF ||
+                                         Q.outputtype == otRFmus)) {
                         const float tof = Q.replaytof[gid];
                         int ts = (int)floorf((tof - Q.twin0) * Q.Rtstep);
                         ts = std::max(0, std::min((int)Q.maxgate - 1, ts));
                         if (Q.replaydet == -1) ts += ((Q.replaydetid[gid] & 0xFFFF) - 1) * (int)Q.maxgate;
-                        float w = (Q.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
-                        w = (Q.outputtype == otWPTOF) ? tof : w;
-                        w *= Q.replayweight[gid];
-                        deposit(idx1d, ts, w);
+                        if (Q.outputtype == otRFmus) {
+                            /* the detected weight times the modulation phasor
+                             * of the photon's time of flight */
+                            float rs, rc;
+                            rf_sincos(Q.omega * tof, &rs, &rc);
+                            if (g_mut == 20) std::swap(rs, rc);
+                            const float rw = Q.replayweight[gid];
+                            deposit(idx1d, ts, rw * rc);
+                            if (Q.omega > 0.f) deposit(idx1d, ts + c.rfshift, rw * rs);
+                        } else {
+                            float w = (Q.outputtype == otDCS) ? (1.f - ctheta) : 1.f;
+                            w = (Q.outputtype == otWPTOF) ? tof : w;
+                            w *= Q.replayweight[gid];
+                            deposit(idx1d, ts, w);
+                        }
                     }
                 }
                 v.w = (float)(int)v.w;
@@ -993,23 +1023,36 @@ struct Thread {
             }
             if ((idx1d != idx1dold || hitintf) && mediaidold) {
                 if (Q.save2pt && f.y >= Q.twin0 && f.y < Q.twin1) {
-                    float weight = 0.f;
+                    float weight = 0.f, imag = 0.f;
                     int tshift = (int)floorf((f.y - Q.twin0) * Q.Rtstep);
                     if (tshift > (int)Q.maxgate - 1) tshift = (int)Q.maxgate - 1;
                     if (Q.outputtype == otEnergy)
                         weight = w0 - p.w;
                     else if (Q.outputtype == otFluence || Q.outputtype == otFlux)
                         weight = (prop.x < EPSF) ? (w0 * f.z) : ((w0 - p.w) / prop.x);
-                    else if (Q.replayseed && (Q.outputtype == otJacobian || Q.outputtype == otWLTOF)) {
+                    else if (Q.replayseed && (Q.outputtype == otJacobian || Q.outputtype == otRF || Q.outputtype == otWLTOF)) {
                         /* src/mcx_core.cu:2208-2220, gate clamped to the range */
                         const float tof = Q.replaytof[gid];
                         weight = Q.replayweight[gid] * f.z;
-                        if (Q.outputtype == otWLTOF) weight = weight * tof;
+                        if (Q.outputtype == otRF) {
+                            /* src/mcx_core.cu:2211-2213, :2254-2257, :2271-2273:
+                             * real part -w L cos(omega tof); imaginary part
+                             * -w L sin(omega tof), or p.w L sin(omega tof)
+                             * for pattern sources */
+                            float rs, rc;
+                            rf_sincos(Q.omega * tof, &rs, &rc);
+                            if (g_mut == 20) std::swap(rs, rc);
+                            imag = (Q.srctype == 5 || Q.srctype == 15) ? p.w * f.z * rs : -weight * rs;
+                            weight = -weight * rc;
+                        } else if (Q.outputtype == otWLTOF) {
+                            weight = weight * tof;
+                        }
                         tshift = std::max(0, std::min((int)Q.maxgate - 1, (int)floorf((tof - Q.twin0) * Q.Rtstep)));
                         if (Q.replaydet == -1) tshift += ((Q.replaydetid[gid] & 0xFFFF) - 1) * (int)Q.maxgate;
                     } else if (Q.outputtype == otL)
                         weight = w0 * f.z;
                     if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight);
+                    if (c.rfshift && Q.omega > 0.f && fabsf(imag) > 0.f) deposit(idx1dold, tshift + c.rfshift, imag);
                 }
                 w0 = p.w;
                 f.z = 0.f;
@@ -1154,6 +1197,7 @@ extern "C" {
 float mcx_oracle_expf(float x) { return om_expf(x); }
 float mcx_oracle_logf(float x) { return om_logf(x); }
 void mcx_oracle_sincos2pi(float u, float* s, float* c) { om_sincos2pi(u, s, c); }
+void mcx_oracle_sincosf(float x, float* s, float* c) { rf_sincos(x, s, c); }
 
 void mcx_oracle_glibc_rand(unsigned int seed, unsigned int n, int* out) {
     GlibcRand g(seed);
@@ -1252,6 +1296,13 @@ int init_ctx(const OracleParams* P, OracleOut* O, Ctx& c, std::vector<Vec4>& gpr
     c.w0offset = c.partialdata + 4;
     c.reclen = c.partialdata + (f & 1) + 3 * (((f >> 4) & 1) + ((f >> 5) & 1)) + ((f >> 6) & 1);
     O->reclen = c.reclen;
+    const unsigned int blocks = (P->replayseed && P->replaydet == -1) ? P->detnum : 1;
+    const bool rf = P->replayseed && (P->outputtype == otRF || P->outputtype == otRFmus);
+    c.rfshift = rf ? (int)(P->maxgate * blocks) : 0;
+    c.shadowoff = (size_t)c.dimlen_z * P->maxgate * blocks * (rf ? 2 : 1);
+    /* the reference's numbering reads the photon before the first when no
+     * thread takes an extra photon */
+    if (P->replayseed && P->replayindex && P->oddphotons == 0 && P->threadphoton > 0) return -3;
     gprop.assign(P->medianum + P->detnum, Vec4{0.f, 0.f, 0.f, 0.f});
     for (unsigned int i = 0; i < P->medianum; i++) gprop[i] = {P->prop[4 * i], P->prop[4 * i + 1], P->prop[4 * i + 2], P->prop[4 * i + 3]};
     for (unsigned int i = 0; i < P->detnum; i++)

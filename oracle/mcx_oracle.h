@@ -39,7 +39,7 @@ typedef struct OracleParams {
     int doreflect;               /* cfg.isreflect */
     int isspecular, voidtime, maxvoidstep;
     unsigned int gscatter;
-    int outputtype;              /* otFlux / otFluence / otEnergy / otL; replay: otJacobian, otWP, otDCS, otWLTOF, otWPTOF */
+    int outputtype;              /* otFlux / otFluence / otEnergy / otL; replay: otJacobian, otWP, otDCS, otRF, otRFmus, otWLTOF, otWPTOF */
     int save2pt;
     unsigned char bc[12];        /* numeric bc codes and det flags */
     int is2d;
@@ -62,11 +62,21 @@ typedef struct OracleParams {
     const float* srcpattern;     /* pattern / pattern3d source intensities (one pattern), or NULL */
     int issaveref;               /* diffuse reflectance into void voxels (src/mcx_core.cu:1124-1145) */
     int mediaformat;             /* 0-4: labels; 96-104: property words (96, 97: 2 words per voxel) */
+    /* RF replay (otRF, otRFmus; src/mcx_core.cu:1700-1703, :2033-2062,
+     * :2208-2213, :2254-2257): modulation angular frequency in rad/s; the
+     * tally then holds a real plane and an imaginary plane of
+     * maxgate x blocks gates each */
+    float omega;
+    /* replayed-photon numbering: 0 = contiguous, idx*threadphoton +
+     * min(idx, oddphotons) + k; 1 = the reference's idx*threadphoton +
+     * min(idx, oddphotons - 1) + k (src/mcx_core.cu:1206, :1701, :2021) */
+    int replayindex;
 } OracleParams;
 
 typedef struct OracleOut {
-    double* field;        /* maxgate*nx*ny*nz FP64 tally, or NULL */
-    float*  field32;      /* 2*maxgate*nx*ny*nz FP32 tally + MAX_ACCUM shadow, or NULL */
+    double* field;        /* planes*blocks*maxgate*nx*ny*nz FP64 tally, or NULL (planes 2 for RF, blocks =
+                             detnum for replaydet -1) */
+    float*  field32;      /* twice that: FP32 tally + MAX_ACCUM shadow, or NULL */
     float*  genergy;      /* 2*nthread: escaped, launched (must be pre-zeroed) */
     float*  detdata;      /* maxdetphoton*reclen records, or NULL */
     unsigned int detected;
@@ -102,6 +112,7 @@ void mcx_oracle_maskdet(unsigned int* vol, unsigned int nx, unsigned int ny, uns
 float mcx_oracle_expf(float x);
 float mcx_oracle_logf(float x);
 void  mcx_oracle_sincos2pi(float u, float* s, float* c);
+void  mcx_oracle_sincosf(float x, float* s, float* c);
 
 /* The transport loop's own pieces, exposed so tests can pin each one
  * against closed forms (the loop calls exactly these functions):
@@ -138,7 +149,8 @@ int mcx_oracle_launch_samples(const OracleParams* P, unsigned int n, float* out)
  *  scale sqrt(E) instead of sqrt(2E);  14 line fan one-sided;  15 slit
  *  broadening widths swapped;  16 hyperboloid focus on the wrong side;
  *  17 pattern3d x/y indices transposed;  18 arcsine zenith over [0, pi/2];
- *  19 ring radius uniform instead of area-uniform.
+ *  19 ring radius uniform instead of area-uniform;  20 RF phasor with sin
+ *  and cos exchanged;  21 sincos(x) reduction without its last pi/2 part.
  * Global to the library: set between runs only. */
 void  mcx_oracle_set_mutation(int id);
 

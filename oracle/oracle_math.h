@@ -15,9 +15,12 @@
  *            (|s| <= 0.1716, truncation < 2e-9 rel.).
  *   sincos(2 pi u): u*4 = q + f, |f| <= 1/2, a = f*pi/2 in [-pi/4, pi/4],
  *            sin/cos by Taylor series to a^9 / a^10, quadrant q selects.
+ *   sincos(x): q = rint(x * 2/pi), r = x - q*pi/2 with pi/2 split in three
+ *            parts (Cody-Waite; 8 + 12 + 24 significant bits), the same
+ *            series on r, quadrant q selects.
  *
  * The reference calls CUDA's fast-math expf/logf/sincosf/acosf
- * (src/mcx_core.cu:2140, :1900, :1932, :1958-1963, built with
+ * (src/mcx_core.cu:2140, :1900, :1932, :1958-1963, :1702, built with
  * -use_fast_math, src/Makefile:194); those are approximations whose bits
  * are unpinned.  The oracle only needs functions accurate to a few ulp:
  * tests/test_oracle_math.py checks these against libm.
@@ -81,13 +84,8 @@ static inline float om_logf(float x) {
     return fmaf(fe, OM_LN2_HI, fmaf(fe, OM_LN2_LO, r));
 }
 
-/* sin(2 pi u), cos(2 pi u) */
-static inline void om_sincos2pi(float u, float* s, float* c) {
-    float x = u * 4.0f;
-    float q = rintf(x);
-    float f = x - q;
-    int qi = ((int)q) & 3;
-    float a = f * 1.57079632679489662f;
+/* sin(a), cos(a) for |a| <= pi/4, then rotated by quadrant qi */
+static inline void om_sincos_series(float a, int qi, float* s, float* c) {
     float a2 = a * a;
     float ps = 1.0f / 362880.0f;
     ps = fmaf(ps, a2, -1.0f / 5040.0f);
@@ -106,6 +104,23 @@ static inline void om_sincos2pi(float u, float* s, float* c) {
         case 2: *s = -sn; *c = -cs; break;
         default: *s = -cs; *c = sn; break;
     }
+}
+
+/* sin(2 pi u), cos(2 pi u) */
+static inline void om_sincos2pi(float u, float* s, float* c) {
+    float x = u * 4.0f;
+    float q = rintf(x);
+    float f = x - q;
+    om_sincos_series(f * 1.57079632679489662f, ((int)q) & 3, s, c);
+}
+
+/* sin(x), cos(x): pi/2 = 1.5703125 + 4.837512969970703125e-4 + 7.549790126404332e-8 */
+static inline void om_sincosf(float x, float* s, float* c) {
+    const float q = rintf(x * 0.636619772367581343f);
+    float r = fmaf(-q, 1.5703125f, x);
+    r = fmaf(-q, 4.837512969970703125e-04f, r);
+    r = fmaf(-q, 7.549790126404332e-08f, r);
+    om_sincos_series(r, ((int)q) & 3, s, c);
 }
 
 /* sin of an angle whose cosine is c, c in [-1,1] */

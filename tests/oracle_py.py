@@ -58,6 +58,8 @@ class OracleParams(ctypes.Structure):
         ("srcpattern", ctypes.POINTER(ctypes.c_float)),
         ("issaveref", ctypes.c_int),
         ("mediaformat", ctypes.c_int),
+        ("omega", ctypes.c_float),
+        ("replayindex", ctypes.c_int),
     ]
 
 
@@ -103,6 +105,7 @@ def lib():
         L.mcx_oracle_logf.argtypes = [ctypes.c_float]
         L.mcx_oracle_logf.restype = ctypes.c_float
         L.mcx_oracle_sincos2pi.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.mcx_oracle_sincosf.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
         for name in ("mcx_oracle_hg_costheta",):
             getattr(L, name).argtypes = [ctypes.c_float, ctypes.c_float]
             getattr(L, name).restype = ctypes.c_float
@@ -275,7 +278,7 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         maxdetphoton=1000000, minenergy=0.0, bc="", gscatter=1000000000, field32=False,
         thread_range=None, ompthreads=0, maxgate=None, seeds=None, voidtime=1, want_field=True,
         saveseed=0, replay=None, replaydet=0, srcpattern=None, issaveref=0, mediaformat=0, words2=None,
-        thread_steps=False, genergy=None):
+        thread_steps=False, genergy=None, omega=0.0, replayindex=0):
     """Run the oracle on a preprocessed problem.
 
     vol: uint8/uint32 labels indexed [x,y,z]; prop: (M,4) mua/mus/g/n per mm;
@@ -284,7 +287,10 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     saveseed: also return the launch RNG state of each detected photon
     ("seeds", uint8 [n, 16]).  replay: dict with "seed" (uint8 [n, 16]),
     "weight", "tof", "detid" per replayed photon (n = nphoton); with
-    replaydet == -1 the field has one block of gates per detector.
+    replaydet == -1 the field has one block of gates per detector; the RF
+    outputs (6 otRF, 8 otRFmus) add an imaginary plane of as many gates
+    after the real one (omega: modulation angular frequency, rad/s).
+    replayindex 1 numbers replayed photons as the reference does.
     mediaformat > 4: vol holds the 32-bit property words of that format
     (words2: the second plane of the two-word formats 96 and 97).
     """
@@ -391,6 +397,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
     P.saveseed = int(saveseed)
     P.issaveref = int(issaveref)
     P.mediaformat = int(mediaformat)
+    P.omega = float(omega)
+    P.replayindex = int(replayindex)
     keep = []
     if srcpattern is not None:
         pat = np.ascontiguousarray(np.asarray(srcpattern, dtype=np.float32).ravel(order="F"))
@@ -410,6 +418,8 @@ def run(vol, prop, srcpos, srcdir, nphoton, nthread, seed, tstart=0.0, tend=5e-9
         P.replaydet = int(replaydet)
         if replaydet == -1:
             blocks = max(detnum, 1)
+    if replay is not None and outputtype in (6, 8):
+        blocks *= 2
     nvox = nx * ny * nz
     field = np.zeros(nvox * maxgate * blocks, dtype=np.float64) if want_field else None
     f32 = np.zeros(2 * nvox * maxgate * blocks, dtype=np.float32) if field32 else None

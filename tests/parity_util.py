@@ -22,8 +22,8 @@ def oracle_run(cfg, nthread, **extra):
     srctype = srctypes.index(st) if isinstance(st, str) else int(st)
     det = cfg.get("detpos")
     issavedet = int(cfg.get("issavedet", 1 if det is not None else 0))
-    ot = {"flux": 0, "fluence": 1, "energy": 2, "jacobian": 3, "wl": 3, "wp": 4, "nscat": 4, "wm": 5, "length": 7,
-          "wltof": 9, "wptof": 10}[cfg.get("outputtype", "flux")]
+    ot = {"flux": 0, "fluence": 1, "energy": 2, "jacobian": 3, "wl": 3, "wp": 4, "nscat": 4, "wm": 5, "rf": 6,
+          "length": 7, "rfmus": 8, "wltof": 9, "wptof": 10}[cfg.get("outputtype", "flux")]
     kw = dict(
         tstart=cfg["tstart"], tend=cfg["tend"], tstep=cfg["tstep"], srctype=srctype,
         srcparam1=cfg.get("srcparam1", (0, 0, 0, 0)), srcparam2=cfg.get("srcparam2", (0, 0, 0, 0)),
@@ -35,6 +35,7 @@ def oracle_run(cfg, nthread, **extra):
         bc=cfg.get("bc", ""),
         srcpattern=cfg.get("srcpattern"),
         issaveref=int(cfg.get("issaveref", 0)),
+        omega=float(cfg.get("omega", 0.0)),
     )
     vol = cfg["vol"]
     if np.asarray(vol).ndim == 4 or int(cfg.get("mediabyte", 0)) > 4:

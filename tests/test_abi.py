@@ -181,6 +181,8 @@ def test_readconfig_json_and_shapes():
     assert (c.srcpos.x, c.srcpos.y, c.srcpos.z) == (29, 29, 0) and c.issrcfrom0 == 1
     assert (c.detpos[1].x, c.detpos[1].y, c.detpos[1].w) == (29, 39, 1)
     assert c.session == b"cube60b" and c.maxgate == 1
+    # the detected-photon file header (src/mcx_utils.c:3203-3207)
+    assert (c.his.maxmedia, c.his.detnum, c.his.srcnum, c.his.savedetflag) == (2, 2, 1, c.savedetflag)
     v = np.ctypeslib.as_array(c.vol, shape=(216000,)).reshape((60, 60, 60), order="F") & 0x7FFFFFFF
     c_ = np.arange(60) + 0.5 - 30
     inside = (c_[:, None, None] ** 2 + c_[None, :, None] ** 2 + c_[None, None, :] ** 2) <= 100
@@ -249,6 +251,87 @@ def test_mch_writer_header_matches_reference_format(tmp_path):
     assert len(b) == 64 + rows.nbytes
     assert b[:64] == fixture
     assert np.array_equal(np.frombuffer(b[64:], dtype="<f4").reshape(4, 6), rows)
+
+
+def _jdata(obj):
+    raw = zlib.decompress(base64.b64decode(obj["_ArrayZipData_"]))
+    dt = {"uint32": "<u4", "single": "<f4", "uint8": "u1"}[obj["_ArrayType_"]]
+    return np.frombuffer(raw, dtype=dt).reshape(obj["_ArraySize_"])
+
+
+def test_jdat_writer_matches_mch_columns(tmp_path):
+    """_detp.jdat (src/mcx_utils.c:1029-1201): one zlib+base64 block per
+    saved column group, named detid/nscat/ppath/mom/p/v/w0/s, uint32 for
+    the detector id and scattering counts and single otherwise, each of
+    size [count, columns]; the seeds as uint8 [count, seedbyte]; the same
+    records written as .mch carry the same numbers."""
+    c = _cfg()
+    c.session = str(tmp_path / "j").encode()
+    c.medianum = 3
+    props = (_lib.Medium * 3)()
+    for i, (mua, mus, g, n) in enumerate([(0, 0, 1, 1), (0.01, 1.0, 0.9, 1.37), (0.02, 2.0, 0.8, 1.4)]):
+        props[i].mua, props[i].mus, props[i].g, props[i].n = mua, mus, g, n
+    c.prop = ctypes.cast(props, ctypes.POINTER(_lib.Medium))
+    c.unitinmm = 1.0
+    c.savedetflag = 0xFF
+    mm = 2
+    cols = [1, mm, mm, mm, 3, 3, 1, 4]
+    ncol = sum(cols)
+    c.his.maxmedia, c.his.detnum, c.his.colcount = mm, 2, ncol
+    c.his.totalphoton, c.his.detected, c.his.savedphoton = 1000, 3, 3
+    c.his.seedbyte, c.his.unitinmm, c.his.savedetflag, c.his.totalsource = 16, 1.0, 0xFF, 1
+    c.issaveseed = 1
+    rng = np.random.default_rng(5)
+    rows = rng.uniform(0.1, 5.0, size=(3, ncol)).astype(np.float32)
+    rows[:, 0] = [1, 2, 1]
+    rows[:, 1:1 + mm] = np.floor(rows[:, 1:1 + mm] * 3)
+    seeds = rng.integers(0, 256, size=(3, 16), dtype=np.uint8)
+    fp = ctypes.POINTER(ctypes.c_float)
+    c.outputformat = 5  # jnii
+    L.mcx_savedetphoton(rows.ctypes.data_as(fp), seeds.ctypes.data, 3, 16, ctypes.byref(c))
+    assert _lib.last_error()[0] == 0
+    doc = json.load(open(tmp_path / "j_detp.jdat"))
+    info, data = doc["MCXData"]["Info"], doc["MCXData"]["PhotonData"]
+    assert info["MediaNum"] == mm and info["ColumnNum"] == ncol and info["DetectedPhoton"] == 3
+    assert info["SeedByte"] == 16 and info["SaveDetFlag"] == 0xFF
+    assert [m["mus"] for m in info["Media"]] == pytest.approx([0.0, 1.0, 2.0])
+    c.outputformat = 0  # mch
+    L.mcx_savedetphoton(rows.ctypes.data_as(fp), seeds.ctypes.data, 3, 16, ctypes.byref(c))
+    b = open(tmp_path / "j.mch", "rb").read()
+    mch = np.frombuffer(b[64:64 + rows.nbytes], dtype="<f4").reshape(3, ncol)
+    assert np.array_equal(np.frombuffer(b[64 + rows.nbytes:], dtype=np.uint8).reshape(3, 16), seeds)
+    col = 0
+    for name, n, dtype in zip(["detid", "nscat", "ppath", "mom", "p", "v", "w0", "s"], cols,
+                              ["uint32", "uint32"] + ["single"] * 6):
+        blk = data[name]
+        assert blk["_ArrayType_"] == dtype and blk["_ArraySize_"] == [3, n] and blk["_ArrayZipType_"] == "zlib"
+        assert np.array_equal(_jdata(blk).astype(np.float32), mch[:, col:col + n]), name
+        col += n
+    assert data["seed"]["_ArraySize_"] == [3, 16] and np.array_equal(_jdata(data["seed"]), seeds)
+
+
+@pytest.mark.parametrize("otype,replaydet,d5", [(6, 0, 2), (8, -1, 6), (3, -1, 3), (6, 2, 2)])
+def test_jnii_sixth_dimension(tmp_path, otype, replaydet, d5):
+    """src/mcx_utils.c:929-951: replay output with one block per detector
+    (replaydet -1) and the RF imaginary plane form a sixth dimension; RF
+    data holds twice the given length"""
+    c = _cfg()
+    c.dim.x, c.dim.y, c.dim.z = 2, 3, 4
+    c.maxgate, c.srcnum, c.tstep = 1, 1, 1e-9
+    c.session = str(tmp_path / "r").encode()
+    c.outputformat = 5
+    c.seed, c.outputtype, c.replaydet, c.detnum = -999, otype, replaydet, 3
+    n = 2 * 3 * 4 * d5
+    data = np.arange(n, dtype=np.float32)
+    rf = otype in (6, 8)
+    L.mcx_savedata(data.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n // (2 if rf else 1), ctypes.byref(c))
+    assert _lib.last_error()[0] == 0
+    doc = json.load(open(tmp_path / "r.jnii"))
+    assert doc["NIFTIHeader"]["Dim"] == [2, 3, 4, 1, 1, d5]
+    assert np.array_equal(_jdata(doc["NIFTIData"]).ravel(), data)
+    c.outputformat = 0
+    L.mcx_savedata(data.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n // (2 if rf else 1), ctypes.byref(c))
+    assert np.array_equal(np.fromfile(tmp_path / "r.mc2", dtype="<f4"), data)
 
 
 def test_run_without_gpu_raises_cleanly():
