@@ -177,13 +177,14 @@ def main():
     ap.add_argument("--workload", default="sphere100", choices=["sphere100", "cube60", "batch256"])
     ap.add_argument("--batch-scale", type=float, default=1.0, help="photon scale of the batch256 workload")
     ap.add_argument("--tally", type=int, default=2, help="0 FP32+shadow, 1 FP32 hi/lo, 2 FP64, 3 FP64 per-XCD copies")
-    ap.add_argument("--tally-blocks", type=int, default=1, help="FP64 tally in 2x2x2 voxel blocks summed per lane in LDS")
+    ap.add_argument("--tally-blocks", type=int, default=2,
+                    help="FP64 tally in 2x2x2 voxel blocks summed per lane in LDS: 0 off, 1 on, 2 auto")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atomic-roof", action="store_true", help="print the scattered-atomic microbenchmark and exit")
     ap.add_argument("--dn-timing", action="store_true", help="time the denoiser U-Net (train step, 256^3 inference)")
     ap.add_argument("--no-tt5", action="store_true", help="skip the time-to-5%% comparison on the head volume")
-    ap.add_argument("--tt5-k", type=int, default=8, help="head volumes per batched launch in the time-to-5%% leg")
+    ap.add_argument("--tt5-k", type=int, default=64, help="head volumes per batched launch in the time-to-5%% leg")
     args = ap.parse_args()
     if args.atomic_roof:
         atomic_roof_table()
@@ -214,7 +215,7 @@ def main():
 
     cfg = mk(nph)
     mcx_amd.set_tally_mode(args.tally)
-    mcx_amd.set_tally_blocks(bool(args.tally_blocks))
+    mcx_amd.set_tally_blocks(args.tally_blocks)
     # warmup; the first one also counts work (steps, deposits) per photon
     stats_w = None
     for i in range(max(args.warmup, 1)):
@@ -332,7 +333,7 @@ def _time_to_5pct(k):
         "batched_denoised_k": d["k"], "batched_denoised_photons": round(d["photons_for_5pct"]),
         "batched_denoised_ms_per_volume": round(d["ms_per_volume_to_5pct"], 2),
         "batched_denoised_error": round(d["error_at"], 4), "denoise_ms_per_volume": round(d["denoise_ms"], 2),
-        "host_ms_per_volume": round(d["host_ms_per_volume"], 2),
+        "launch_ms": round(d["launch_ms"], 2),
         "ratio": round(r["ratio"], 4), "target_ratio": r["target_ratio"], "seconds": round(r["seconds"], 1),
         "points": {"plain": p["points"], "batched_denoised": d["points"]},
     }

@@ -42,15 +42,34 @@ void mcx_amd_set_counters(int enable);
  * left at full occupancy allows.  Returns the previous setting. */
 int  mcx_amd_set_hot_box(int enable);
 
-/* Blocked FP64 tally (default on): in subsequent mcx_run_simulation calls
- * with tally mode 2, the tally is stored in 2 x 2 x 2 voxel blocks (the 8
- * cells of a block in one 64-byte line), each lane sums its deposits into
- * the block it is in, in LDS, and when a lane moves to another block the
- * wave adds the blocks left by its lanes to the tally with 8 lanes to a
- * block; the device's atomic unit takes one line per request.  The output
- * is the same volume (merged back to the reference's order).  Returns the
- * previous setting. */
-int  mcx_amd_set_tally_blocks(int enable);
+/* Blocked FP64 tally: in subsequent mcx_run_simulation calls with tally
+ * mode 2, the tally is stored in 2 x 2 x 2 voxel blocks (the 8 cells of a
+ * block in one 64-byte line), each lane sums its deposits into the block
+ * it is in, in LDS, and when a lane moves to another block the wave adds
+ * the blocks left by its lanes to the tally with 8 lanes to a block; the
+ * device's atomic unit takes one line per request.  The output is the same
+ * volume (merged back to the reference's order).  mode 0 off, 1 on, 2
+ * (default) on for label volumes whose mean scattering coefficient times
+ * the voxel length is at most 2 (more scattering per voxel means fewer
+ * deposits per step, where the blocks only add work).  Returns the
+ * previous mode, or -1 for an invalid one. */
+int  mcx_amd_set_tally_blocks(int mode);
+
+/* K independent runs of one forward simulation in one launch of the
+ * single-simulation kernel (no counterpart in the reference; for callers
+ * that need several independent estimates of one volume, such as a
+ * denoiser's inputs).  K is a power of two from 1 to 64; virtual thread
+ * idx belongs to run idx & (K - 1), so every run has its own RNG streams
+ * and nthread / K of them, and cfg->nphoton photons are launched per run.
+ * Each run keeps its own FP64 tally and launched energy and is normalised
+ * like mcx_run_simulation's flux; the outputs are reduced on the device to
+ * time-integrated maps.  Forward runs only (no detected photons, replay,
+ * respin or gate groups; output type flux).  cw: K x nvox floats (x
+ * fastest), host or device memory; energy: K pairs (escaped, launched);
+ * stats: [0] tally clear
+ * ms, [1] transport kernel ms, [2] reduction ms, [3] kernel launch+sync
+ * wall ms.  Returns 0, or the error id (message via mcx_last_error). */
+int  mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* energy, double* stats);
 
 /* Numbering of replayed photons (seed = SEED_FROM_FILE) in subsequent
  * mcx_run_simulation calls.  0 (default): photon k of virtual thread idx is

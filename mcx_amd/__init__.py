@@ -4,7 +4,7 @@ mcx_run_simulation path).
 The engine is libmcx_amd.so (HIP kernel + C++ host layer, C ABI declared in
 include/mcx_abi.h).  This package exposes the pmcx-style dictionary API.
 """
-from .pmcx import gpuinfo, run, version  # noqa: F401
+from .pmcx import gpuinfo, run, run_replicas, version  # noqa: F401
 from .batch import run_batch  # noqa: F401
 from . import _lib  # noqa: F401
 
@@ -30,11 +30,16 @@ def set_hot_box(enable):
     return bool(_lib.lib().mcx_amd_set_hot_box(1 if enable else 0))
 
 
-def set_tally_blocks(enable):
+def set_tally_blocks(mode):
     """FP64 tally in 2x2x2 voxel blocks, summed per lane in LDS and added
-    to the tally 8 lanes to a block (default on).  Returns the previous
-    setting."""
-    return bool(_lib.lib().mcx_amd_set_tally_blocks(1 if enable else 0))
+    to the tally 8 lanes to a block: False/0 off, True/1 on, 2 or "auto"
+    (default) on when the medium scatters at most twice per voxel length.
+    Returns the previous mode."""
+    m = 2 if mode == "auto" else int(mode)
+    old = _lib.lib().mcx_amd_set_tally_blocks(m)
+    if old < 0:
+        raise ValueError("invalid tally block mode %r" % (mode,))
+    return old
 
 
 def set_replay_index(reference):

@@ -115,6 +115,7 @@ ABI_SYMBOLS = [
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_set_replay_index",
     "mcx_amd_set_tally_blocks",
+    "mcx_amd_run_replicas",
     "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
     "mcx_dn_conv3_fwd", "mcx_dn_conv3_fwd_cat", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
@@ -156,6 +157,9 @@ def _setup(L):
     L.mcx_amd_set_counters.argtypes = [ctypes.c_int]
     L.mcx_amd_set_hot_box.argtypes = [ctypes.c_int]
     L.mcx_amd_set_hot_box.restype = ctypes.c_int
+    L.mcx_amd_run_replicas.argtypes = [P(Config), P(GPUInfo), ctypes.c_int, P(ctypes.c_float), P(ctypes.c_double),
+                                       P(ctypes.c_double)]
+    L.mcx_amd_run_replicas.restype = ctypes.c_int
     L.mcx_amd_set_tally_blocks.argtypes = [ctypes.c_int]
     L.mcx_amd_set_tally_blocks.restype = ctypes.c_int
     L.mcx_amd_set_replay_index.argtypes = [ctypes.c_int]

@@ -36,6 +36,9 @@ extern "C" hipError_t mcxk_launch_transport(const KParam* dkp, int ispencil, int
 extern "C" hipError_t mcxk_kernel_attrs(int ispencil, int isreflect, int tally, int media32, int* numregs, int* maxthreads);
 extern "C" hipError_t mcxk_launch_rngtest(float* out, const uint32_t* seeds, uint32_t n, hipStream_t stream);
 extern "C" hipError_t mcxk_launch_merge(int tally, float* out, const void* a, const void* b, const void* c, size_t n, hipStream_t stream);
+extern "C" hipError_t mcxk_launch_replica_cw(float* cw, const double* d, const double* scale, uint32_t nx, uint32_t ny,
+                                             uint32_t nz, uint32_t planes, uint32_t nmaps, uint32_t bnx, uint32_t bny,
+                                             uint32_t bnz, hipStream_t stream);
 extern "C" hipError_t mcxk_launch_merge_blocked(float* out, const double* a, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t bnx,
                                                  uint32_t bny, uint32_t bnz, size_t n, hipStream_t stream);
 
@@ -48,8 +51,20 @@ static int g_counters = 0;
 static int g_hotbox = 1;
 /* replayed-photon numbering: 0 contiguous, 1 the reference's */
 static int g_replayindex = 0;
-/* blocked FP64 tally with per-lane LDS blocks (KParam::tblock) */
-static int g_tblock = 1;
+/* blocked FP64 tally with per-lane LDS blocks (KParam::tblock): 0 off,
+ * 1 on, 2 (default) on when the medium's mean scattering coefficient is
+ * at most 2 per voxel length: where photons scatter many times within a
+ * voxel, deposits are few per step, the atomics do not bound the kernel
+ * and the blocks only add instructions */
+static int g_tblock = 2;
+/* a request for K replicas of the next run (mcx_amd_run_replicas) */
+struct ReplicaReq {
+    int k;
+    float* cw;
+    double* energy;
+    double* stats;
+};
+static ReplicaReq* g_rep = nullptr;
 
 static bool supported_source(int t) { return t >= MCX_SRC_PENCIL && t <= MCX_SRC_RING; }
 
@@ -262,6 +277,52 @@ static void rng_test_path(Config* cfg, GPUInfo* gpu, int gpuid, size_t fieldlen)
     if (cfg->flog) fprintf(cfg->flog, "generated %zu random numbers\n", fieldlen);
 }
 
+/* After a replica launch: launched/escaped energy per replica (lanes
+ * idx & (K-1)), each replica's flux normalisation (src/mcx_core.cu:3823-3834)
+ * times tstep, and the time-integrated maps reduced on the device.  stats:
+ * [0] clear ms, [1] transport kernel ms, [2] reduction ms, [3] launch+sync
+ * wall ms of the kernel. */
+static void finish_replicas(Config* cfg, ReplicaReq& R, DevMem& dgenergy, DevMem& dfield64, unsigned int nthread, size_t nvox,
+                            uint32_t planes, uint32_t bnx, uint32_t bny, uint32_t bnz, EventPair& ev, EventPair& evc,
+                            double wall_ms, hipStream_t s) {
+    const unsigned int K = (unsigned int)R.k;
+    std::vector<float> energy((size_t)nthread * 2);
+    HIP_CHECK(hipMemcpyAsync(energy.data(), dgenergy.p, energy.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<double> esc(K, 0.0), tot(K, 0.0), scale(K, 0.0);
+    for (unsigned int i = 0; i < nthread; i++) {
+        esc[i & (K - 1)] += energy[2 * i];
+        tot[i & (K - 1)] += energy[2 * i + 1];
+    }
+    const float Vvox = cfg->steps.x * cfg->steps.y * cfg->steps.z;
+    for (unsigned int m = 0; m < K; m++) {
+        const float sc = tot[m] > 0.0 ? (float)(cfg->unitinmm / (tot[m] * Vvox * cfg->tstep)) : 0.f;
+        scale[m] = (double)sc * (double)cfg->tstep;
+        R.energy[2 * m] = esc[m];
+        R.energy[2 * m + 1] = tot[m];
+    }
+    DevMem dscale, dcw;
+    dscale.alloc(K * sizeof(double));
+    dcw.alloc((size_t)K * nvox * sizeof(float));
+    HIP_CHECK(hipMemcpyAsync(dscale.p, scale.data(), K * sizeof(double), hipMemcpyHostToDevice, s));
+    EventPair er;
+    HIP_CHECK(hipEventRecord(er.a, s));
+    HIP_CHECK(mcxk_launch_replica_cw(dcw.as<float>(), dfield64.as<double>(), dscale.as<double>(), cfg->dim.x, cfg->dim.y,
+                                     cfg->dim.z, planes, K, bnx, bny, bnz, s));
+    HIP_CHECK(hipEventRecord(er.b, s));
+    /* R.cw may be host or device memory (unified addressing) */
+    HIP_CHECK(hipMemcpyAsync(R.cw, dcw.p, dcw.bytes, hipMemcpyDefault, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    float clear_ms = 0.f, kern_ms = 0.f, red_ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&clear_ms, evc.a, ev.a));
+    HIP_CHECK(hipEventElapsedTime(&kern_ms, ev.a, ev.b));
+    HIP_CHECK(hipEventElapsedTime(&red_ms, er.a, er.b));
+    R.stats[0] = clear_ms;
+    R.stats[1] = kern_ms;
+    R.stats[2] = red_ms;
+    R.stats[3] = wall_ms;
+}
+
 static void run_impl(Config* cfg, GPUInfo* gpu) {
     check_supported(cfg);
     int threadid = 0;
@@ -297,7 +358,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     cfg->energytot = cfg->energyabs = cfg->energyesc = 0.0;
     cfg->runtime = 0;
 
-    const int tally = g_tally_mode;
+    const int tally = g_rep ? MCX_TALLY_F64 : g_tally_mode;
     const int ispencil = (cfg->srctype == MCX_SRC_PENCIL);
     int isref = cfg->isreflect;
     for (int i = 0; i < 6; i++)
@@ -306,10 +367,22 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     /* media: uint8 with the detector bit in bit 7 when labels fit, uint32
      * labels, or the 32-bit property words of the non-label formats */
     int media32 = cfg->mediabyte == MCX_MEDIA_2LABEL_SPLIT ? 3 : 2;
+    double musgrid = 0.0; /* mean mus x voxel length over tissue voxels */
     if (cfg->mediabyte <= 4) {
         unsigned int maxlabel = 0;
-        for (size_t i = 0; i < nvox; i++) maxlabel = std::max(maxlabel, cfg->vol[i] & MCX_MED_MASK);
+        std::vector<size_t> hist(cfg->medianum, 0);
+        for (size_t i = 0; i < nvox; i++) {
+            const unsigned int l = cfg->vol[i] & MCX_MED_MASK;
+            maxlabel = std::max(maxlabel, l);
+            if (l < cfg->medianum) hist[l]++;
+        }
         media32 = (maxlabel > 127);
+        size_t nt = 0;
+        for (unsigned int l = 1; l < cfg->medianum; l++) {
+            musgrid += (double)hist[l] * cfg->prop[l].mus * cfg->unitinmm;
+            nt += hist[l];
+        }
+        musgrid = nt ? musgrid / (double)nt : 0.0;
     }
 
     int numregs = 0, maxthreads = 0;
@@ -341,7 +414,15 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const bool rf = replay && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
     const size_t planelen = nvox * maxgate * blocks;
     const size_t fieldlen = planelen * (rf ? 2 : 1);
-    if (cfg->exportfield == NULL && cfg->issave2pt) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
+    /* replicas: K independent runs in one launch, one set of gate planes each */
+    const unsigned int nrep = g_rep ? (unsigned int)g_rep->k : 1u;
+    if (g_rep) {
+        if (replay || cfg->issavedet) MCX_THROW(-1, "replicas are forward runs without detected photons");
+        if (totalgates > maxgate || abs(cfg->respin) != 1) MCX_THROW(-1, "replicas need every time gate in one pass and no respin");
+        if (cfg->outputtype != otFlux) MCX_THROW(-1, "replicas output the time-integrated flux");
+    }
+    const size_t tallylen = fieldlen * nrep;
+    if (cfg->exportfield == NULL && cfg->issave2pt && !g_rep) cfg->exportfield = (float*)calloc(fieldlen, sizeof(float));
     if (cfg->exportdetected == NULL) cfg->exportdetected = (float*)malloc(std::max<size_t>(1, (size_t)hostdetreclen * cfg->maxdetphoton) * sizeof(float));
 
     if (cfg->debuglevel & MCX_DEBUG_RNG) {
@@ -350,7 +431,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
 
     /* work split (src/mcx_core.cu:2980-3022): one device takes all photons */
-    const size_t gpuphoton = cfg->nphoton;
+    const size_t gpuphoton = cfg->nphoton * nrep;
     if (gpuphoton == 0) return;
     int threadphoton, oddphotons;
     const unsigned int nthread = (unsigned int)G.autothread;
@@ -396,10 +477,10 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const bool savedet = cfg->issavedet != 0;
     /* blocked FP64 tally: 2 x 2 x 2 voxel blocks of every gate plane */
     const uint32_t bnx = (cfg->dim.x + 1) / 2, bny = (cfg->dim.y + 1) / 2, bnz = (cfg->dim.z + 1) / 2;
-    const uint64_t nblk = (uint64_t)(fieldlen / nvox) * bnz * bny * bnx;
-    bool tblock = g_tblock && tally == MCX_TALLY_F64 && nblk < 0xFFFFFFFFull;
+    const uint64_t nblk = (uint64_t)(tallylen / nvox) * bnz * bny * bnx;
+    bool tblock = (g_tblock == 1 || (g_tblock == 2 && musgrid <= 2.0)) && tally == MCX_TALLY_F64 && nblk < 0xFFFFFFFFull;
     if (tally == MCX_TALLY_F64) {
-        dfield64.alloc((tblock ? std::max<size_t>(fieldlen, (size_t)nblk * 8) : fieldlen) * sizeof(double));
+        dfield64.alloc((tblock ? std::max<size_t>(tallylen, (size_t)nblk * 8) : tallylen) * sizeof(double));
     } else if (tally == MCX_TALLY_F64_XCD) {
         dfield64.alloc(fieldlen * MCX_NUM_XCD * sizeof(double));
     } else {
@@ -407,7 +488,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         dfield2.alloc(fieldlen * sizeof(float));
         if (tally == MCX_TALLY_F32_HILO) dfieldk.alloc(fieldlen * sizeof(int32_t));
     }
-    dmerged.alloc(fieldlen * sizeof(float));
+    if (!g_rep) dmerged.alloc(fieldlen * sizeof(float));
     dgenergy.alloc((size_t)nthread * 2 * sizeof(float));
     HIP_CHECK(hipMemsetAsync(dgenergy.p, 0, dgenergy.bytes, st.s));
     dseeds.alloc((size_t)nthread * 4 * sizeof(uint32_t));
@@ -458,6 +539,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.replaytof = drtof.as<float>();
     kp.replaydetid = drdetid.as<int32_t>();
     kp.replayindex = g_replayindex;
+    kp.nmaps_mask = nrep - 1;
+    kp.mapplanes = (uint32_t)(fieldlen / nvox);
     kp.omega = cfg->omega;
     kp.rfshift = rf ? (int32_t)(maxgate * blocks) : 0;
 
@@ -514,12 +597,13 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     int nlaunch = 0;
     uint32_t counts[4] = {0, 0, 0, 0};
     uint32_t errlanes = 0;
-    EventPair ev;
+    EventPair ev, evc;
 
     for (unsigned int timegate = 0; timegate < totalgates; timegate += maxgate) {
         kp.twin0 = cfg->tstart + cfg->tstep * timegate;
         kp.twin1 = kp.twin0 + cfg->tstep * maxgate;
         for (int iter = 0; iter < abs(cfg->respin); iter++) {
+            if (g_rep) HIP_CHECK(hipEventRecord(evc.a, st.s));
             if (tally == MCX_TALLY_F64 || tally == MCX_TALLY_F64_XCD) {
                 HIP_CHECK(hipMemsetAsync(dfield64.p, 0, dfield64.bytes, st.s));
             } else {
@@ -552,6 +636,13 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             nlaunch++;
             errlanes += counts[1];
             uint32_t detected = counts[0];
+            if (g_rep) {
+                finish_replicas(cfg, *g_rep, dgenergy, dfield64, nthread, nvox, (uint32_t)(fieldlen / nvox),
+                                tblock ? bnx : 0u, bny, bnz, ev, evc, wall_ms, st.s);
+                if (counts[1]) MCX_THROW(-2, "transport kernel: a lane stopped on an internal error");
+                cfg->runtime = (unsigned int)llround(wall_ms);
+                return;
+            }
 
             if (savedet) {
                 if (detected > cfg->maxdetphoton) {
@@ -841,9 +932,34 @@ int mcx_amd_set_replay_index(int reference) {
     return old;
 }
 
-int mcx_amd_set_tally_blocks(int enable) {
+int mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* energy, double* stats) {
+    EntryGuard g;
+    GPUInfo* own = NULL;
+    ReplicaReq req{k, cw, energy, stats};
+    try {
+        if (k < 1 || k > 64 || (k & (k - 1))) MCX_THROW(-1, "the replica count must be a power of two from 1 to 64");
+        if (!cw || !energy || !stats) MCX_THROW(-1, "replica outputs can not be NULL");
+        if (gpu == NULL) {
+            if (list_gpu_impl(cfg, &own) == 0) MCX_THROW(-1, "no active GPU device");
+            gpu = own;
+        }
+        g_rep = &req;
+        run_impl(cfg, gpu);
+        g_rep = nullptr;
+    } catch (const Error& e) {
+        g_rep = nullptr;
+        if (own) free(own);
+        record_error(e.id, e.msg.c_str());
+        return e.id ? e.id : -1;
+    }
+    if (own) free(own);
+    return 0;
+}
+
+int mcx_amd_set_tally_blocks(int mode) {
+    if (mode < 0 || mode > 2) return -1;
     int old = g_tblock;
-    g_tblock = enable ? 1 : 0;
+    g_tblock = mode;
     return old;
 }
 

@@ -100,6 +100,9 @@ struct KParam {
      * bq_off).  The device's atomic unit takes one 64-byte line per
      * request, so a block's cells cost one request instead of one each. */
     uint32_t tblock, bnx, bny, bnz, blk_off, bq_off;
+    /* Replicas: virtual thread idx tallies into replica idx & nmaps_mask,
+     * whose gate planes start mapplanes x replica planes on (0: one run) */
+    uint32_t nmaps_mask, mapplanes;
 };
 
 

@@ -1043,6 +1043,7 @@ struct Lane {
      * loop iteration), which may wait in the lane's block */
     __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight, bool main = false) {
         ndep++;
+        tshift += (int)((idx & kp.nmaps_mask) * kp.mapplanes);
         if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
             const uint32_t tz = __umulhi(vox, kp.hot_mxy);
             const uint32_t z = (tz + ((vox - tz) >> (kp.hot_shxy & 0xffu))) >> (kp.hot_shxy >> 8), r = vox - z * kp.dimxy;
@@ -1797,6 +1798,45 @@ __global__ void mcx_merge_f64_blocked(float* __restrict__ out, const double* __r
         const size_t b = (((size_t)g * bnz + (z >> 1)) * bny + (y >> 1)) * bnx + (x >> 1);
         out[i] = (float)d[(b << 3) + (x & 1u) + ((y & 1u) << 1) + ((z & 1u) << 2)];
     }
+}
+
+/* Replicas: the time-integrated map of replica m = tstep x the sum over
+ * its gates of the normalised tally (scale[m] includes tstep), FP64 tally
+ * linear or blocked (bnx > 0); one lane per (replica, voxel) */
+__global__ void mcx_replica_cw(float* __restrict__ cw, const double* __restrict__ d, const double* __restrict__ scale,
+                               uint32_t nx, uint32_t ny, uint32_t nz, uint32_t planes, uint32_t nmaps, uint32_t bnx,
+                               uint32_t bny, uint32_t bnz) {
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t n = nvox * nmaps;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const uint32_t m = (uint32_t)(i / nvox);
+        const uint32_t v = (uint32_t)(i - (size_t)m * nvox);
+        const uint32_t z = v / (nx * ny), r = v - z * nx * ny, y = r / nx, x = r - y * nx;
+        double t = 0.0;
+        for (uint32_t g = 0; g < planes; g++) {
+            const uint32_t gp = m * planes + g;
+            if (bnx) {
+                const size_t b = (((size_t)gp * bnz + (z >> 1)) * bny + (y >> 1)) * bnx + (x >> 1);
+                t += d[(b << 3) + (x & 1u) + ((y & 1u) << 1) + ((z & 1u) << 2)];
+            } else {
+                t += d[(size_t)gp * nvox + v];
+            }
+        }
+        cw[i] = (float)(t * scale[m]);
+    }
+}
+
+extern "C" hipError_t mcxk_launch_replica_cw(float* cw, const double* d, const double* scale, uint32_t nx, uint32_t ny,
+                                             uint32_t nz, uint32_t planes, uint32_t nmaps, uint32_t bnx, uint32_t bny,
+                                             uint32_t bnz, hipStream_t stream) {
+    const size_t n = (size_t)nx * ny * nz * nmaps;
+    size_t g = (n + 255) / 256;
+    const unsigned int grid = (unsigned int)(g > 65536 ? 65536 : (g ? g : 1));
+    hipLaunchKernelGGL(mcx_replica_cw, dim3(grid), dim3(256), 0, stream, cw, d, scale, nx, ny, nz, planes, nmaps, bnx, bny,
+                       bnz);
+    return hipGetLastError();
 }
 
 /* ---- host-side dispatch (called from mcx_host.cpp) ------------------- */

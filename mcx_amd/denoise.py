@@ -191,13 +191,14 @@ class UNet:
         return p[wo:wo + cout * kp].reshape(cout, kp)[:, :27 * cin], p[bo:bo + cout]
 
     def _alloc(self, nb, S):
+        """S: the full-resolution grid (X, Y, Z), each a multiple of 8"""
         torch = _torch()
         key = (nb, S)
         if self._acts is not None and self._acts["key"] == key:
             return self._acts
         bf = dict(dtype=torch.bfloat16, device=self.device)
-        s1, s2, s3, s4 = S, S // 2, S // 4, S // 8
-        n1, n2, n3, n4 = nb * s1 ** 3, nb * s2 ** 3, nb * s3 ** 3, nb * s4 ** 3
+        s1, s2, s3, s4 = (tuple(d // f for d in S) for f in (1, 2, 4, 8))
+        n1, n2, n3, n4 = (nb * math.prod(s) for s in (s1, s2, s3, s4))
         A = {"key": key, "dims": (s1, s2, s3, s4), "n": (n1, n2, n3, n4)}
         A["x0"] = torch.zeros(n1 * NFEAT, **bf)
         for nm, n, c in [("e1a", n1, 16), ("cat1", n1, 48), ("p1", n2, 16), ("e2a", n2, 32), ("cat2", n2, 96),
@@ -258,24 +259,27 @@ class UNet:
         wo, bo, cin, cout, kp = self.layout[name]
         S = self._dims_of(name, self._acts)
         _chk(self.L.mcx_dn_conv3_fwd(_ptr(x, xoff), xcs, None, 0, _ptr(self.pb, wo), kp, _ptr(self.p, bo), _ptr(y, yoff),
-                                     ycs, nb, S, S, S, cin, cout, 1, self._stream()), "conv3_fwd " + name)
+                                     ycs, nb, *S, cin, cout, 1, self._stream()), "conv3_fwd " + name)
 
     def _conv_cat(self, name, xu, xucs, cu, x, xcs, y, ycs, nb):
         """conv of cat(upsample2(xu), x) without writing the concatenation"""
         wo, bo, cin, cout, kp = self.layout[name]
         S = self._dims_of(name, self._acts)
         _chk(self.L.mcx_dn_conv3_fwd_cat(_ptr(xu), xucs, cu, _ptr(x), xcs, _ptr(self.pb, wo), kp, _ptr(self.p, bo),
-                                         _ptr(y), ycs, nb, S, S, S, cin, cout, 1, self._stream()), "conv3_fwd_cat " + name)
+                                         _ptr(y), ycs, nb, *S, cin, cout, 1, self._stream()), "conv3_fwd_cat " + name)
 
     def forward(self, x0, target=None):
-        """x0: torch bf16 [nb, S, S, S, 8] on the device (S multiple of 8).
-        Returns the FP32 prediction [nb*S^3]; with a target (FP32 [nb*S^3])
+        """x0: torch bf16 [nb, X, Y, Z, 8] on the device (X, Y, Z multiples
+        of 8).  Returns the FP32 prediction [nb*X*Y*Z]; with a target (FP32
+        [nb*X*Y*Z])
         also the MSE loss (a device scalar, summed on the host).  Without a
         target (inference) the two full-resolution skip concatenations are
         not written: the decoder reads the encoder outputs and the upsampled
         coarse maps through mcx_dn_conv3_fwd_cat (same result), and backward
         needs a forward with a target."""
-        nb, S = int(x0.shape[0]), int(x0.shape[1])
+        nb, S = int(x0.shape[0]), tuple(int(d) for d in x0.shape[1:4])
+        if any(d % 8 for d in S):
+            raise ValueError("the grid must be a multiple of 8 in every dimension")
         A = self._alloc(nb, S)
         A["x0"].copy_(x0.reshape(-1))
         L, st = self.L, self._stream()
@@ -286,28 +290,28 @@ class UNet:
         e2, e2o, e2c = (A["e2s"], 0, 32) if infer else (A["cat2"], 64, 96)
         self._conv("e1a", A["x0"], 0, NFEAT, A["e1a"], 0, 16, nb)
         self._conv("e1", A["e1a"], 0, 16, e1, e1o, e1c, nb)
-        _chk(L.mcx_dn_maxpool_fwd(_ptr(e1, e1o), e1c, _ptr(A["p1"]), 16, _ptr(A["arg1"]), nb, s1, s1, s1, 16, st), "pool1")
+        _chk(L.mcx_dn_maxpool_fwd(_ptr(e1, e1o), e1c, _ptr(A["p1"]), 16, _ptr(A["arg1"]), nb, *s1, 16, st), "pool1")
         self._conv("e2a", A["p1"], 0, 16, A["e2a"], 0, 32, nb)
         self._conv("e2", A["e2a"], 0, 32, e2, e2o, e2c, nb)
-        _chk(L.mcx_dn_maxpool_fwd(_ptr(e2, e2o), e2c, _ptr(A["p2"]), 32, _ptr(A["arg2"]), nb, s2, s2, s2, 32, st), "pool2")
+        _chk(L.mcx_dn_maxpool_fwd(_ptr(e2, e2o), e2c, _ptr(A["p2"]), 32, _ptr(A["arg2"]), nb, *s2, 32, st), "pool2")
         self._conv("e3a", A["p2"], 0, 32, A["e3a"], 0, 64, nb)
         self._conv("e3", A["e3a"], 0, 64, A["cat3"], 128, 192, nb)
-        _chk(L.mcx_dn_maxpool_fwd(_ptr(A["cat3"], 128), 192, _ptr(A["p3"]), 64, _ptr(A["arg3"]), nb, s3, s3, s3, 64, st), "pool3")
+        _chk(L.mcx_dn_maxpool_fwd(_ptr(A["cat3"], 128), 192, _ptr(A["p3"]), 64, _ptr(A["arg3"]), nb, *s3, 64, st), "pool3")
         self._conv("b1", A["p3"], 0, 64, A["b1"], 0, 128, nb)
         self._conv("b2", A["b1"], 0, 128, A["b2"], 0, 128, nb)
-        _chk(L.mcx_dn_upsample_fwd(_ptr(A["b2"]), 128, _ptr(A["cat3"]), 192, nb, s4, s4, s4, 128, st), "up3")
+        _chk(L.mcx_dn_upsample_fwd(_ptr(A["b2"]), 128, _ptr(A["cat3"]), 192, nb, *s4, 128, st), "up3")
         self._conv("d3a", A["cat3"], 0, 192, A["d3a"], 0, 64, nb)
         self._conv("d3", A["d3a"], 0, 64, A["d3"], 0, 64, nb)
         if infer:
             self._conv_cat("d2a", A["d3"], 64, 64, e2, e2c, A["d2a"], 32, nb)
         else:
-            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d3"]), 64, _ptr(A["cat2"]), 96, nb, s3, s3, s3, 64, st), "up2")
+            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d3"]), 64, _ptr(A["cat2"]), 96, nb, *s3, 64, st), "up2")
             self._conv("d2a", A["cat2"], 0, 96, A["d2a"], 0, 32, nb)
         self._conv("d2", A["d2a"], 0, 32, A["d2"], 0, 32, nb)
         if infer:
             self._conv_cat("d1a", A["d2"], 32, 32, e1, e1c, A["d1a"], 16, nb)
         else:
-            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d2"]), 32, _ptr(A["cat1"]), 48, nb, s2, s2, s2, 32, st), "up1")
+            _chk(L.mcx_dn_upsample_fwd(_ptr(A["d2"]), 32, _ptr(A["cat1"]), 48, nb, *s2, 32, st), "up1")
             self._conv("d1a", A["cat1"], 0, 48, A["d1a"], 0, 16, nb)
         self._conv("d1", A["d1a"], 0, 16, A["d1"], 0, 16, nb)
         ho = self.head_off
@@ -330,7 +334,7 @@ class UNet:
         nv = self._nvox_of(name, A)
         ns = self._nsplit(nv, cin, cout)
         _chk(L.mcx_dn_conv3_wgrad(_ptr(x, xoff), xcs, _ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(A["wpart"]), ns, nb,
-                                  S, S, S, cin, cout, st), "wgrad " + name)
+                                  *S, cin, cout, st), "wgrad " + name)
         _chk(L.mcx_dn_wgrad_reduce(_ptr(A["wpart"]), ns, _ptr(self.g, wo), kp, cin, cout, st), "wgrad_reduce " + name)
         _chk(L.mcx_dn_bias_grad(_ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(A["wpart"]), ns, _ptr(self.g, bo), nv, cout,
                                 st), "bias_grad " + name)
@@ -338,7 +342,7 @@ class UNet:
             kpt = kpad_of(cout)
             _chk(L.mcx_dn_weight_flip(_ptr(self.pb, wo), kp, _ptr(self.flip[name]), kpt, cin, cout, st), "flip " + name)
             _chk(L.mcx_dn_conv3_fwd(_ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(self.flip[name]), kpt, None,
-                                    _ptr(gx, gxoff), gxcs, nb, S, S, S, cout, cin, 0, st), "dgrad " + name)
+                                    _ptr(gx, gxoff), gxcs, nb, *S, cout, cin, 0, st), "dgrad " + name)
 
     def backward(self):
         """Gradients of the last forward's loss into self.g."""
@@ -352,25 +356,25 @@ class UNet:
                                _ptr(A["hpart"]), self.hblocks, _ptr(self.g, ho), n1, 16, st), "head_bwd")
         self._conv_bwd("d1", A["d1a"], 0, 16, A["d1"], 0, 16, A["g_d1"], 0, 16, A["g_d1a"], 0, 16, nb)
         self._conv_bwd("d1a", A["cat1"], 0, 48, A["d1a"], 0, 16, A["g_d1a"], 0, 16, A["g_cat1"], 0, 48, nb)
-        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat1"]), 48, _ptr(A["g_d2"]), 32, nb, s2, s2, s2, 32, st), "up1 bwd")
+        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat1"]), 48, _ptr(A["g_d2"]), 32, nb, *s2, 32, st), "up1 bwd")
         self._conv_bwd("d2", A["d2a"], 0, 32, A["d2"], 0, 32, A["g_d2"], 0, 32, A["g_d2a"], 0, 32, nb)
         self._conv_bwd("d2a", A["cat2"], 0, 96, A["d2a"], 0, 32, A["g_d2a"], 0, 32, A["g_cat2"], 0, 96, nb)
-        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat2"]), 96, _ptr(A["g_d3"]), 64, nb, s3, s3, s3, 64, st), "up2 bwd")
+        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat2"]), 96, _ptr(A["g_d3"]), 64, nb, *s3, 64, st), "up2 bwd")
         self._conv_bwd("d3", A["d3a"], 0, 64, A["d3"], 0, 64, A["g_d3"], 0, 64, A["g_d3a"], 0, 64, nb)
         self._conv_bwd("d3a", A["cat3"], 0, 192, A["d3a"], 0, 64, A["g_d3a"], 0, 64, A["g_cat3"], 0, 192, nb)
-        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat3"]), 192, _ptr(A["g_b2"]), 128, nb, s4, s4, s4, 128, st), "up3 bwd")
+        _chk(L.mcx_dn_upsample_bwd(_ptr(A["g_cat3"]), 192, _ptr(A["g_b2"]), 128, nb, *s4, 128, st), "up3 bwd")
         self._conv_bwd("b2", A["b1"], 0, 128, A["b2"], 0, 128, A["g_b2"], 0, 128, A["g_b1"], 0, 128, nb)
         self._conv_bwd("b1", A["p3"], 0, 64, A["b1"], 0, 128, A["g_b1"], 0, 128, A["g_p3"], 0, 64, nb)
         # e3 feeds pool3 and the skip slice of cat3
-        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p3"]), 64, _ptr(A["arg3"]), _ptr(A["pool3"]), 64, nb, s3, s3, s3, 64, st), "pool3 bwd")
+        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p3"]), 64, _ptr(A["arg3"]), _ptr(A["pool3"]), 64, nb, *s3, 64, st), "pool3 bwd")
         _chk(L.mcx_dn_add(_ptr(A["pool3"]), 64, _ptr(A["g_cat3"], 128), 192, _ptr(A["g_cat3"], 128), 192, n3, 64, st), "add3")
         self._conv_bwd("e3", A["e3a"], 0, 64, A["cat3"], 128, 192, A["g_cat3"], 128, 192, A["g_e3a"], 0, 64, nb)
         self._conv_bwd("e3a", A["p2"], 0, 32, A["e3a"], 0, 64, A["g_e3a"], 0, 64, A["g_p2"], 0, 32, nb)
-        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p2"]), 32, _ptr(A["arg2"]), _ptr(A["pool2"]), 32, nb, s2, s2, s2, 32, st), "pool2 bwd")
+        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p2"]), 32, _ptr(A["arg2"]), _ptr(A["pool2"]), 32, nb, *s2, 32, st), "pool2 bwd")
         _chk(L.mcx_dn_add(_ptr(A["pool2"]), 32, _ptr(A["g_cat2"], 64), 96, _ptr(A["g_cat2"], 64), 96, n2, 32, st), "add2")
         self._conv_bwd("e2", A["e2a"], 0, 32, A["cat2"], 64, 96, A["g_cat2"], 64, 96, A["g_e2a"], 0, 32, nb)
         self._conv_bwd("e2a", A["p1"], 0, 16, A["e2a"], 0, 32, A["g_e2a"], 0, 32, A["g_p1"], 0, 16, nb)
-        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p1"]), 16, _ptr(A["arg1"]), _ptr(A["pool1"]), 16, nb, s1, s1, s1, 16, st), "pool1 bwd")
+        _chk(L.mcx_dn_maxpool_bwd(_ptr(A["g_p1"]), 16, _ptr(A["arg1"]), _ptr(A["pool1"]), 16, nb, *s1, 16, st), "pool1 bwd")
         _chk(L.mcx_dn_add(_ptr(A["pool1"]), 16, _ptr(A["g_cat1"], 32), 48, _ptr(A["g_cat1"], 32), 48, n1, 16, st), "add1")
         self._conv_bwd("e1", A["e1a"], 0, 16, A["cat1"], 32, 48, A["g_cat1"], 32, 48, A["g_e1a"], 0, 16, nb)
         self._conv_bwd("e1a", A["x0"], 0, NFEAT, A["e1a"], 0, 16, A["g_e1a"], 0, 16, None, 0, 0, nb, need_dx=False)

@@ -149,37 +149,138 @@ def denoise_maps(net, feats, flip_z=False):
 class HeadDenoiser:
     """The denoised pipeline on one volume with everything resident on the
     GPU: feature kernel, U-Net, inverse kernel (the volume is mirrored in z
-    because its source enters from +z)."""
+    because its source enters from +z).
 
-    def __init__(self, net, cfg):
+    roi: the network runs on a box around the voxels whose input fluence
+    exceeds roi_threshold x its maximum: per axis from the roi_quantile to
+    the 1 - roi_quantile quantile of their coordinates (isolated voxels
+    reached by a stray photon do not widen it), widened by roi_margin
+    voxels and aligned to 8-voxel cells of the full grid (so the U-Net's
+    pooling cells are the full-volume run's); voxels outside the box keep
+    the input fluence."""
+
+    def __init__(self, net, cfg, roi=True, roi_threshold=3e-6, roi_margin=16, roi_quantile=1e-3):
         import torch
         self.net = net
         self.shape = cfg["vol"].shape
+        X, Y, Z = self.shape
         self.vol = torch.from_numpy(np.asfortranarray(cfg["vol"], dtype=np.uint8).ravel(order="F")).cuda()
         self.tab = torch.from_numpy(denoise._media_table(cfg["prop"], float(cfg.get("unitinmm", 1.0)))).cuda()
+        self.roi, self.thr, self.margin, self.q = roi, float(roi_threshold), int(roi_margin), float(roi_quantile)
+        self.last_box = (0, X, 0, Y, 0, Z)
 
     def upload(self, phi):
         import torch
         return torch.from_numpy(np.asfortranarray(phi, dtype=np.float32).ravel(order="F")).cuda()
 
-    def run(self, dphi, photons=None):
-        """device fluence (x-fastest) made with `photons` photons -> device
-        denoised fluence"""
+    def box(self, dphi):
+        """(x0, x1, y0, y1, z0, z1) of the region the network runs on"""
+        X, Y, Z = self.shape
+        if not self.roi:
+            return (0, X, 0, Y, 0, Z)
+        p = dphi.view(Z, Y, X)
+        m = p > self.thr * p.max()
+        out = []
+        for ax, n in ((2, X), (1, Y), (0, Z)):
+            other = tuple(a for a in (0, 1, 2) if a != ax)
+            idx = m.any(dim=other[1]).any(dim=other[0]).nonzero()
+            if idx.numel() == 0:
+                return (0, X, 0, Y, 0, Z)
+            lo, hi = int(idx.min()) - self.margin, int(idx.max()) + 1 + self.margin
+            lo, hi = max(0, lo // 8 * 8), min(n, -(-hi // 8) * 8)
+            out += [lo, hi]
+        return tuple(out)
+
+    def boxes(self, maps):
+        """box() of every map of a [K, nvox] device tensor, with one host
+        synchronisation"""
         import torch
         X, Y, Z = self.shape
-        f = torch.empty((1, X, Y, Z, denoise.NFEAT), dtype=torch.bfloat16, device="cuda")
+        K = maps.shape[0]
+        if not self.roi:
+            return [(0, X, 0, Y, 0, Z)] * K
+        p = maps.view(K, Z, Y, X)
+        thr = p.reshape(K, -1).max(dim=1).values * self.thr
+        m = (p > thr.view(K, 1, 1, 1)).to(torch.float32)
+        # voxels above the threshold per x, y and z slice; the box spans
+        # the central 1 - 2q of them along each axis
+        proj = [m.sum(dim=2).sum(dim=1), m.sum(dim=3).sum(dim=1), m.sum(dim=3).sum(dim=2)]  # x, y, z
+        lims = []
+        for q in proj:
+            n = q.shape[1]
+            c = q.cumsum(dim=1)
+            tot = c[:, -1:]
+            lo = (c <= self.q * tot).sum(dim=1)
+            hi = n - 1 - (c >= (1.0 - self.q) * tot).sum(dim=1) + 1
+            hi = torch.where(tot[:, 0] > 0, hi, torch.full_like(hi, -1))
+            lims.append(torch.stack([lo, hi], dim=1))
+        L = torch.stack(lims, dim=1).cpu().numpy()  # [K, axis, lo/hi]
+        out = []
+        for k in range(K):
+            r = []
+            for a, n in enumerate((X, Y, Z)):
+                lo, hi = int(L[k, a, 0]), int(L[k, a, 1])
+                if hi < 0:
+                    r += [0, n]
+                    continue
+                lo, hi = lo - self.margin, hi + 1 + self.margin
+                r += [max(0, lo // 8 * 8), min(n, -(-hi // 8) * 8)]
+            out.append(tuple(r))
+        return out
+
+    @staticmethod
+    def union(boxes):
+        b = list(boxes)
+        return (min(x[0] for x in b), max(x[1] for x in b), min(x[2] for x in b), max(x[3] for x in b),
+                min(x[4] for x in b), max(x[5] for x in b))
+
+    def common_size(self, boxes):
+        """every box widened about its centre to the largest extent of each
+        axis among them (8-voxel cells, inside the grid): one U-Net buffer
+        size for all"""
+        b = list(boxes)
+        X, Y, Z = self.shape
+        ext = [max(x[2 * a + 1] - x[2 * a] for x in b) for a in range(3)]
+        out = []
+        for x in b:
+            r = []
+            for a, n in enumerate((X, Y, Z)):
+                lo = (x[2 * a] + x[2 * a + 1] - ext[a]) // 2 // 8 * 8
+                lo = min(max(lo, 0), n - ext[a])
+                r += [lo, lo + ext[a]]
+            out.append(tuple(r))
+        return out
+
+    def run(self, dphi, photons=None, box=None):
+        """device fluence (x-fastest) made with `photons` photons -> device
+        denoised fluence; box: the region to run on (default: self.box)"""
+        import torch
+        X, Y, Z = self.shape
+        x0, x1, y0, y1, z0, z1 = self.last_box = box if box is not None else self.box(dphi)
+        cx, cy, cz = x1 - x0, y1 - y0, z1 - z0
+        full = (cx, cy, cz) == (X, Y, Z)
+        if full:
+            sub, vsub = dphi, self.vol
+        else:
+            sub = dphi.view(Z, Y, X)[z0:z1, y0:y1, x0:x1].contiguous().view(-1)
+            vsub = self.vol.view(Z, Y, X)[z0:z1, y0:y1, x0:x1].contiguous().view(-1)
+        f = torch.empty((1, cx, cy, cz, denoise.NFEAT), dtype=torch.bfloat16, device="cuda")
         L = self.net.L
         st = self.net._stream()
-        denoise._chk(L.mcx_dn_features(denoise._ptr(dphi), denoise._ptr(self.vol), denoise._ptr(self.tab),
-                                       int(self.tab.shape[0]), denoise._ptr(f), X, Y, Z, 1, denoise.LOG_EPS,
+        denoise._chk(L.mcx_dn_features(denoise._ptr(sub), denoise._ptr(vsub), denoise._ptr(self.tab),
+                                       int(self.tab.shape[0]), denoise._ptr(f), cx, cy, cz, 1, denoise.LOG_EPS,
                                        denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "features")
         if photons:
             f[..., 6].fill_(denoise.photon_feature(photons))
         pred = self.net.predict(f)
-        out = torch.empty(X * Y * Z, dtype=torch.float32, device="cuda")
-        denoise._chk(L.mcx_dn_inv_features(denoise._ptr(pred), denoise._ptr(out), X, Y, Z, 1, denoise.LOG_EPS,
+        out = torch.empty(cx * cy * cz, dtype=torch.float32, device="cuda")
+        denoise._chk(L.mcx_dn_inv_features(denoise._ptr(pred), denoise._ptr(out), cx, cy, cz, 1, denoise.LOG_EPS,
                                            denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "inv_features")
-        return out
+        if full:
+            return out
+        res = dphi.clone()
+        res.view(Z, Y, X)[z0:z1, y0:y1, x0:x1] = out.view(cz, cy, cx)
+        return res
 
     def download(self, dout):
         return dout.cpu().numpy().reshape(self.shape, order="F")

@@ -321,6 +321,57 @@ def run(cfg=None, **kwargs):
     return output
 
 
+def run_replicas(cfg, k, out=None):
+    """k independent runs of one forward simulation (k a power of two, at
+    most 64) in one launch of the single-simulation kernel; virtual thread
+    idx belongs to run idx % k, so each run has nthread / k streams and
+    about cfg['nphoton'] photons.  Returns {"cw": float32 [k, Nx, Ny, Nz]
+    time-integrated flux (sum over gates x tstep, each run normalised by
+    its own launched energy), "energytot", "energyabs": [k] FP64,
+    "clear_ms", "kernel_ms", "reduce_ms" (device time of the tally clear,
+    the transport kernel and the reduction), "wall_ms"}.  out: an optional
+    device tensor of k * nvox float32 (x fastest) that receives the maps
+    instead (then "cw" is absent)."""
+    user_cfg = dict(cfg)
+    user_cfg.setdefault("issavedet", 0)
+    L = _lib.lib()
+    c = Config()
+    gpu = ctypes.POINTER(GPUInfo)()
+    try:
+        _parse_config(user_cfg, c)
+        ndev = L.mcx_list_gpu(ctypes.byref(c), ctypes.byref(gpu))
+        _check()
+        if not ndev:
+            raise RuntimeError("No GPU device found")
+        L.mcx_validatecfg(ctypes.byref(c), None, None, 0)
+        _check()
+        nx, ny, nz = c.dim.x, c.dim.y, c.dim.z
+        if out is not None:
+            if out.numel() != int(k) * nx * ny * nz or not out.is_cuda:
+                raise ValueError("out must be a device tensor of k * nvox floats")
+            cw = None
+            dst = ctypes.cast(ctypes.c_void_p(out.data_ptr()), ctypes.POINTER(ctypes.c_float))
+        else:
+            cw = np.empty((int(k), nz, ny, nx), dtype=np.float32)
+            dst = cw.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        energy = np.zeros(2 * int(k), dtype=np.float64)
+        stats = np.zeros(4, dtype=np.float64)
+        rc = L.mcx_amd_run_replicas(ctypes.byref(c), gpu, int(k), dst,
+                                    energy.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    stats.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if rc != 0:
+            raise RuntimeError("mcx_amd_run_replicas failed: %s" % (_lib.last_error(),))
+    finally:
+        if gpu:
+            L.mcx_cleargpuinfo(ctypes.byref(gpu))
+        L.mcx_clearcfg(ctypes.byref(c))
+    res = {"energytot": energy[1::2].copy(), "energyabs": energy[1::2] - energy[0::2], "clear_ms": float(stats[0]),
+           "kernel_ms": float(stats[1]), "reduce_ms": float(stats[2]), "wall_ms": float(stats[3])}
+    if cw is not None:
+        res["cw"] = np.ascontiguousarray(cw.transpose(0, 3, 2, 1))
+    return res
+
+
 def gpuinfo():
     """List HIP devices as dicts (src/pmcx.cpp:1404-1439)."""
     L = _lib.lib()

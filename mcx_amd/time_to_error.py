@@ -11,19 +11,21 @@ voxel is reliable where that mean's coefficient of variation, estimated
 from the 10 runs, is below 1 %.  The error of a map is the median over
 reliable voxels of |x - ref| / ref.
 
-Plain arm: mcx_run_simulation (FP64 tally, autopilot) at increasing photon
-counts; the error is fitted as c / sqrt(N), the count reaching 5 % is run,
-and its launch+sync wall time (the reference's runtime definition,
+Plain arm: the single-simulation kernel on one run (FP64 tally,
+autopilot) at increasing photon counts; the error is fitted as
+c / sqrt(N), the count reaching 5 % is run three times, and the median of
+their launch+sync wall times (the reference's runtime definition,
 src/mcx_core.cu:4009-4012) is the plain time to 5 %.
 
-Batched + denoised arm: K runs of the head volume with independent seeds
-in one persistent launch (mcx_run_batch, FP64 tally, launch shape chosen
-by its autotuner, time-integrated maps reduced on the device); each map is denoised on the GPU (feature kernel, U-Net, inverse
-kernel).  Time per volume = the launch's device time
-(clear, transport, energy sums, merge, reduction) / K + one volume's
-denoising.  The host round trip of the maps is not counted (reported
-separately as host_ms).  The photon count reaching 5 % is found by a
-log-log fit of the denoised error and then run to confirm it.
+Batched + denoised arm: K runs of the head volume in one launch of the
+same kernel (mcx_amd_run_replicas: virtual thread idx belongs to run
+idx % K, each run with its own streams, tally and launched energy; the
+time-integrated maps are reduced on the device); each map is denoised on
+the GPU (feature kernel, U-Net, inverse kernel).  Time per volume = the
+launch's device time (tally clear, transport, reduction) / K + one
+volume's denoising.  The photon count reaching 5 % is found by a log-log
+fit of the denoised error and then run to confirm it.  Maps and scores
+stay on the device.
 """
 import os
 import time
@@ -53,6 +55,41 @@ def _batched(hc, nph, seeds, shape=None):
     return dx.cw_fluence(res, cfgs), float(info["total_ms"]), info
 
 
+class _Scorer:
+    """Median voxel error against the reference on the device: the
+    reference's values at its reliable voxels and their indices in the
+    x-fastest order the engine writes maps in."""
+
+    def __init__(self, ref, mask):
+        import torch
+        idx = np.nonzero(np.asarray(mask).ravel(order="F"))[0]
+        self.idx = torch.from_numpy(idx.astype(np.int64)).cuda()
+        self.ref = torch.from_numpy(np.asarray(ref, dtype=np.float64).ravel(order="F")[idx]).cuda()
+
+    def __call__(self, est):
+        """est: device map (x fastest, any float type) -> median |est - ref| / ref,
+        the mean of the two middle values for an even count as numpy's"""
+        v = ((est.reshape(-1)[self.idx].double() - self.ref).abs() / self.ref).sort().values
+        n = v.numel()
+        return float((v[(n - 1) // 2] + v[n // 2]) / 2)
+
+
+def _replicas(hc, nph, K, seed):
+    """K runs of the head volume in one launch of the single-simulation
+    kernel (mcx_amd_run_replicas: virtual thread idx belongs to run
+    idx % K), the time-integrated maps left on the device: (device maps
+    [K, nvox] in grid units, x fastest; device ms of the tally clear +
+    transport + reduction; the run's statistics)."""
+    import torch
+    import mcx_amd
+    nvox = int(np.prod(hc["vol"].shape))
+    out = torch.empty((K, nvox), dtype=torch.float32, device="cuda")
+    r = mcx_amd.run_replicas(dict(hc, nphoton=int(nph), seed=int(seed)), K, out=out)
+    u = float(hc.get("unitinmm", 1.0))
+    out.mul_(u * u)
+    return out, r["clear_ms"] + r["kernel_ms"] + r["reduce_ms"], r
+
+
 def reference(hc, runs=10, photons=1e8, log=print):
     t0 = time.perf_counter()
     maps, ms, info = _batched(hc, photons, [1000 + k for k in range(runs)])
@@ -62,14 +99,14 @@ def reference(hc, runs=10, photons=1e8, log=print):
     return ref, mask
 
 
-def plain_arm(hc, ref, mask, counts=(1e6, 2e6, 4e6, 8e6), log=print):
-    import mcx_amd
+def plain_arm(hc, score, counts=(1e6, 2e6, 4e6, 8e6), log=print):
+    """mcx_run_simulation's kernel on one run (a replica launch with K = 1:
+    the same kernel and tally, its map reduced on the device); time = the
+    kernel's launch+sync wall time, the reference's runtime."""
 
     def run(nph, seed):
-        c = dict(hc, nphoton=int(nph), seed=int(seed))
-        r = mcx_amd.run(c)
-        st = mcx_amd.last_stats()
-        return dx.median_error(dx.cw_fluence([r], [c])[0], ref, mask), float(st["wall_ms"])
+        m, _, r = _replicas(hc, nph, 1, seed)
+        return score(m[0]), float(r["wall_ms"])
 
     pts = []
     for k, n in enumerate(counts):
@@ -80,43 +117,64 @@ def plain_arm(hc, ref, mask, counts=(1e6, 2e6, 4e6, 8e6), log=print):
     er = np.array([p["error"] for p in pts])
     c = float(np.exp(np.mean(np.log(er) + 0.5 * np.log(ph))))
     n5 = (c / TARGET) ** 2
-    e5, ms5 = run(n5, 31)
-    log("time-to-5%%: plain %.3g photons -> error %.4f in %.1f ms" % (n5, e5, ms5))
-    return {"points": pts, "photons_for_5pct": n5, "ms_to_5pct": ms5, "error_at": e5}
+    # the count reaching 5 %, three independent runs: the median time (a
+    # launch's end is set by its longest histories, which vary run to run)
+    for attempt in range(3):
+        conf = [run(n5, 31 + 3 * attempt + i) for i in range(3)]
+        for e, ms in conf:
+            log("time-to-5%%: plain %.3g photons -> error %.4f in %.1f ms" % (n5, e, ms))
+        e5 = float(np.median([e for e, _ in conf]))
+        ms5 = float(np.median([ms for _, ms in conf]))
+        if e5 <= TARGET:
+            break
+        n5 *= (e5 / TARGET) ** 2 * 1.01  # error ~ c / sqrt(N)
+    return {"points": pts, "photons_for_5pct": n5, "ms_to_5pct": ms5, "error_at": e5,
+            "runs_at_5pct": [{"error": e, "ms": ms} for e, ms in conf]}
 
 
-def denoised_arm(hc, ref, mask, weights, K=8, scales=(1.25e5, 2.5e5, 5e5), log=print):
+def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=print, roi_threshold=3e-6):
+    """K runs of the head volume per launch (replicas), each map denoised on
+    the GPU; time per volume = (tally clear + transport + reduction) / K +
+    one volume's denoising (feature kernel, U-Net, inverse kernel)."""
     import torch
     log("time-to-5%: denoised arm, loading the U-Net")
     net = denoise.UNet(seed=2024)
     net.load_state(np.load(weights)["p"])
-    dn = dx.HeadDenoiser(net, hc)
-    warm = [False]
+    dn = dx.HeadDenoiser(net, hc, roi_threshold=roi_threshold)
 
     def point(nph, salt):
-        log("time-to-5%%: batched launch of %d x %.3g photons" % (K, nph))
         t0 = time.perf_counter()
-        maps, ms, info = _batched(hc, nph, [5000 + 97 * salt + i for i in range(K)])
-        host_ms = (time.perf_counter() - t0) * 1e3 - ms
-        log("time-to-5%%: launch done (%.0f ms on the device at %d photons per lane, %.1f s in all); denoising"
-            % (ms, info["photons_per_lane"], time.perf_counter() - t0))
-        dphis = [dn.upload(m) for m in maps]
-        if not warm[0]:  # the U-Net's buffers are allocated on its first call
-            dn.run(dphis[0], nph)
-            warm[0] = True
+        maps, ms, r = _replicas(hc, nph, K, 5000 + 97 * salt)
+        # the region the U-Net runs on: each map's box, all widened to one
+        # size so the launch's volumes share one set of U-Net buffers
+        # (allocated by an untimed first run)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        boxes = dn.common_size(dn.boxes(maps))
+        ev[1].record()
+        box = boxes[0]
+        dn.run(maps[0], nph, box=box)
         torch.cuda.synchronize()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        outs = [dn.run(d, nph) for d in dphis]
-        b.record()
+        outs = []
+        ev[2].record()
+        for k in range(K):
+            outs.append(dn.run(maps[k], nph, box=boxes[k]))
+        ev[3].record()
         torch.cuda.synchronize()
-        dn_ms = a.elapsed_time(b) / K
-        low = [dx.median_error(m, ref, mask) for m in maps]
-        den = [dx.median_error(dn.download(o), ref, mask) for o in outs]
-        row = {"photons": float(nph), "launch_ms": ms, "ms_per_volume": ms / K + dn_ms, "denoise_ms": dn_ms,
-               "host_ms": host_ms, "low_error": float(np.median(low)), "error": float(np.median(den))}
-        log("time-to-5%%: batched %d x %.3g photons: %.1f ms per volume (launch %.0f ms, denoise %.2f ms), "
-            "error %.4f -> %.4f" % (K, nph, row["ms_per_volume"], ms, dn_ms, row["low_error"], row["error"]))
+        dn_ms = (ev[0].elapsed_time(ev[1]) + ev[2].elapsed_time(ev[3])) / K
+        full = dn.run(maps[0], nph, box=(0, hc["vol"].shape[0], 0, hc["vol"].shape[1], 0, hc["vol"].shape[2]))
+        full_err = score(full)
+        low = [score(maps[k]) for k in range(K)]
+        den = [score(o) for o in outs]
+        row = {"photons": float(nph), "launch_ms": ms, "clear_ms": r["clear_ms"], "kernel_ms": r["kernel_ms"],
+               "reduce_ms": r["reduce_ms"], "ms_per_volume": ms / K + dn_ms, "denoise_ms": dn_ms,
+               "low_error": float(np.median(low)), "error": float(np.median(den)), "box": [int(v) for v in box],
+               "first_error_region": den[0], "first_error_whole_volume": full_err,
+               "seconds": time.perf_counter() - t0}
+        log("time-to-5%%: %d x %.3g photons in one launch: %.2f ms per volume (launch %.0f ms: clear %.1f, kernel %.1f, "
+            "reduce %.1f; denoise %.2f ms on %s), error %.4f -> %.4f (first volume %.4f; on the whole volume %.4f)"
+            % (K, nph, row["ms_per_volume"], ms, r["clear_ms"], r["kernel_ms"], r["reduce_ms"], dn_ms, box,
+               row["low_error"], row["error"], den[0], full_err))
         return row
 
     pts = [point(n, k) for k, n in enumerate(scales)]
@@ -125,12 +183,18 @@ def denoised_arm(hc, ref, mask, weights, K=8, scales=(1.25e5, 2.5e5, 5e5), log=p
     slope, icpt = np.polyfit(np.log(ph), np.log(er), 1)
     n5 = float(np.exp((np.log(TARGET) - icpt) / slope)) if slope < 0 else float(ph[-1] * 4)
     n5 = float(min(max(n5, ph[0] / 4), ph[-1] * 16))
-    conf = point(n5, 40)
+    # run the fitted count; while its error is above the target, scale the
+    # count by the fitted power law and run again
+    for attempt in range(3):
+        conf = point(n5, 40 + attempt)
+        if conf["error"] <= TARGET or slope >= 0:
+            break
+        n5 *= (conf["error"] / TARGET) ** (-1.0 / slope) * 1.01
     return {"k": K, "points": pts, "photons_for_5pct": n5, "ms_per_volume_to_5pct": conf["ms_per_volume"],
-            "error_at": conf["error"], "denoise_ms": conf["denoise_ms"], "host_ms_per_volume": conf["host_ms"] / K}
+            "error_at": conf["error"], "denoise_ms": conf["denoise_ms"], "launch_ms": conf["launch_ms"]}
 
 
-def measure(weights=WEIGHTS, K=8, log=print):
+def measure(weights=WEIGHTS, K=64, log=print):
     """The whole comparison; returns the dict reported by bench.py."""
     if not os.path.exists(weights):
         return {"skipped": "no trained weights at %s" % os.path.relpath(weights)}
@@ -139,8 +203,9 @@ def measure(weights=WEIGHTS, K=8, log=print):
     log("time-to-5%: building the head volume")
     hc = _head()
     ref, mask = reference(hc, log=log)
-    plain = plain_arm(hc, ref, mask, log=log)
-    den = denoised_arm(hc, ref, mask, weights, K=K, log=log)
+    score = _Scorer(ref, mask)
+    plain = plain_arm(hc, score, log=log)
+    den = denoised_arm(hc, score, weights, K=K, log=log)
     out = {"workload": "head256: 256^3 five-label head volume (seed 99), 0.5 mm voxels, 20 gates x 0.25 ns, "
                        "Russian roulette below %g; reference 10 x 1e8 photons" % MINENERGY,
            "reliable_voxels": int(mask.sum()), "plain": plain, "batched_denoised": den,
