@@ -142,7 +142,9 @@ def bench_batch(args):
                    "graph_ms": round(g_ms, 3), "lanes": info_w["lanes"], "photons_per_lane": info_w["photons_per_lane"],
                    "block_size": info_w["block_size"], "tally_cache_cells": info_w["tally_cache_cells"],
                    "vgprs": info_w["vgprs"],
-                   "steps_per_photon": round(steps_pp, 2), "deposits_per_photon": round(dep_pp, 2)},
+                   "steps_per_photon": round(steps_pp, 2), "deposits_per_photon": round(dep_pp, 2),
+                   "lane_occupancy": round(ctr["nstep"] / (64.0 * ctr["wave_iterations"]), 4)
+                   if ctr.get("wave_iterations") else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3),
@@ -287,13 +289,16 @@ def main():
                 nph, ["FP32+shadow", "FP32 hi/lo", "FP64", "FP64 per-XCD"][args.tally]),
             "kernel_ms": round(k_ms, 3),
             "launch_sync_ms": round(sum(wall_ms) / len(wall_ms), 3),
-            "lanes": int(stats_w["nthread"]),
+            "lanes": int(stats_w.get("lanes", stats_w["nthread"])),
+            "virtual_threads": int(stats_w["nthread"]),
             "vgprs": int(stats_w["vgprs"]),
             "steps_per_photon": round(steps_pp, 2),
             "deposits_per_photon": round(dep_pp, 2),
             "lds_box_deposits_per_photon": round(hot_pp, 2),
             "lds_box_cells": int(stats_w["hot_cells"]),
             "global_atomics_per_photon": round(glob_pp, 2),
+            "lane_occupancy": round(stats_w["nstep"] / (64.0 * stats_w["wave_iterations"]), 4)
+            if stats_w.get("wave_iterations") else None,
             "tally_blocks": bool(stats_w.get("tally_blocks", 0)),
             "absorbed_fraction": round(absorbed, 6),
         },

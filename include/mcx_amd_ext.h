@@ -71,6 +71,17 @@ int  mcx_amd_set_tally_blocks(int mode);
  * wall ms.  Returns 0, or the error id (message via mcx_last_error). */
 int  mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* energy, double* stats);
 
+/* Lane refill under autopilot (default on): mcx_run_simulation and
+ * mcx_amd_run_replicas with autopilot launch one lane per resident SIMD
+ * slot and R = round(photons per lane / 4), at most 4, times as many
+ * virtual threads (nthread = R x lanes, each with its own RNG stream and
+ * the reference's photon split); a lane whose virtual thread has launched
+ * all its photons takes the next unstarted one, so lanes keep working until
+ * the last virtual threads start rather than idling until the slowest lane
+ * of their wave is done.  The results are those of a launch with one lane
+ * per virtual thread and the same nthread.  Returns the previous setting. */
+int  mcx_amd_set_refill(int enable);
+
 /* Numbering of replayed photons (seed = SEED_FROM_FILE) in subsequent
  * mcx_run_simulation calls.  0 (default): photon k of virtual thread idx is
  * saved photon idx*threadphoton + min(idx, oddphotons) + k, which visits
@@ -95,6 +106,10 @@ int  mcx_amd_set_replay_index(int reference);
  *  out[14] 1 when the tally was blocked (mcx_amd_set_tally_blocks)
  *  out[15] global tally atomic requests of the lanes (with counters; a
  *          blocked tally's block flush counts once)
+ *  out[16] step-loop iterations summed over waves (with counters): out[2]
+ *          / (64 x out[16]) is the mean fraction of a wave's lanes
+ *          stepping
+ *  out[17] lanes launched (out[7] virtual threads; more with refill)
  * Returns the number of values written (<= n). */
 int  mcx_amd_last_stats(double* out, int n);
 
@@ -139,6 +154,8 @@ typedef struct McxBatchOptions {
                                 > 0 at most this many cells, 0 autotuned (the largest that fits when the
                                 autotuner does not run) */
     int   chosen_tally_cache_cells; /* out: cells per wave used (0: none) */
+    double wave_iterations;  /* out, with counters: step-loop iterations summed over waves; counters[0] /
+                                (64 x this) is the mean fraction of a wave's lanes stepping */
 } McxBatchOptions;
 
 /* Runs `ncfg` independent simulations in one persistent kernel launch.

@@ -42,6 +42,13 @@ def set_tally_blocks(mode):
     return old
 
 
+def set_refill(enable):
+    """Autopilot lanes run several virtual threads each, taking the next
+    unstarted one when theirs is done (default on).  Returns the previous
+    setting."""
+    return bool(_lib.lib().mcx_amd_set_refill(1 if enable else 0))
+
+
 def set_replay_index(reference):
     """Number replayed photons as the reference does (True) or contiguously
     (False, default).  Returns the previous setting."""

@@ -114,7 +114,7 @@ ABI_SYMBOLS = [
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_set_replay_index",
-    "mcx_amd_set_tally_blocks",
+    "mcx_amd_set_tally_blocks", "mcx_amd_set_refill",
     "mcx_amd_run_replicas",
     "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
@@ -160,6 +160,8 @@ def _setup(L):
     L.mcx_amd_run_replicas.argtypes = [P(Config), P(GPUInfo), ctypes.c_int, P(ctypes.c_float), P(ctypes.c_double),
                                        P(ctypes.c_double)]
     L.mcx_amd_run_replicas.restype = ctypes.c_int
+    L.mcx_amd_set_refill.argtypes = [ctypes.c_int]
+    L.mcx_amd_set_refill.restype = ctypes.c_int
     L.mcx_amd_set_tally_blocks.argtypes = [ctypes.c_int]
     L.mcx_amd_set_tally_blocks.restype = ctypes.c_int
     L.mcx_amd_set_replay_index.argtypes = [ctypes.c_int]
@@ -208,7 +210,7 @@ class McxBatchOptions(ctypes.Structure):
         ("chosen_block_size", ctypes.c_int), ("vgprs", ctypes.c_int), ("kernel_ms", ctypes.c_float),
         ("total_ms", ctypes.c_float), ("photons", ctypes.c_double), ("lanes", ctypes.c_double),
         ("counters", ctypes.c_double * 4), ("tally_cache_cells", ctypes.c_int),
-        ("chosen_tally_cache_cells", ctypes.c_int),
+        ("chosen_tally_cache_cells", ctypes.c_int), ("wave_iterations", ctypes.c_double),
     ]
 
 
@@ -271,9 +273,9 @@ def atomic_bench(mode, nelem, nper, nblocks):
 
 
 def last_stats():
-    out = (ctypes.c_double * 16)()
-    n = lib().mcx_amd_last_stats(out, 16)
+    out = (ctypes.c_double * 18)()
+    n = lib().mcx_amd_last_stats(out, 18)
     keys = ["kernel_ms", "wall_ms", "nstep", "ndeposit", "nscatter", "nlaunch", "vgprs", "nthread", "nblock",
             "nlaunches", "errlanes", "hot_cells", "lds_bytes", "ndeposit_hot", "tally_blocks",
-            "tally_requests"]
+            "tally_requests", "wave_iterations", "lanes"]
     return dict(zip(keys, list(out[:n])))

@@ -100,7 +100,8 @@ def run_batch(cfgs, tally=1, photons_per_lane=0, block_size=0, use_graph=False, 
                 "total_ms": float(opt.total_ms), "photons_per_lane": int(opt.chosen_photons_per_lane),
                 "block_size": int(opt.chosen_block_size), "vgprs": int(opt.vgprs),
                 "tally_cache_cells": int(opt.chosen_tally_cache_cells),
-                "counters": dict(zip(("nstep", "ndeposit", "nscatter", "nlaunch"), list(opt.counters)))}
+                "counters": dict(zip(("nstep", "ndeposit", "nscatter", "nlaunch"), list(opt.counters)),
+                                 wave_iterations=float(opt.wave_iterations))}
     finally:
         if gpu:
             L.mcx_cleargpuinfo(ctypes.byref(gpu))

@@ -103,7 +103,7 @@ class BatchRun {
      * time-integrated map (nvox floats), 0: none.  Returns the device time
      * of the time-integrated reduction in ms (0 without it). */
     float download(hipStream_t s, int fields);
-    void read_counters(hipStream_t s, double* out);
+    void read_counters(hipStream_t s, double* out, double* wave_iterations);
 
     double photons = 0.0;
     unsigned long long lanes = 0;
@@ -488,13 +488,14 @@ float BatchRun::download(hipStream_t s, int fields) {
     return cw_ms;
 }
 
-void BatchRun::read_counters(hipStream_t s, double* out) {
-    uint64_t c[4] = {0, 0, 0, 0};
+void BatchRun::read_counters(hipStream_t s, double* out, double* wave_iterations) {
+    uint64_t c[7] = {0, 0, 0, 0, 0, 0, 0};
     if (counters_enabled()) {
         HIP_CHECK(hipMemcpyAsync(c, dzero_.as<char>() + counters_off_, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
     for (int i = 0; i < 4; i++) out[i] = (double)c[i];
+    *wave_iterations = (double)c[6];
 }
 
 /* Autotuner.  Photons per lane: a wave runs as long as its slowest lane,
@@ -637,7 +638,7 @@ static int run_batch_impl(Config** cfgs, int n, GPUInfo* gpu, McxBatchOptions* o
     phase("run done", t0);
     if (opt->download < 0 || opt->download > 2) MCX_THROW(-1, "mcx_run_batch: download must be 0, 1 or 2");
     const float cw_ms = r.download(st.s, opt->download);
-    r.read_counters(st.s, opt->counters);
+    r.read_counters(st.s, opt->counters, &opt->wave_iterations);
     phase("download done", t0);
     for (int i = 0; i < n; i++) cfgs[i]->runtime = (unsigned int)llround(tms + cw_ms);
     opt->kernel_ms = kms;

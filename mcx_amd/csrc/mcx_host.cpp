@@ -45,7 +45,7 @@ extern "C" hipError_t mcxk_launch_merge_blocked(float* out, const double* a, uin
 namespace mcx {
 
 static std::mutex g_statmu;
-static double g_stats[16];
+static double g_stats[18];
 static int g_tally_mode = MCX_TALLY_F64;
 static int g_counters = 0;
 static int g_hotbox = 1;
@@ -57,6 +57,8 @@ static int g_replayindex = 0;
  * voxel, deposits are few per step, the atomics do not bound the kernel
  * and the blocks only add instructions */
 static int g_tblock = 2;
+/* autopilot lanes run several virtual threads each (KParam::refill) */
+static int g_refill = 1;
 /* a request for K replicas of the next run (mcx_amd_run_replicas) */
 struct ReplicaReq {
     int k;
@@ -433,6 +435,19 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     /* work split (src/mcx_core.cu:2980-3022): one device takes all photons */
     const size_t gpuphoton = cfg->nphoton * nrep;
     if (gpuphoton == 0) return;
+    /* Autopilot launches one lane per resident SIMD slot; with refill those
+     * lanes run R times as many virtual threads, each lane taking the next
+     * unstarted one when its current one is done (R up to 4, about 4 photons
+     * per virtual thread at least), so lanes stay busy until the last
+     * virtual threads start instead of waiting on the slowest lane of
+     * their wave */
+    const unsigned int nlanes = (unsigned int)G.autothread;
+    unsigned int refill = 1;
+    if (cfg->autopilot && g_refill && cfg->respin >= 1) {
+        const double ppl = (double)gpuphoton / (double)nlanes;
+        refill = (unsigned int)std::max(1.0, std::min(4.0, std::floor(ppl / 4.0 + 0.5)));
+    }
+    G.autothread = (int)(nlanes * refill);
     int threadphoton, oddphotons;
     const unsigned int nthread = (unsigned int)G.autothread;
     if (cfg->respin >= 1) {
@@ -494,7 +509,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     dseeds.alloc((size_t)nthread * 4 * sizeof(uint32_t));
     if (savedet) ddet.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton * hostdetreclen) * sizeof(float));
     dcount.alloc(4 * sizeof(uint32_t));
-    dcounters.alloc(6 * sizeof(uint64_t));
+    dcounters.alloc(7 * sizeof(uint64_t));
     const bool saveseed = savedet && cfg->issaveseed;
     if (saveseed) dsaved.alloc(std::max<size_t>(1, (size_t)cfg->maxdetphoton) * 16);
     if (replay) {
@@ -540,7 +555,9 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     kp.replaydetid = drdetid.as<int32_t>();
     kp.replayindex = g_replayindex;
     kp.nmaps_mask = nrep - 1;
-    kp.mapplanes = (uint32_t)(fieldlen / nvox);
+    kp.refill = refill > 1 ? 1u : 0u;
+    kp.nlanes = nlanes;
+    kp.nvthreads = (uint32_t)std::min<size_t>(nthread, (size_t)threadphoton * nthread + oddphotons);
     kp.omega = cfg->omega;
     kp.rfshift = rf ? (int32_t)(maxgate * blocks) : 0;
 
@@ -554,12 +571,12 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const size_t lane_blk = tblock ? 40 : 0;
     if (cfg->autopilot && g_hotbox && tally != MCX_TALLY_F32_SHADOW && maxgate > 0) {
         const unsigned int whole = 64u * 4u * (unsigned int)waves_per_simd(numregs);
-        if (whole > block && whole <= (unsigned int)maxthreads && nthread % whole == 0 &&
+        if (whole > block && whole <= (unsigned int)maxthreads && nlanes % whole == 0 &&
             (size_t)ntable * 16 + (size_t)whole * lane_lds <= 16384 &&
             (size_t)ntable * 16 + (size_t)whole * (lane_lds + lane_blk) <= 65536)
             block = whole;
     }
-    const unsigned int grid = nthread / block;
+    const unsigned int grid = nlanes / block;
     const size_t shmem0 = (size_t)ntable * 16 + (size_t)block * lane_lds;
     if (shmem0 > 65536) MCX_THROW(-1, "per-workgroup LDS request exceeds 64 KB; reduce nblocksize or the media count");
     size_t shmemb = shmem0;
@@ -580,7 +597,10 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             tblock = false;
         }
     }
-    const size_t shmem = set_hot_box(cfg, tally, block, numregs, shmemb, kp);
+    kp.mapstride = tblock ? (uint64_t)(fieldlen / nvox) * bnz * bny * bnx * 8 : (uint64_t)fieldlen;
+    /* replicas: the box's cells would mix the runs of a workgroup's lanes
+     * (and each run's source cells see a K-th of the deposits) */
+    const size_t shmem = nrep > 1 ? shmemb : set_hot_box(cfg, tally, block, numregs, shmemb, kp);
 
     if (cfg->flog && cfg->printnum >= 0) {
         fprintf(cfg->flog, "GPU=%d (%s) threadph=%d extra=%d np=%.0f nthread=%u maxgate=%u repetition=%d\n", gpuid + 1, G.name,
@@ -704,7 +724,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
     cfg->runtime = (unsigned int)llround(wall_ms);
 
-    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t ctr[7] = {0, 0, 0, 0, 0, 0, 0};
     if (g_counters) {
         HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
     }
@@ -726,6 +746,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
         g_stats[13] = (double)ctr[4];
         g_stats[14] = tblock ? 1.0 : 0.0;
         g_stats[15] = (double)ctr[5];
+        g_stats[16] = (double)ctr[6];
+        g_stats[17] = nlanes;
     }
 
     /* normalisation (src/mcx_core.cu:3823-3925) */
@@ -956,6 +978,12 @@ int mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* en
     return 0;
 }
 
+int mcx_amd_set_refill(int enable) {
+    int old = g_refill;
+    g_refill = enable ? 1 : 0;
+    return old;
+}
+
 int mcx_amd_set_tally_blocks(int mode) {
     if (mode < 0 || mode > 2) return -1;
     int old = g_tblock;
@@ -971,7 +999,7 @@ int mcx_amd_set_hot_box(int enable) {
 
 int mcx_amd_last_stats(double* out, int n) {
     std::lock_guard<std::mutex> lk(g_statmu);
-    int k = std::min(n, 16);
+    int k = std::min(n, 18);
     for (int i = 0; i < k; i++) out[i] = g_stats[i];
     return k;
 }

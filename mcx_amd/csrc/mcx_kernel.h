@@ -101,8 +101,12 @@ struct KParam {
      * request, so a block's cells cost one request instead of one each. */
     uint32_t tblock, bnx, bny, bnz, blk_off, bq_off;
     /* Replicas: virtual thread idx tallies into replica idx & nmaps_mask,
-     * whose gate planes start mapplanes x replica planes on (0: one run) */
-    uint32_t nmaps_mask, mapplanes;
+     * whose FP64 tally starts mapstride elements x replica on (0: one run) */
+    uint32_t nmaps_mask;
+    uint64_t mapstride;
+    /* refill = 1: nlanes lanes run the nvthreads virtual threads, lane l
+     * first thread l, then each the next unstarted one (detcount[3]) */
+    uint32_t refill, nlanes, nvthreads;
 };
 
 

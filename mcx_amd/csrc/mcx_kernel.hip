@@ -904,7 +904,35 @@ struct Lane {
             savedet(live && (isdet & DET_MASK) == DET_MASK && (mediaid == 0 || (SPLIT && sv_label() == 0)), isdet);
             for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;
         }
-        if ((int)f.w >= nphoton - 1) return 1;
+        if ((int)f.w >= nphoton - 1) {
+            /* With kp.refill a lane runs virtual thread after virtual
+             * thread: once one has launched all its photons the lane takes
+             * the next unstarted one from a global counter (detcount[3]),
+             * so lanes keep stepping until every virtual thread has started
+             * instead of idling while the rest of their wave finishes.
+             * Each virtual thread keeps its own RNG stream, photon count
+             * and energy sums, so the results are those of a launch with
+             * one lane per virtual thread. */
+            if (!kp.refill) return 1;
+            glb(kp.genergy)[2 * idx] = esc;
+            glb(kp.genergy)[2 * idx + 1] = launched;
+            const uint32_t next = kp.nlanes + atom_add_u32(kp.detcount + 3, 1u);
+            if (next >= kp.nvthreads) return 1;
+            if (TALLY == MCX_TALLY_F64 && kp.nmaps_mask) {
+                /* the open block belongs to the old run's tally */
+                block_finish();
+                tally64 = kp.field64 + (size_t)(next & kp.nmaps_mask) * kp.mapstride;
+            }
+            idx = next;
+            nphoton = kp.threadphoton + ((int)idx < kp.oddphotons);
+            esc = glb(kp.genergy)[2 * idx];
+            launched = glb(kp.genergy)[2 * idx + 1];
+            const MCX_GLB uint32_t* sd = glb(kp.seeds) + 4 * idx;
+            t0 = ((uint64_t)sd[0] << 32) | sd[1];
+            t1 = ((uint64_t)sd[2] << 32) | sd[3];
+            f.w = -1.f;
+            for (uint32_t i = 0; i < kp.partialdata; i++) ppath(i) = 0.f;
+        }
         /* src/mcx_core.cu:1205-1215: a replayed photon restarts from its
          * saved RNG state; with seed saving the launch state is kept (in
          * this lane's LDS column after the partial paths) until detection */
@@ -1043,7 +1071,6 @@ struct Lane {
      * loop iteration), which may wait in the lane's block */
     __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight, bool main = false) {
         ndep++;
-        tshift += (int)((idx & kp.nmaps_mask) * kp.mapplanes);
         if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
             const uint32_t tz = __umulhi(vox, kp.hot_mxy);
             const uint32_t z = (tz + ((vox - tz) >> (kp.hot_shxy & 0xffu))) >> (kp.hot_shxy >> 8), r = vox - z * kp.dimxy;
@@ -1134,6 +1161,11 @@ struct Lane {
             atomicAdd((unsigned long long*)kp.counters + 3, (unsigned long long)nlaunch);
             if (kp.hot_cells) atomicAdd((unsigned long long*)kp.counters + 4, (unsigned long long)nhot);
             atomicAdd((unsigned long long*)kp.counters + 5, (unsigned long long)nreq);
+            /* the wave loops until its last lane is done: nstep + 1
+             * iterations for its longest-stepping lane */
+            const uint32_t wit = __reduce_max_sync(__activemask(), nstep) + 1u;
+            if (__lane_id() == (uint32_t)__builtin_amdgcn_readfirstlane(__lane_id()))
+                atomicAdd((unsigned long long*)kp.counters + 6, (unsigned long long)wit);
         }
     }
 
@@ -1154,6 +1186,7 @@ struct Lane {
         pend_i = ~(size_t)0;
         tally64 = kp.field64;
         if (TALLY == MCX_TALLY_F64_XCD) tally64 += (size_t)xcc_id() * kp.fieldlen;
+        if (TALLY == MCX_TALLY_F64 && kp.nmaps_mask) tally64 += (size_t)(idx & kp.nmaps_mask) * kp.mapstride;
         bopen = MCX_NO_BLOCK;
         if (BLOCKED && kp.tblock) {
             extern __shared__ F4 lds[];

@@ -55,6 +55,52 @@ def head_crop_volume(seed, n=128, crop=64):
     return cfg
 
 
+def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 40.0), high=1e8, minenergy=1e-4,
+                  edge=64, seed=6000, log=print):
+    """Training pairs from full-size head volumes (workloads.head256's
+    generator at 256^3, seeds seed.. seed+nheads-1, one CW gate): per head
+    one target run of `high` photons and maps_per_head input runs with
+    photon counts log-uniform in [low/span[0], low*span[1]], each map cut
+    into crops_per_map edge^3 crops spread over the region the test volume's
+    error is measured on (x, y origins 32-160, z origins 128-192, mirrored in
+    z so the source side comes first).  Runs are single-run replica launches
+    with the maps left on the device; only the crops come back.  Returns
+    (features [n, edge, edge, edge, 8] float32, targets [n, edge^3])."""
+    import torch
+    import mcx_amd
+    g = np.random.default_rng(seed)
+    feats, targs = [], []
+    t0 = time.perf_counter()
+    for h in range(nheads):
+        hc = workloads.head256(nphoton=int(high), seed=int(seed + h), n=256, tstep=5e-9, quiet=True, minenergy=minenergy)
+        X, Y, Z = hc["vol"].shape
+        u = float(hc.get("unitinmm", 1.0))
+        tgt = torch.empty((1, X * Y * Z), dtype=torch.float32, device="cuda")
+        mcx_amd.run_replicas(dict(hc, seed=int(100000 + h)), 1, out=tgt)
+        tgt3 = tgt.view(Z, Y, X)
+        vol = np.asarray(hc["vol"])
+        for m in range(maps_per_head):
+            n = float(np.exp(g.uniform(np.log(low / span[0]), np.log(low * span[1]))))
+            inp = torch.empty((1, X * Y * Z), dtype=torch.float32, device="cuda")
+            mcx_amd.run_replicas(dict(hc, nphoton=int(n), seed=int(200000 + 16 * h + m)), 1, out=inp)
+            inp3 = inp.view(Z, Y, X)
+            for c in range(crops_per_map):
+                x0, y0 = (int(v) for v in g.integers(32, 161, size=2))
+                z0 = int(g.integers(128, 193))
+                sl = (slice(x0, x0 + edge), slice(y0, y0 + edge), slice(z0, z0 + edge))
+                # device maps are [z][y][x]; crops back in [x, y, z], mirrored in z
+                def cut(t3):
+                    return t3[z0:z0 + edge, y0:y0 + edge, x0:x0 + edge].permute(2, 1, 0).cpu().numpy()[:, :, ::-1] * (u * u)
+                v = vol[sl][:, :, ::-1]
+                feats.append(denoise.features(cut(inp3), v, hc["prop"], u, n))
+                t = denoise.log_feature(cut(tgt3)).astype(np.float32)
+                t[(v == 0) | (cut(tgt3) <= 0)] = np.nan
+                targs.append(t.reshape(-1))
+        if log and (h % 8 == 7 or h == nheads - 1):
+            log("head256 training pairs: %d heads, %d crops (%.0f s)" % (h + 1, len(feats), time.perf_counter() - t0))
+    return np.stack(feats), np.stack(targs)
+
+
 def crop_flip(a, cfg):
     """The training crop of a [X, Y, Z] array (identity for uncropped)."""
     if "_crop" not in cfg:
@@ -560,6 +606,9 @@ def main(argv=None):
                     help="batched denoised runs: photons per run as multiples of --head-low")
     ap.add_argument("--batch-plain-scale", default="2,4",
                     help="batched plain runs: photons per run as multiples of --head-low")
+    ap.add_argument("--head256-train", type=int, default=0,
+                    help="also train on crops of this many full-size head volumes (head256_pairs)")
+    ap.add_argument("--head256-high", type=float, default=1e8, help="photons of each full-size head's target run")
     args = ap.parse_args(argv)
     import torch
 
@@ -595,9 +644,19 @@ def main(argv=None):
         hi_ph += ph
         log("train targets: %d of %d volumes (%.0f ms)" % (min(c0 + 64, len(tr)), len(tr), hi_ms))
     log("train targets: %d x %.0e photons, %.0f ms (%.0f photons/ms)" % (len(tr), args.high, hi_ms, hi_ph / hi_ms))
-    X = to_device_bf16(make_features(lo, tr, lows if args.photon_feature else None))
-    T = torch.from_numpy(make_targets(hi, tr)).to("cuda")
+    Xh = make_features(lo, tr, lows if args.photon_feature else None)
+    Th = make_targets(hi, tr)
     del lo, hi
+    if args.head256_train:
+        fx, ft = head256_pairs(args.head256_train, low=args.low, span=(sa, sb), high=args.head256_high,
+                               minenergy=args.minenergy, edge=args.edge, log=log)
+        if not args.photon_feature:
+            fx[..., 6] = 0.0
+        Xh, Th = np.concatenate([Xh, fx]), np.concatenate([Th, ft])
+        del fx, ft
+    X = to_device_bf16(Xh)
+    T = torch.from_numpy(Th).to("cuda")
+    del Xh, Th
     net = denoise.UNet(seed=2024)
     R["params"] = net.count
     losses, train_s = train(net, X, T, args.steps, args.batch, args.lr, log=log)

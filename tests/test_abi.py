@@ -351,7 +351,7 @@ def test_bench_refuses_multi_gpu():
 def test_batch_options_layout_and_empty_batch():
     """McxBatchOptions mirror matches the compiled struct; an empty batch is
     rejected before any device call."""
-    assert ctypes.sizeof(_lib.McxBatchOptions) == 104
+    assert ctypes.sizeof(_lib.McxBatchOptions) == 112
     opt = _lib.McxBatchOptions()
     rc = L.mcx_run_batch(None, 0, None, ctypes.byref(opt))
     assert rc < 0
