@@ -618,6 +618,17 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     uint32_t counts[4] = {0, 0, 0, 0};
     uint32_t errlanes = 0;
     EventPair ev, evc;
+    /* launch statistics returned by mcx_amd_last_stats */
+    auto record_stats = [&]() {
+        uint64_t ctr[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (g_counters) HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
+        std::lock_guard<std::mutex> lk(g_statmu);
+        const double v[18] = {kernel_ms, wall_ms, (double)ctr[0], (double)ctr[1], (double)ctr[2], (double)ctr[3],
+                              (double)numregs, (double)nthread, (double)block, (double)nlaunch, (double)errlanes,
+                              (double)kp.hot_cells, (double)shmem, (double)ctr[4], tblock ? 1.0 : 0.0,
+                              (double)ctr[5], (double)ctr[6], (double)nlanes};
+        std::copy(v, v + 18, g_stats);
+    };
 
     for (unsigned int timegate = 0; timegate < totalgates; timegate += maxgate) {
         kp.twin0 = cfg->tstart + cfg->tstep * timegate;
@@ -661,6 +672,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
                                 tblock ? bnx : 0u, bny, bnz, ev, evc, wall_ms, st.s);
                 if (counts[1]) MCX_THROW(-2, "transport kernel: a lane stopped on an internal error");
                 cfg->runtime = (unsigned int)llround(wall_ms);
+                record_stats();
                 return;
             }
 
@@ -724,31 +736,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
     cfg->runtime = (unsigned int)llround(wall_ms);
 
-    uint64_t ctr[7] = {0, 0, 0, 0, 0, 0, 0};
-    if (g_counters) {
-        HIP_CHECK(hipMemcpy(ctr, dcounters.p, sizeof(ctr), hipMemcpyDeviceToHost));
-    }
-    {
-        std::lock_guard<std::mutex> lk(g_statmu);
-        g_stats[0] = kernel_ms;
-        g_stats[1] = wall_ms;
-        g_stats[2] = (double)ctr[0];
-        g_stats[3] = (double)ctr[1];
-        g_stats[4] = (double)ctr[2];
-        g_stats[5] = (double)ctr[3];
-        g_stats[6] = numregs;
-        g_stats[7] = nthread;
-        g_stats[8] = block;
-        g_stats[9] = nlaunch;
-        g_stats[10] = errlanes;
-        g_stats[11] = kp.hot_cells;
-        g_stats[12] = (double)shmem;
-        g_stats[13] = (double)ctr[4];
-        g_stats[14] = tblock ? 1.0 : 0.0;
-        g_stats[15] = (double)ctr[5];
-        g_stats[16] = (double)ctr[6];
-        g_stats[17] = nlanes;
-    }
+    record_stats();
 
     /* normalisation (src/mcx_core.cu:3823-3925) */
     if (cfg->issave2pt && cfg->isnormalized) {

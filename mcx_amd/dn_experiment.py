@@ -58,21 +58,29 @@ def head_crop_volume(seed, n=128, crop=64):
 def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 40.0), high=1e8, minenergy=1e-4,
                   edge=64, seed=6000, log=print):
     """Training pairs from full-size head volumes (workloads.head256's
-    generator at 256^3, seeds seed.. seed+nheads-1, one CW gate): per head
-    one target run of `high` photons and maps_per_head input runs with
-    photon counts log-uniform in [low/span[0], low*span[1]], each map cut
+    generator at 256^3, seeds seed .. seed+nheads-1, the test volume's 20
+    gates summed over time): per head one target run of `high` photons and
+    maps_per_head input runs with photon counts log-uniform in [low/span[0], low*span[1]], each map cut
     into crops_per_map edge^3 crops spread over the region the test volume's
-    error is measured on (x, y origins 32-160, z origins 128-192, mirrored in
+    error is measured on (x, y origins 48-144, z origins 128-192, mirrored in
     z so the source side comes first).  Runs are single-run replica launches
     with the maps left on the device; only the crops come back.  Returns
     (features [n, edge, edge, edge, 8] float32, targets [n, edge^3])."""
+    from concurrent.futures import ThreadPoolExecutor
     import torch
     import mcx_amd
     g = np.random.default_rng(seed)
     feats, targs = [], []
     t0 = time.perf_counter()
-    for h in range(nheads):
-        hc = workloads.head256(nphoton=int(high), seed=int(seed + h), n=256, tstep=5e-9, quiet=True, minenergy=minenergy)
+
+    def make(h):
+        hc = workloads.head256(nphoton=int(high), seed=int(seed + h), n=256, quiet=True, minenergy=minenergy)
+        hc["autopilot"] = 1
+        return hc
+
+    # the volumes are built on host threads while the GPU runs the previous ones
+    pool = ThreadPoolExecutor(4)
+    for h, hc in enumerate(pool.map(make, range(nheads))):
         X, Y, Z = hc["vol"].shape
         u = float(hc.get("unitinmm", 1.0))
         tgt = torch.empty((1, X * Y * Z), dtype=torch.float32, device="cuda")
@@ -85,7 +93,7 @@ def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 
             mcx_amd.run_replicas(dict(hc, nphoton=int(n), seed=int(200000 + 16 * h + m)), 1, out=inp)
             inp3 = inp.view(Z, Y, X)
             for c in range(crops_per_map):
-                x0, y0 = (int(v) for v in g.integers(32, 161, size=2))
+                x0, y0 = (int(v) for v in g.integers(48, 145, size=2))
                 z0 = int(g.integers(128, 193))
                 sl = (slice(x0, x0 + edge), slice(y0, y0 + edge), slice(z0, z0 + edge))
                 # device maps are [z][y][x]; crops back in [x, y, z], mirrored in z
@@ -98,6 +106,7 @@ def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 
                 targs.append(t.reshape(-1))
         if log and (h % 8 == 7 or h == nheads - 1):
             log("head256 training pairs: %d heads, %d crops (%.0f s)" % (h + 1, len(feats), time.perf_counter() - t0))
+    pool.shutdown()
     return np.stack(feats), np.stack(targs)
 
 

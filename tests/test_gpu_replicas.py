@@ -64,3 +64,7 @@ def test_replicas_head_volume_autopilot():
     assert np.all(np.abs(rep["energytot"] - 250_000) < 2e3)
     a = rep["energyabs"] / rep["energytot"]
     assert np.all(a > 0) and np.all(a < 1) and a.std() < 0.02
+    # the launch statistics describe this launch
+    st = mcx_amd.last_stats()
+    assert st["kernel_ms"] == pytest.approx(rep["kernel_ms"], rel=1e-6)
+    assert st["nlaunches"] == 1 and st["errlanes"] == 0 and st["lanes"] > 0
