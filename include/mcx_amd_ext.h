@@ -71,16 +71,33 @@ int  mcx_amd_set_tally_blocks(int mode);
  * wall ms.  Returns 0, or the error id (message via mcx_last_error). */
 int  mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* energy, double* stats);
 
-/* Lane refill under autopilot (default on): mcx_run_simulation and
+/* Lane refill under autopilot: mcx_run_simulation and
  * mcx_amd_run_replicas with autopilot launch one lane per resident SIMD
- * slot and R = round(photons per lane / 4), at most 4, times as many
- * virtual threads (nthread = R x lanes, each with its own RNG stream and
- * the reference's photon split); a lane whose virtual thread has launched
- * all its photons takes the next unstarted one, so lanes keep working until
- * the last virtual threads start rather than idling until the slowest lane
- * of their wave is done.  The results are those of a launch with one lane
- * per virtual thread and the same nthread.  Returns the previous setting. */
-int  mcx_amd_set_refill(int enable);
+ * slot and R times as many virtual threads (nthread = R x lanes, each with
+ * its own RNG stream and the reference's photon split); a lane whose
+ * virtual thread has launched all its photons takes the next unstarted
+ * one, so lanes keep working until the last virtual threads start rather
+ * than idling until the slowest lane of their wave is done.  The results
+ * are those of a launch with one lane per virtual thread and the same
+ * nthread.  mode 0: off (R = 1); 1 (default): R = photons per lane / 64,
+ * at most 4, so every virtual thread keeps at least 64 photons; 2: R =
+ * photons per lane, at most 32 (about one photon per virtual thread, the
+ * shortest tail).  The first random number of every stream the host
+ * seeding makes is biased (mean 0.4914; the reference draws the same),
+ * which shifts results by O(1/photons per thread) (cube60's absorbed
+ * fraction: +0.018/P relative, below 2.8e-4 at P >= 64): mode 2 trades
+ * that for speed on small runs.  Returns the previous mode, or -1 for a
+ * mode outside 0-2. */
+int  mcx_amd_set_refill(int mode);
+
+/* The per-thread seed words of the host seeding (src/mcx_core.cu:3247-3251,
+ * :3381-3387): srand(seed), then for each word (rand() << 16) |
+ * (rand() << 1) | (rand() >> 14), evaluated left to right, as
+ * mcx_run_simulation draws them for each launch.  ncalls consecutive draws
+ * of counts[i] words each from one generator are written back to back to
+ * out (large draws are generated on several host threads).  Host only; no
+ * counterpart in the reference, which calls rand() inline.  Returns 0. */
+int  mcx_amd_seed_words(unsigned int seed, unsigned int* out, const unsigned long long* counts, int ncalls);
 
 /* Numbering of replayed photons (seed = SEED_FROM_FILE) in subsequent
  * mcx_run_simulation calls.  0 (default): photon k of virtual thread idx is

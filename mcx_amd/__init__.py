@@ -42,11 +42,18 @@ def set_tally_blocks(mode):
     return old
 
 
-def set_refill(enable):
+def set_refill(mode):
     """Autopilot lanes run several virtual threads each, taking the next
-    unstarted one when theirs is done (default on).  Returns the previous
-    setting."""
-    return bool(_lib.lib().mcx_amd_set_refill(1 if enable else 0))
+    unstarted one when theirs is done.  0/False: off; 1/True (default): at
+    least 64 photons per virtual thread, at most 4 virtual threads per lane;
+    2 or "fast": about one photon per virtual thread, at most 32 per lane
+    (shortest tail; each stream's biased first random number then weighs on
+    every photon or two).  Returns the previous mode."""
+    m = 2 if mode == "fast" else int(mode)
+    old = _lib.lib().mcx_amd_set_refill(m)
+    if old < 0:
+        raise ValueError("invalid refill mode %r" % (mode,))
+    return old
 
 
 def set_replay_index(reference):

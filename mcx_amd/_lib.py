@@ -114,7 +114,7 @@ ABI_SYMBOLS = [
     "mcx_validatecfg", "mcx_cleargpuinfo", "mcx_error", "mcx_flush", "mcx_normalize", "mcx_readconfig",
     "mcx_run_from_json", "mcx_savedata", "mcx_savedetphoton", "mcx_float2half2", "mcx_last_error", "mcx_clear_error",
     "mcx_rng_test", "mcx_amd_set_tally_mode", "mcx_amd_set_counters", "mcx_amd_set_hot_box", "mcx_amd_set_replay_index",
-    "mcx_amd_set_tally_blocks", "mcx_amd_set_refill",
+    "mcx_amd_set_tally_blocks", "mcx_amd_set_refill", "mcx_amd_seed_words",
     "mcx_amd_run_replicas",
     "mcx_amd_last_stats",
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
@@ -162,6 +162,8 @@ def _setup(L):
     L.mcx_amd_run_replicas.restype = ctypes.c_int
     L.mcx_amd_set_refill.argtypes = [ctypes.c_int]
     L.mcx_amd_set_refill.restype = ctypes.c_int
+    L.mcx_amd_seed_words.argtypes = [ctypes.c_uint, P(ctypes.c_uint), P(ctypes.c_ulonglong), ctypes.c_int]
+    L.mcx_amd_seed_words.restype = ctypes.c_int
     L.mcx_amd_set_tally_blocks.argtypes = [ctypes.c_int]
     L.mcx_amd_set_tally_blocks.restype = ctypes.c_int
     L.mcx_amd_set_replay_index.argtypes = [ctypes.c_int]
@@ -270,6 +272,19 @@ def atomic_bench(mode, nelem, nper, nblocks):
     if rc != 0:
         raise RuntimeError("atomic bench failed (%d)" % rc)
     return {"atomics_per_s": out[0], "ms": out[1], "sum_ratio": out[2], "xcc_workgroups": list(out[3:11])}
+
+
+def seed_words(seed, counts):
+    """The host seeding's per-thread seed words: consecutive draws of
+    counts[i] words from srand(seed), back to back (uint32 array)."""
+    import numpy as np
+    counts = [int(c) for c in counts]
+    out = np.zeros(max(1, sum(counts)), dtype=np.uint32)
+    cc = (ctypes.c_ulonglong * max(1, len(counts)))(*counts)
+    rc = lib().mcx_amd_seed_words(int(seed), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), cc, len(counts))
+    if rc != 0:
+        raise RuntimeError("mcx_amd_seed_words failed (%d)" % rc)
+    return out[:sum(counts)]
 
 
 def last_stats():

@@ -261,11 +261,7 @@ void BatchRun::setup() {
         /* the single-simulation path's seeding: srand(seed + 0), then
          * Pseed[i] = (rand()<<16) | (rand()<<1) | (rand()>>14) */
         GlibcRand rnd((unsigned int)(cfg->seed > 0 ? cfg->seed : (int)time(0)));
-        uint32_t* sd = hseed.data() + P.off_seed / 4;
-        for (size_t k = 0; k < (size_t)P.nthread * 4; k++) {
-            uint32_t a = (uint32_t)rnd.next(), b = (uint32_t)rnd.next(), c = (uint32_t)rnd.next();
-            sd[k] = (a << 16) | (b << 1) | (c >> 14);
-        }
+        rnd.seed_words(hseed.data() + P.off_seed / 4, (size_t)P.nthread * 4);
     }
     dmedia_.alloc(media_b);
     dprop_.alloc(prop_b);

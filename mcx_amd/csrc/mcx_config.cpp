@@ -76,6 +76,102 @@ int GlibcRand::next() {
     return (int)(v >> 1);
 }
 
+void GlibcRand::state(uint32_t x[31]) const {
+    for (int j = 0; j < 31; j++) x[j] = ring_[(pos_ + j) % 31];
+}
+
+void GlibcRand::set_state(const uint32_t x[31]) {
+    for (int j = 0; j < 31; j++) ring_[j] = x[j];
+    pos_ = 0;
+}
+
+namespace {
+/* One output of the additive generator maps the window x of the 31 most
+ * recent values (oldest first) to x' = A x over Z/2^32: x'[j] = x[j + 1]
+ * for j < 30 and x'[30] = x[0] + x[28]. */
+struct Jump {
+    uint32_t m[31][31];
+};
+
+Jump jump_mul(const Jump& a, const Jump& b) {
+    Jump c;
+    memset(&c, 0, sizeof(c));
+    for (int i = 0; i < 31; i++)
+        for (int k = 0; k < 31; k++) {
+            const uint32_t v = a.m[i][k];
+            if (!v) continue;
+            for (int j = 0; j < 31; j++) c.m[i][j] += v * b.m[k][j];
+        }
+    return c;
+}
+
+/* A^k: the window k outputs further on */
+Jump jump_pow(uint64_t k) {
+    Jump r, a;
+    memset(&r, 0, sizeof(r));
+    memset(&a, 0, sizeof(a));
+    for (int i = 0; i < 31; i++) r.m[i][i] = 1;
+    for (int j = 0; j < 30; j++) a.m[j][j + 1] = 1;
+    a.m[30][0] = a.m[30][28] = 1;
+    while (k) {
+        if (k & 1) r = jump_mul(r, a);
+        a = jump_mul(a, a);
+        k >>= 1;
+    }
+    return r;
+}
+
+void jump_apply(const Jump& a, const uint32_t* x, uint32_t* y) {
+    for (int i = 0; i < 31; i++) {
+        uint32_t v = 0;
+        for (int k = 0; k < 31; k++) v += a.m[i][k] * x[k];
+        y[i] = v;
+    }
+}
+
+/* the generator's values in blocks: v[i] = v[i - 31] + v[i - 3] over a
+ * buffer whose first 31 entries are the window; rand() is v >> 1 */
+void seed_words_seq(GlibcRand& r, uint32_t* out, size_t n) {
+    const size_t kBlock = 1024; /* seed words per block */
+    uint32_t v[31 + 3 * kBlock];
+    r.state(v);
+    for (size_t i0 = 0; i0 < n; i0 += kBlock) {
+        const size_t m = std::min(kBlock, n - i0);
+        for (size_t i = 31; i < 31 + 3 * m; i++) v[i] = v[i - 31] + v[i - 3];
+        for (size_t i = 0; i < m; i++) {
+            const uint32_t a = v[31 + 3 * i] >> 1, b = v[32 + 3 * i] >> 1, c = v[33 + 3 * i] >> 1;
+            out[i0 + i] = (a << 16) | (b << 1) | (c >> 14);
+        }
+        memmove(v, v + 3 * m, 31 * sizeof(uint32_t));
+    }
+    r.set_state(v);
+}
+}  // namespace
+
+void GlibcRand::seed_words(uint32_t* out, size_t n) {
+    const size_t kChunks = 256;
+    if (n < (size_t)1 << 18) {
+        seed_words_seq(*this, out, n);
+        return;
+    }
+    const size_t w = (n + kChunks - 1) / kChunks;
+    const Jump step = jump_pow(3 * (uint64_t)w);
+    std::vector<uint32_t> starts(31 * (kChunks + 1));
+    state(starts.data());
+    for (size_t c = 0; c < kChunks; c++) jump_apply(step, &starts[31 * c], &starts[31 * (c + 1)]);
+#pragma omp parallel for schedule(dynamic)
+    for (long long c = 0; c < (long long)kChunks; c++) {
+        const size_t b = (size_t)c * w, e = std::min(n, b + w);
+        if (b >= e) continue;
+        GlibcRand g(1u);
+        g.set_state(&starts[31 * c]);
+        seed_words_seq(g, out + b, e - b);
+    }
+    uint32_t fin[31];
+    jump_apply(jump_pow(3 * (uint64_t)n), &starts[0], fin);
+    set_state(fin);
+}
+
 unsigned int det_partialdata(const Config* cfg) {
     const unsigned int f = cfg->savedetflag;
     return (cfg->medianum - 1) * (MCX_SAVE_NSCAT(f) + MCX_SAVE_PPATH(f) + MCX_SAVE_MOM(f));

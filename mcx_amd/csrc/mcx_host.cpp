@@ -58,7 +58,22 @@ static int g_replayindex = 0;
  * and the blocks only add instructions */
 static int g_tblock = 2;
 /* autopilot lanes run several virtual threads each (KParam::refill) */
+/* lane refill: 0 off, 1 (default) at least kMinStreamPhotons photons per
+ * virtual thread and at most kDefaultRefill virtual threads per lane, 2
+ * about one photon per virtual thread */
 static int g_refill = 1;
+/* virtual threads per lane at most under refill (each holds 24 bytes of
+ * seeds and energies) */
+static const unsigned int kMaxRefill = 32, kDefaultRefill = 4;
+/* The first xorshift128+ output of a stream seeded with the host's seed
+ * words is not uniform (the words OR overlapping rand() values, so their
+ * bits are biased towards 1): its mean is 0.4914 and -log of it averages
+ * 1.039, so the first photon of each thread starts with a longer
+ * scattering length.  The reference draws the same numbers; the effect on
+ * a result shrinks as 1/(photons per thread) (cube60: +0.018/P relative in
+ * the absorbed fraction), so refill 1 keeps P >= 64: a shift below 2.8e-4,
+ * the precision of the reference's published four-digit answers. */
+static const double kMinStreamPhotons = 64.0;
 /* a request for K replicas of the next run (mcx_amd_run_replicas) */
 struct ReplicaReq {
     int k;
@@ -263,10 +278,7 @@ static void rng_test_path(Config* cfg, GPUInfo* gpu, int gpuid, size_t fieldlen)
     /* no srand() precedes this path in the reference: glibc default state */
     GlibcRand r(1);
     uint32_t seeds[4];
-    for (int i = 0; i < 4; i++) {
-        uint32_t a = (uint32_t)r.next(), b = (uint32_t)r.next(), c = (uint32_t)r.next();
-        seeds[i] = (a << 16) | (b << 1) | (c >> 14);
-    }
+    r.seed_words(seeds, 4);
     Stream st;
     DevMem dseed, dout;
     dseed.alloc(sizeof(seeds));
@@ -414,7 +426,10 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     if (replay && totalgates > maxgate) MCX_THROW(-1, "photon replay needs all time gates in one pass (reduce the gate count)");
     /* RF replay: an imaginary plane follows the real one (src/mcx_core.cu:2909-2911) */
     const bool rf = replay && (cfg->outputtype == otRF || cfg->outputtype == otRFmus);
-    const size_t planelen = nvox * maxgate * blocks;
+    /* replicas return time-integrated maps only: every gate of a pass is
+     * tallied into one plane (the gate windows still end the photons) */
+    const unsigned int tgates = g_rep ? 1u : maxgate;
+    const size_t planelen = nvox * tgates * blocks;
     const size_t fieldlen = planelen * (rf ? 2 : 1);
     /* replicas: K independent runs in one launch, one set of gate planes each */
     const unsigned int nrep = g_rep ? (unsigned int)g_rep->k : 1u;
@@ -437,15 +452,18 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     if (gpuphoton == 0) return;
     /* Autopilot launches one lane per resident SIMD slot; with refill those
      * lanes run R times as many virtual threads, each lane taking the next
-     * unstarted one when its current one is done (R up to 4, about 4 photons
-     * per virtual thread at least), so lanes stay busy until the last
-     * virtual threads start instead of waiting on the slowest lane of
-     * their wave */
+     * unstarted one when its current one is done, so lanes stay busy until
+     * the last virtual threads start instead of waiting on the slowest lane
+     * of their wave.  R = photons per lane / kMinStreamPhotons, at most
+     * kDefaultRefill (refill 1), or photons per lane, at most kMaxRefill
+     * (refill 2: about one photon per virtual thread, the shortest tail) */
     const unsigned int nlanes = (unsigned int)G.autothread;
     unsigned int refill = 1;
     if (cfg->autopilot && g_refill && cfg->respin >= 1) {
         const double ppl = (double)gpuphoton / (double)nlanes;
-        refill = (unsigned int)std::max(1.0, std::min(4.0, std::floor(ppl / 4.0 + 0.5)));
+        const double r = g_refill == 2 ? std::min((double)kMaxRefill, std::floor(ppl + 0.5))
+                                       : std::min((double)kDefaultRefill, std::floor(ppl / kMinStreamPhotons));
+        refill = (unsigned int)std::max(1.0, r);
     }
     G.autothread = (int)(nlanes * refill);
     int threadphoton, oddphotons;
@@ -528,6 +546,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
 
     KParam kp;
     fill_kparam(cfg, maxgate, is2d, kp);
+    kp.maxgate = tgates;
     kp.threadphoton = threadphoton;
     kp.oddphotons = oddphotons;
     kp.nthread = nthread;
@@ -644,11 +663,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             }
             if (savedet) HIP_CHECK(hipMemsetAsync(ddet.p, 0, ddet.bytes, st.s));
             HIP_CHECK(hipMemsetAsync(dcount.p, 0, dcount.bytes, st.s));
-            /* Pseed[i] = (rand()<<16) | (rand()<<1) | (rand()>>14), evaluated left to right */
-            for (size_t i = 0; i < seeds.size(); i++) {
-                uint32_t a = (uint32_t)rnd.next(), b = (uint32_t)rnd.next(), c = (uint32_t)rnd.next();
-                seeds[i] = (a << 16) | (b << 1) | (c >> 14);
-            }
+            rnd.seed_words(seeds.data(), seeds.size());
             HIP_CHECK(hipMemcpyAsync(dseeds.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st.s));
             HIP_CHECK(hipMemcpyAsync(dkp.p, &kp, sizeof(kp), hipMemcpyHostToDevice, st.s));
             HIP_CHECK(hipStreamSynchronize(st.s));
@@ -908,10 +923,7 @@ int mcx_rng_test(unsigned int seed, float* out, unsigned int n) {
     try {
         GlibcRand r(seed);
         uint32_t seeds[4];
-        for (int i = 0; i < 4; i++) {
-            uint32_t a = (uint32_t)r.next(), b = (uint32_t)r.next(), c = (uint32_t)r.next();
-            seeds[i] = (a << 16) | (b << 1) | (c >> 14);
-        }
+        r.seed_words(seeds, 4);
         Stream st;
         DevMem dseed, dout;
         dseed.alloc(sizeof(seeds));
@@ -942,6 +954,22 @@ int mcx_amd_set_replay_index(int reference) {
     return old;
 }
 
+int mcx_amd_seed_words(unsigned int seed, unsigned int* out, const unsigned long long* counts, int ncalls) {
+    EntryGuard g;
+    try {
+        if (!out || !counts || ncalls < 0) MCX_THROW(-1, "seed words: NULL output or negative call count");
+        GlibcRand r(seed);
+        for (int i = 0; i < ncalls; i++) {
+            r.seed_words(out, (size_t)counts[i]);
+            out += counts[i];
+        }
+        return 0;
+    } catch (const Error& e) {
+        record_error(e.id, e.msg.c_str());
+        return e.id;
+    }
+}
+
 int mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* energy, double* stats) {
     EntryGuard g;
     GPUInfo* own = NULL;
@@ -966,9 +994,10 @@ int mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* en
     return 0;
 }
 
-int mcx_amd_set_refill(int enable) {
+int mcx_amd_set_refill(int mode) {
+    if (mode < 0 || mode > 2) return -1;
     int old = g_refill;
-    g_refill = enable ? 1 : 0;
+    g_refill = mode;
     return old;
 }
 

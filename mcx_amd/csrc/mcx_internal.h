@@ -4,6 +4,9 @@
 #ifndef MCX_AMD_INTERNAL_H
 #define MCX_AMD_INTERNAL_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #include <stdexcept>
 #include <string>
 
@@ -38,6 +41,15 @@ class GlibcRand {
   public:
     explicit GlibcRand(unsigned int seed);
     int next();
+    /* n per-thread seed words, each (rand() << 16) | (rand() << 1) |
+     * (rand() >> 14) evaluated left to right (src/mcx_core.cu:3381-3387),
+     * leaving the generator 3n outputs further on.  Large requests are split
+     * into chunks generated on host threads, each chunk's start state
+     * reached by jumping ahead (the recurrence is linear over Z/2^32). */
+    void seed_words(uint32_t* out, size_t n);
+    /* the 31 most recent values, oldest first */
+    void state(uint32_t x[31]) const;
+    void set_state(const uint32_t x[31]);
   private:
     unsigned int ring_[31];
     int pos_;

@@ -434,3 +434,18 @@ def test_batch_download_refused_beyond_host_memory():
     with pytest.raises(RuntimeError, match="host memory"):
         batch._check_host_memory([huge], 1)
     batch._check_host_memory([workloads.cube60(1000)], 1)
+
+
+def test_seed_words_match_sequential_glibc_draws():
+    """The per-thread seed words the engine uploads: large draws are made on
+    host threads from jumped-ahead generator states; they must equal one
+    sequential glibc rand() stream (the oracle's), across consecutive draws
+    of one generator (respins, gate groups)."""
+    import oracle_py
+    counts = [300_000, 5, (1 << 19) + 7, 1 << 18]
+    for seed in (1648335518, 7):
+        got = _lib.seed_words(seed, counts)
+        want = oracle_py.seedwords(seed, sum(counts))
+        assert got.shape == want.shape
+        bad = np.nonzero(got != want)[0]
+        assert bad.size == 0, (seed, bad[:5])

@@ -314,3 +314,18 @@ def test_2d_sheet_is_orientation_independent():
     sig = ma > 0.02 * ma.max()
     rel = np.abs(ma[sig] - mb[sig]) / ma[sig]
     assert np.median(rel) < 0.05, np.median(rel)
+
+
+def test_first_number_of_host_seeded_streams_is_biased():
+    """Why lane refill keeps >= 64 photons per virtual thread by default:
+    the host's seed words OR overlapping rand() values, so the first
+    xorshift128+ output of each stream is far from uniform (the reference
+    draws the same numbers); the second output is uniform."""
+    n = 1 << 16
+    w = oracle_py.seedwords(29012392, 4 * n).reshape(n, 4)
+    u = np.array([oracle_py.rng_floats(w[i], 2) for i in range(n)], dtype=np.float64)
+    e = n / 16.0
+    chi = [float((((np.histogram(u[:, k], bins=16, range=(0, 1))[0] - e) ** 2) / e).sum()) for k in range(2)]
+    assert chi[0] > 120, chi  # 15 degrees of freedom
+    assert chi[1] < 50, chi
+    assert np.mean(-np.log(u[:, 0] + 1e-30)) > 1.02  # first scattering length about 4 % long
