@@ -66,6 +66,10 @@ int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char* arg, void*
 /* nearest-neighbour x2 upsampling; d, h, wd: low-resolution dims */
 int mcx_dn_upsample_fwd(const void* x, int xcs, void* y, int ycs, int nb, int d, int h, int wd, int c, void* stream);
 int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs, int nb, int d, int h, int wd, int c, void* stream);
+/* g[v][ch] = 0 where y[v][ch] is not > 0, in place: the ReLU derivative
+ * applied once to an output gradient, after which the weight, bias and
+ * input gradients take no mask (same results as passing y as their mask) */
+int mcx_dn_relu_mask(void* g, int gcs, const void* y, int ycs, long long nvox, int c, void* stream);
 /* out = a + b */
 int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* out, int ocs, long long nvox, int c, void* stream);
 /* output head: pred[v] = b + sum_c f[v][c] w[c] + res[v*rcs] (FP32); with a

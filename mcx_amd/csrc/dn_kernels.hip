@@ -63,6 +63,14 @@ __device__ __forceinline__ Frag zero8() {
     return f;
 }
 
+/* the halves of a BF16 pair that are > 0 (1 .. 0x7F80: positive, not NaN)
+ * as 0xFFFF, the others as 0 */
+__device__ __forceinline__ uint32_t keep_mask2(uint32_t m) {
+    const uint32_t lo = ((m & 0xFFFFu) - 1u) < 0x7F80u ? 0x0000FFFFu : 0u;
+    const uint32_t hi = ((m >> 16) - 1u) < 0x7F80u ? 0xFFFF0000u : 0u;
+    return lo | hi;
+}
+
 /* dz, dy, dx of tap t = 9*(dz+1) + 3*(dy+1) + (dx+1) */
 __device__ __forceinline__ void tap_offsets(int t, int& dz, int& dy, int& dx) {
     dz = t / 9 - 1;
@@ -483,9 +491,10 @@ __global__ void __launch_bounds__(256) conv3_wgrad(const uint16_t* __restrict__ 
 
 /* dw[co][t*cin + ci] = sum_s partial[s][t][co][ci]; padding entries of dw
  * (k >= 27*cin) are set to zero */
-/* ---- weight gradient, 16 or 32 output channels, all taps per staging ---
+/* ---- weight gradient, 16 to 64 output channels, all taps per staging ---
  * Same partial sums as conv3_wgrad (partial[s][tap][co][ci]) for cout =
- * 16 * NCO and cin <= 16 * NCI.  The work unit is a segment of 32 voxels along x
+ * 16 * NCO; workgroup (s, y) takes input tiles y*NCI .. y*NCI + NCI - 1
+ * (16 channels each) of the segments of split s.  The work unit is a segment of 32 voxels along x
  * (one x-row of one z-plane).  For a segment the workgroup stages, per
  * 16-channel input tile, the 9 neighbouring input rows (dz, dy in -1..1)
  * transposed to voxel-contiguous order, in three copies shifted by dx =
@@ -508,6 +517,7 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
     __shared__ __attribute__((aligned(16))) uint16_t gs[16 * NCO * SV];   /* [co][v] */
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
+    const int c0 = (int)blockIdx.y * NCI; /* first input tile of this workgroup */
     const int nxs = (W + 31) / 32;
     const long long nseg = (long long)nb * D * H * nxs;
     const long long sb = (long long)blockIdx.x * segs;
@@ -520,77 +530,92 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
 #pragma unroll
             for (int o = 0; o < NCO; o++) acc[j][c][o] = f32x4{0.f, 0.f, 0.f, 0.f};
     /* staging items of this thread: x: (row, 8-voxel block, channel slice,
-     * dx copy) = 9 * 4 * 2 * 3 = 216; dy: (block, slice) = 4 * 2 * NCO */
+     * dx copy) = 9 * 4 * 2 * 3 = 216; dy: (block, slice) = 4 * 2 * NCO.
+     * The work items are (segment, input tile) pairs; the global loads of
+     * the next item are issued before the MFMAs of the current one, so
+     * their latency overlaps the matrix work.  A mask is applied as the
+     * output gradient arrives, which waits for its loads: callers that
+     * train apply it beforehand (mcx_dn_relu_mask) and pass none. */
     const int it = threadIdx.x;
+    const bool xthr = it < 216, gthr = it >= 216 && it < 216 + 8 * NCO;
     const int xrow = it / 24, xb = (it / 6) % 4, xsl = (it / 3) % 2, xdx = it % 3;
-    for (long long sg = sb; sg < se; sg++) {
-        long long q = sg;
+    const int gb = (it - 216) / (2 * NCO), gsl = (it - 216) % (2 * NCO);
+    const long long nitem = (se > sb ? se - sb : 0) * NCI;
+    Frag in[8];
+    auto fetch = [&](long long item) {
+        long long q = sb + item / NCI;
+        const int c = (int)(item % NCI);
         const int x0 = (int)(q % nxs) * 32; q /= nxs;
         const int yy = (int)(q % H); q /= H;
         const int zz = (int)(q % D);
         const int nn = (int)(q / D);
-        for (int c = 0; c < NCI; c++) {
-            __syncthreads(); /* the previous tile's reads are done */
-            if (it < 216) {
-                const int gz = zz + xrow / 3 - 1, gy = yy + xrow % 3 - 1;
-                const int ch = 16 * c + 8 * xsl;
-                Frag in[8];
+        if (xthr) {
+            const int gz = zz + xrow / 3 - 1, gy = yy + xrow % 3 - 1;
+            const int ch = 16 * (c0 + c) + 8 * xsl;
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int gx = x0 + 8 * xb + k + xdx - 1;
-                    in[k] = zero8();
-                    if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D && ch < cin)
-                        in[k] = load8(x + ((((long long)nn * D + gz) * H + gy) * W + gx) * xcs + ch);
-                }
-#pragma unroll
-                for (int ci = 0; ci < 8; ci++) {
-                    Frag o;
-#pragma unroll
-                    for (int k = 0; k < 8; k++) o.h[k] = in[k].h[ci];
-                    *(uint4*)(xs + ((xdx * 9 + xrow) * 16 + 8 * xsl + ci) * SV + 8 * xb) = o.u;
-                }
+            for (int k = 0; k < 8; k++) {
+                const int gx = x0 + 8 * xb + k + xdx - 1;
+                in[k] = zero8();
+                if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D && ch < cin)
+                    in[k] = load8(x + ((((long long)nn * D + gz) * H + gy) * W + gx) * xcs + ch);
             }
-            if (c == 0 && it >= 216 && it < 216 + 8 * NCO) {
-                const int gb = (it - 216) / (2 * NCO), gsl = (it - 216) % (2 * NCO);
-                Frag in[8];
+        } else if (gthr && c == 0) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int gx = x0 + 8 * gb + k;
-                    in[k] = zero8();
-                    if (gx < W) {
-                        const long long v = (((long long)nn * D + zz) * H + yy) * W + gx;
-                        in[k] = load8(dy + v * dycs + 8 * gsl);
-                        if (mask) {
-                            const Frag mk = load8(mask + v * mcs + 8 * gsl);
-#pragma unroll
-                            for (int i = 0; i < 8; i++)
-                                if (!(bf2f(mk.h[i]) > 0.f)) in[k].h[i] = 0;
-                        }
+            for (int k = 0; k < 8; k++) {
+                const int gx = x0 + 8 * gb + k;
+                in[k] = zero8();
+                if (gx < W) {
+                    const long long v = (((long long)nn * D + zz) * H + yy) * W + gx;
+                    in[k] = load8(dy + v * dycs + 8 * gsl);
+                    if (mask) {
+                        const Frag m = load8(mask + v * mcs + 8 * gsl);
+                        in[k].u.x &= keep_mask2(m.u.x);
+                        in[k].u.y &= keep_mask2(m.u.y);
+                        in[k].u.z &= keep_mask2(m.u.z);
+                        in[k].u.w &= keep_mask2(m.u.w);
                     }
                 }
-#pragma unroll
-                for (int co = 0; co < 8; co++) {
-                    Frag o;
-#pragma unroll
-                    for (int k = 0; k < 8; k++) o.h[k] = in[k].h[co];
-                    *(uint4*)(gs + (8 * gsl + co) * SV + 8 * gb) = o.u;
-                }
             }
-            __syncthreads();
-            Frag a[NCO];
+        }
+    };
+    if (nitem > 0) fetch(0);
+    for (long long item = 0; item < nitem; item++) {
+        const int c = (int)(item % NCI);
+        __syncthreads(); /* the previous item's reads are done */
+        if (xthr) {
 #pragma unroll
-            for (int o = 0; o < NCO; o++) a[o].u = *(const uint4*)(gs + (16 * o + r) * SV + 8 * g);
+            for (int ci = 0; ci < 8; ci++) {
+                Frag o;
 #pragma unroll
-            for (int j = 0; j < 7; j++) {
-                const int t = wv + 4 * j;
-                if (t < 27) {
-                    const int row = (t / 9) * 3 + (t / 3) % 3, dxi = t % 3;
-                    Frag b;
-                    b.u = *(const uint4*)(xs + ((dxi * 9 + row) * 16 + r) * SV + 8 * g);
+                for (int k = 0; k < 8; k++) o.h[k] = in[k].h[ci];
+                *(uint4*)(xs + ((xdx * 9 + xrow) * 16 + 8 * xsl + ci) * SV + 8 * xb) = o.u;
+            }
+        } else if (gthr && c == 0) {
 #pragma unroll
-                    for (int o = 0; o < NCO; o++)
-                        acc[j][c][o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[o].v, b.v, acc[j][c][o], 0, 0, 0);
-                }
+            for (int co = 0; co < 8; co++) {
+                Frag o;
+#pragma unroll
+                for (int k = 0; k < 8; k++) o.h[k] = in[k].h[co];
+                *(uint4*)(gs + (8 * gsl + co) * SV + 8 * gb) = o.u;
+            }
+        }
+        __syncthreads();
+        if (item + 1 < nitem) fetch(item + 1);
+        Frag a[NCO];
+#pragma unroll
+        for (int o = 0; o < NCO; o++) a[o].u = *(const uint4*)(gs + (16 * o + r) * SV + 8 * g);
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            const int t = wv + 4 * j;
+            if (t < 27) {
+                const int row = (t / 9) * 3 + (t / 3) % 3, dxi = t % 3;
+                Frag b;
+                b.u = *(const uint4*)(xs + ((dxi * 9 + row) * 16 + r) * SV + 8 * g);
+#pragma unroll
+                for (int o = 0; o < NCO; o++)
+#pragma unroll
+                    for (int cc = 0; cc < NCI; cc++)
+                        if (cc == c) acc[j][cc][o] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[o].v, b.v, acc[j][cc][o], 0, 0, 0);
             }
         }
     }
@@ -602,7 +627,7 @@ __global__ void __launch_bounds__(256) conv3_wgrad_rows(const uint16_t* __restri
         float* out = partial + ((long long)blockIdx.x * 27 + t) * 16 * NCO * cin;
 #pragma unroll
         for (int c = 0; c < NCI; c++) {
-            const int ci = 16 * c + r;
+            const int ci = 16 * (c0 + c) + r;
             if (ci >= cin) continue;
 #pragma unroll
             for (int o = 0; o < NCO; o++)
@@ -910,6 +935,34 @@ __global__ void add_bf16_8(const uint16_t* __restrict__ a, int acs, const uint16
     }
 }
 
+/* g[v][ch] = 0 where y[v][ch] is not > 0 (the ReLU derivative applied to
+ * an output gradient in place), 8 channels per thread */
+template <typename I>
+__global__ void relu_mask_8(uint16_t* __restrict__ g, int gcs, const uint16_t* __restrict__ y, int ycs, I nvox, int c) {
+    const int c8 = c / 8;
+    const I n = nvox * c8;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
+        const I v = i / c8;
+        const int ch = (int)(i % c8) * 8;
+        Frag fg = load8(g + v * gcs + ch);
+        const Frag fy = load8(y + v * ycs + ch);
+        fg.u.x &= keep_mask2(fy.u.x);
+        fg.u.y &= keep_mask2(fy.u.y);
+        fg.u.z &= keep_mask2(fy.u.z);
+        fg.u.w &= keep_mask2(fy.u.w);
+        store8(g + v * gcs + ch, fg);
+    }
+}
+
+__global__ void relu_mask(uint16_t* __restrict__ g, int gcs, const uint16_t* __restrict__ y, int ycs, long long nvox, int c) {
+    const long long n = nvox * c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long v = i / c;
+        const int ch = (int)(i % c);
+        if (!(bf2f(y[v * ycs + ch]) > 0.f)) g[v * gcs + ch] = 0;
+    }
+}
+
 /* out = a + b (BF16, FP32 add) */
 __global__ void add_bf16(const uint16_t* __restrict__ a, int acs, const uint16_t* __restrict__ b, int bcs,
                          uint16_t* __restrict__ out, int ocs, long long nvox, int c) {
@@ -1178,6 +1231,14 @@ extern "C" int mcx_dn_weight_flip(const void* w, int kpad, void* wt, int kpadt, 
     return dn_ret();
 }
 
+/* layers whose weight gradient runs as conv3_wgrad_rows: 16, 32 or 64
+ * output channels and rows of at least 32 voxels (shorter rows leave
+ * most of each 32-voxel segment empty; the per-tap kernel packs voxels
+ * regardless of row length and wins there) */
+static bool wgrad_rows_ok(int cin, int cout, int wd) {
+    return (cout == 16 || cout == 32 || cout == 64) && wd >= 32 && cin > 0;
+}
+
 extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dycs, const void* mask, int mcs, float* partial,
                                   int nsplit, int nb, int d, int h, int wd, int cin, int cout, void* stream) {
     if (!x || !dy || !partial || nsplit <= 0 || nsplit > 65535 || nb <= 0 || d <= 0 || h <= 0 || wd <= 0) return -22;
@@ -1193,22 +1254,24 @@ extern "C" int mcx_dn_conv3_wgrad(const void* x, int xcs, const void* dy, int dy
     if (shmem > 65536) return -22;
     hipStream_t st = (hipStream_t)stream;
     const uint16_t *xp = (const uint16_t*)x, *dp = (const uint16_t*)dy, *mp = (const uint16_t*)mask;
-    if ((cout == 16 && (cinp == 16 || cinp == 48)) || (cout == 32 && (cinp == 16 || cinp == 32))) {
-        /* all 27 taps per staging of 32-voxel x-row segments */
+    if (wgrad_rows_ok(cin, cout, wd)) {
+        /* all 27 taps per staging of 32-voxel x-row segments; input tiles
+         * split over blockIdx.y */
         const long long nseg = (long long)nb * d * h * ((wd + 31) / 32);
         const long long segs = (nseg + nsplit - 1) / nsplit;
         if (segs > 0x7fffffffLL) return -22;
-#define DN_WGR(NCI, NCO)                                                                                                 \
-    hipLaunchKernelGGL((conv3_wgrad_rows<NCI, NCO>), dim3((unsigned int)nsplit), dim3(256), 0, st, xp, xcs, dp, dycs, mp, \
-                       mcs, partial, (int)segs, nb, d, h, wd, cin)
-        if (cout == 16 && cinp == 16)
+        const int tiles = cinp / 16;
+#define DN_WGR(NCI, NCO)                                                                                        \
+    hipLaunchKernelGGL((conv3_wgrad_rows<NCI, NCO>), dim3((unsigned int)nsplit, (unsigned int)(tiles / NCI)), dim3(256), 0, \
+                       st, xp, xcs, dp, dycs, mp, mcs, partial, (int)segs, nb, d, h, wd, cin)
+        if (cout == 16)
             DN_WGR(1, 1);
-        else if (cout == 16)
-            DN_WGR(3, 1);
-        else if (cinp == 16)
+        else if (cout == 32 && tiles % 2 == 0)
+            DN_WGR(2, 2);
+        else if (cout == 32)
             DN_WGR(1, 2);
         else
-            DN_WGR(2, 2);
+            DN_WGR(1, 4);
 #undef DN_WGR
         return dn_ret();
     }
@@ -1358,6 +1421,15 @@ extern "C" int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* 
     else
         hipLaunchKernelGGL(add_bf16, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (const uint16_t*)a, acs,
                            (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
+    return dn_ret();
+}
+extern "C" int mcx_dn_relu_mask(void* g, int gcs, const void* y, int ycs, long long nvox, int c, void* stream) {
+    if (!g || !y || c <= 0 || nvox <= 0 || gcs < c || ycs < c) return -22;
+    if (vec8(c, gcs, ycs, g, y))
+        DN_LAUNCH8(relu_mask_8, fits32(nvox, gcs, ycs), nvox * c / 8, (uint16_t*)g, gcs, (const uint16_t*)y, ycs, nvox, c);
+    else
+        hipLaunchKernelGGL(relu_mask, dim3(dn_grid(nvox * c)), dim3(256), 0, (hipStream_t)stream, (uint16_t*)g, gcs,
+                           (const uint16_t*)y, ycs, nvox, c);
     return dn_ret();
 }
 #undef DN_LAUNCH8

@@ -234,15 +234,17 @@ class UNet:
     @staticmethod
     def _nsplit(nvox, cin=None, cout=None):
         """Voxel-range splits of a weight gradient (partial sums reduced by
-        a second kernel).  The 16/32-output-channel layers whose gradient
-        kernel works on 32-voxel row segments, one workgroup per split, get
-        about 8 segments per split (at most 2048); the others, one row of 27
-        workgroups per split, one split per 4096 voxels (at most 1024)."""
+        a second kernel), measured per layer of the batch-8 64^3 step: one
+        per 512 voxels (at most 2048) for the 16/32-channel layers, whose
+        gradient kernel stages 32-voxel rows for all taps, one per 256
+        voxels (at most 1024) for the wider ones, and no more than keep the
+        partial sums (splits x 27 x cout x cin floats) within 256 MB."""
+        per, cap = 256, 1024
         if cin is not None:
-            cinp = (cin + 15) // 16 * 16
-            if (cout == 16 and cinp in (16, 48)) or (cout == 32 and cinp in (16, 32)):
-                return int(min(2048, max(1, nvox // 256)))
-        return int(min(1024, max(1, nvox // 4096)))
+            if cout <= 32:
+                per, cap = 512, 2048
+            cap = min(cap, max(1, (64 << 20) // (27 * cout * cin)))
+        return int(min(cap, max(1, nvox // per)))
 
     def _nvox_of(self, name, A):
         n1, n2, n3, n4 = A["n"]
@@ -333,15 +335,18 @@ class UNet:
         S = self._dims_of(name, A)
         nv = self._nvox_of(name, A)
         ns = self._nsplit(nv, cin, cout)
-        _chk(L.mcx_dn_conv3_wgrad(_ptr(x, xoff), xcs, _ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(A["wpart"]), ns, nb,
+        # the ReLU derivative, applied to gy in place once (gy is final here
+        # and read by nothing after this layer's three gradients)
+        _chk(L.mcx_dn_relu_mask(_ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, nv, cout, st), "relu_mask " + name)
+        _chk(L.mcx_dn_conv3_wgrad(_ptr(x, xoff), xcs, _ptr(gy, gyoff), gycs, None, 0, _ptr(A["wpart"]), ns, nb,
                                   *S, cin, cout, st), "wgrad " + name)
         _chk(L.mcx_dn_wgrad_reduce(_ptr(A["wpart"]), ns, _ptr(self.g, wo), kp, cin, cout, st), "wgrad_reduce " + name)
-        _chk(L.mcx_dn_bias_grad(_ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(A["wpart"]), ns, _ptr(self.g, bo), nv, cout,
+        _chk(L.mcx_dn_bias_grad(_ptr(gy, gyoff), gycs, None, 0, _ptr(A["wpart"]), ns, _ptr(self.g, bo), nv, cout,
                                 st), "bias_grad " + name)
         if need_dx:
             kpt = kpad_of(cout)
             _chk(L.mcx_dn_weight_flip(_ptr(self.pb, wo), kp, _ptr(self.flip[name]), kpt, cin, cout, st), "flip " + name)
-            _chk(L.mcx_dn_conv3_fwd(_ptr(gy, gyoff), gycs, _ptr(y, yoff), ycs, _ptr(self.flip[name]), kpt, None,
+            _chk(L.mcx_dn_conv3_fwd(_ptr(gy, gyoff), gycs, None, 0, _ptr(self.flip[name]), kpt, None,
                                     _ptr(gx, gxoff), gxcs, nb, *S, cout, cin, 0, st), "dgrad " + name)
 
     def backward(self):

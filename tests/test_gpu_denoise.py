@@ -160,20 +160,31 @@ def test_conv3_input_gradient(cin, cout):
     _bf_close(_host_f32(gx, ref.shape), ref)
 
 
-@pytest.mark.parametrize("cin,cout,nsplit", [(8, 16, 3), (16, 16, 7), (48, 16, 1), (16, 32, 2), (32, 32, 3), (32, 64, 5),
-                                             (192, 64, 2), (128, 128, 4)])
-def test_conv3_weight_and_bias_gradient(cin, cout, nsplit):
-    rng = np.random.default_rng(cin + cout)
-    x = dn_ref.bf16_round(rng.standard_normal(SHAPE + (cin,)))
-    gy = dn_ref.bf16_round(rng.standard_normal(SHAPE + (cout,)))
-    mask = dn_ref.bf16_round(rng.standard_normal(SHAPE + (cout,)))
+ROWS = (1, 5, 6, 37)  # rows of 37 voxels: the all-taps row kernel, a full and a ragged segment
+
+
+@pytest.mark.parametrize("cin,cout,nsplit,shape", [(8, 16, 3, SHAPE), (16, 16, 7, SHAPE), (48, 16, 1, SHAPE),
+                                                   (16, 32, 2, SHAPE), (32, 32, 3, SHAPE), (32, 64, 5, SHAPE),
+                                                   (192, 64, 2, SHAPE), (128, 128, 4, SHAPE),
+                                                   (8, 16, 3, ROWS), (48, 16, 4, ROWS), (16, 32, 2, ROWS),
+                                                   (96, 32, 5, ROWS), (48, 32, 3, ROWS), (32, 64, 2, ROWS),
+                                                   (192, 64, 3, ROWS), (64, 128, 2, ROWS)])
+def test_conv3_weight_and_bias_gradient(cin, cout, nsplit, shape):
+    """SHAPE (rows of 12) runs the per-tap kernel; ROWS the all-taps row
+    kernel for 16-64 output channels (input tiles split over workgroups:
+    96 -> 32 in pairs, 48 -> 32 and 192 -> 64 one tile each)"""
+    SHAPE_ = shape
+    rng = np.random.default_rng(cin + cout + shape[3])
+    x = dn_ref.bf16_round(rng.standard_normal(SHAPE_ + (cin,)))
+    gy = dn_ref.bf16_round(rng.standard_normal(SHAPE_ + (cout,)))
+    mask = dn_ref.bf16_round(rng.standard_normal(SHAPE_ + (cout,)))
     kp = (27 * cin + 31) // 32 * 32
-    nv = int(np.prod(SHAPE))
+    nv = int(np.prod(SHAPE_))
     dx, dgy, dmask = _dev_bf16(x), _dev_bf16(gy), _dev_bf16(mask)
     part = torch.zeros(nsplit * 27 * cout * cin, dtype=torch.float32, device="cuda")
     dw = torch.full((cout * kp,), 7.0, dtype=torch.float32, device="cuda")
     db = torch.zeros(cout, dtype=torch.float32, device="cuda")
-    n, d, h, wd = SHAPE
+    n, d, h, wd = SHAPE_
     assert L.mcx_dn_conv3_wgrad(_p(dx), cin, _p(dgy), cout, _p(dmask), cout, _p(part), nsplit, n, d, h, wd, cin, cout,
                                 None) == 0
     assert L.mcx_dn_wgrad_reduce(_p(part), nsplit, _p(dw), kp, cin, cout, None) == 0
@@ -218,6 +229,29 @@ def test_pool_upsample_add(c):
     s = torch.zeros(int(np.prod(shape)), dtype=torch.int16, device="cuda")
     assert L.mcx_dn_add(_p(dx), c, _p(g), c, _p(s), c, int(np.prod(shape[:4])), c, None) == 0
     _bf_close(_host_f32(s, shape), x.astype(np.float64) + dn_ref.maxpool_bwd(gy, rarg, shape))
+
+
+@pytest.mark.parametrize("c,gcs", [(16, 48), (4, 4)])
+def test_relu_mask_in_place(c, gcs):
+    """g[v][ch] = 0 where y is not > 0 (negative, zero, -0, NaN), in a
+    channel slice of a wider buffer; c = 4 runs the scalar kernel"""
+    rng = np.random.default_rng(21 + c)
+    nv = 777
+    y = dn_ref.bf16_round(rng.standard_normal((nv, c)))
+    y[::7] = 0.0
+    y[3::11] = -0.0
+    y[5::13] = np.nan
+    y[6::17] = 1e-40  # BF16 subnormal, > 0
+    g = dn_ref.bf16_round(rng.standard_normal((nv, gcs)))
+    dg = _dev_bf16(g)
+    off = gcs - c
+    assert L.mcx_dn_relu_mask(_p(dg, off), gcs, _p(_dev_bf16(y)), c, nv, c, None) == 0
+    got = _host_f32(dg, (nv, gcs))
+    yb = dn_ref.from_bf16_bits(dn_ref.to_bf16_bits(y.astype(np.float32))).astype(np.float64)
+    want = g.astype(np.float64).copy()
+    keep = np.greater(yb, 0.0, where=~np.isnan(yb), out=np.zeros_like(yb, dtype=bool))
+    want[:, off:] = np.where(keep, want[:, off:], 0.0)
+    assert np.array_equal(got, want)
 
 
 def test_head_and_adam():
