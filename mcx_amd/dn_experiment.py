@@ -337,6 +337,46 @@ class HeadDenoiser:
         res.view(Z, Y, X)[z0:z1, y0:y1, x0:x1] = out.view(cz, cy, cx)
         return res
 
+    def run_many(self, maps, photons, boxes, nb=16):
+        """run() over the maps of a [K, nvox] device tensor, nb volumes per
+        U-Net call: the boxes must share one size (common_size); each map's
+        features are written into its slot of one batched input, the
+        network runs once per nb maps, and each prediction is pasted into a
+        copy of its map.  Returns K device maps."""
+        import torch
+        X, Y, Z = self.shape
+        K = int(maps.shape[0])
+        cx, cy, cz = (boxes[0][1] - boxes[0][0], boxes[0][3] - boxes[0][2], boxes[0][5] - boxes[0][4])
+        if any((b[1] - b[0], b[3] - b[2], b[5] - b[4]) != (cx, cy, cz) for b in boxes):
+            raise ValueError("run_many needs boxes of one size")
+        L = self.net.L
+        st = self.net._stream()
+        n = cx * cy * cz
+        outs = []
+        for k0 in range(0, K, nb):
+            ks = list(range(k0, min(K, k0 + nb)))
+            f = torch.empty((len(ks), cx, cy, cz, denoise.NFEAT), dtype=torch.bfloat16, device="cuda")
+            for j, k in enumerate(ks):
+                x0, x1, y0, y1, z0, z1 = boxes[k]
+                sub = maps[k].view(Z, Y, X)[z0:z1, y0:y1, x0:x1].contiguous().view(-1)
+                vsub = self.vol.view(Z, Y, X)[z0:z1, y0:y1, x0:x1].contiguous().view(-1)
+                denoise._chk(L.mcx_dn_features(denoise._ptr(sub), denoise._ptr(vsub), denoise._ptr(self.tab),
+                                               int(self.tab.shape[0]), denoise._ptr(f[j]), cx, cy, cz, 1,
+                                               denoise.LOG_EPS, denoise.LOG_SHIFT, denoise.LOG_SCALE, st), "features")
+            if photons:
+                f[..., 6].fill_(denoise.photon_feature(photons))
+            pred = self.net.predict(f)
+            for j, k in enumerate(ks):
+                x0, x1, y0, y1, z0, z1 = boxes[k]
+                out = torch.empty(n, dtype=torch.float32, device="cuda")
+                denoise._chk(L.mcx_dn_inv_features(denoise._ptr(pred[j * n:(j + 1) * n]), denoise._ptr(out), cx, cy,
+                                                   cz, 1, denoise.LOG_EPS, denoise.LOG_SHIFT, denoise.LOG_SCALE, st),
+                             "inv_features")
+                res = maps[k].clone()
+                res.view(Z, Y, X)[z0:z1, y0:y1, x0:x1] = out.view(cz, cy, cx)
+                outs.append(res)
+        return outs
+
     def download(self, dout):
         return dout.cpu().numpy().reshape(self.shape, order="F")
 
@@ -618,6 +658,8 @@ def main(argv=None):
     ap.add_argument("--head256-train", type=int, default=0,
                     help="also train on crops of this many full-size head volumes (head256_pairs)")
     ap.add_argument("--head256-high", type=float, default=1e8, help="photons of each full-size head's target run")
+    ap.add_argument("--head256-maps", type=int, default=4, help="input runs per full-size head")
+    ap.add_argument("--head256-crops", type=int, default=2, help="crops per input run of a full-size head")
     args = ap.parse_args(argv)
     import torch
 
@@ -657,8 +699,9 @@ def main(argv=None):
     Th = make_targets(hi, tr)
     del lo, hi
     if args.head256_train:
-        fx, ft = head256_pairs(args.head256_train, low=args.low, span=(sa, sb), high=args.head256_high,
-                               minenergy=args.minenergy, edge=args.edge, log=log)
+        fx, ft = head256_pairs(args.head256_train, maps_per_head=args.head256_maps, crops_per_map=args.head256_crops,
+                               low=args.low, span=(sa, sb), high=args.head256_high, minenergy=args.minenergy,
+                               edge=args.edge, log=log)
         if not args.photon_feature:
             fx[..., 6] = 0.0
         Xh, Th = np.concatenate([Xh, fx]), np.concatenate([Th, ft])

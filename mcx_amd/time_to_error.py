@@ -21,10 +21,10 @@ src/mcx_core.cu:4009-4012) is the plain time to 5 %.
 Batched + denoised arm: K runs of the head volume in one launch of the
 same kernel (mcx_amd_run_replicas: virtual thread idx belongs to run
 idx % K, each run with its own streams, tally and launched energy; the
-time-integrated maps are reduced on the device); each map is denoised on
-the GPU (feature kernel, U-Net, inverse kernel).  Time per volume = the
-launch's device time (tally clear, transport, reduction) / K + one
-volume's denoising.  The photon count reaching 5 % is found by a log-log
+time-integrated maps are reduced on the device); the maps are denoised on
+the GPU, 16 per U-Net call (feature kernels, U-Net, inverse kernels).
+Time per volume = (the launch's device time: tally clear, transport,
+reduction + the denoising of all K maps) / K.  The photon count reaching 5 % is found by a log-log
 fit of the denoised error and then run to confirm it.  Maps and scores
 stay on the device.
 """
@@ -133,10 +133,11 @@ def plain_arm(hc, score, counts=(1e6, 2e6, 4e6, 8e6), log=print):
             "runs_at_5pct": [{"error": e, "ms": ms} for e, ms in conf]}
 
 
-def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=print, roi_threshold=3e-6):
+def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=print, roi_threshold=3e-6, nb=16):
     """K runs of the head volume per launch (replicas), each map denoised on
-    the GPU; time per volume = (tally clear + transport + reduction) / K +
-    one volume's denoising (feature kernel, U-Net, inverse kernel)."""
+    the GPU, nb maps per U-Net call; time per volume = (tally clear +
+    transport + reduction) / K + the denoising (boxes, feature kernels,
+    U-Net, inverse kernels) / K."""
     import torch
     log("time-to-5%: denoised arm, loading the U-Net")
     net = denoise.UNet(seed=2024)
@@ -154,12 +155,10 @@ def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=prin
         boxes = dn.common_size(dn.boxes(maps))
         ev[1].record()
         box = boxes[0]
-        dn.run(maps[0], nph, box=box)
+        dn.run_many(maps[:nb], nph, boxes[:nb], nb=nb)
         torch.cuda.synchronize()
-        outs = []
         ev[2].record()
-        for k in range(K):
-            outs.append(dn.run(maps[k], nph, box=boxes[k]))
+        outs = dn.run_many(maps, nph, boxes, nb=nb)
         ev[3].record()
         torch.cuda.synchronize()
         dn_ms = (ev[0].elapsed_time(ev[1]) + ev[2].elapsed_time(ev[3])) / K
