@@ -66,6 +66,15 @@ int mcx_dn_maxpool_bwd(const void* dy, int dycs, const unsigned char* arg, void*
 /* nearest-neighbour x2 upsampling; d, h, wd: low-resolution dims */
 int mcx_dn_upsample_fwd(const void* x, int xcs, void* y, int ycs, int nb, int d, int h, int wd, int c, void* stream);
 int mcx_dn_upsample_bwd(const void* dy, int dycs, void* dx, int dxcs, int nb, int d, int h, int wd, int c, void* stream);
+/* Signal box of k maps of nvox floats each (x fastest): out[k] = max(0,
+ * the map's maximum), out zeroed by the caller; nvox a multiple of 4 and
+ * maps 16-byte aligned */
+int mcx_dn_roi_max(const float* maps, int k, long long nvox, float* out, void* stream);
+/* voxels of map k above thr[k]: rowcount[k][z*ny + y] per x row and
+ * xcount[k][x] added (zeroed by the caller); nx a multiple of 4, at most
+ * 1024 */
+int mcx_dn_roi_counts(const float* maps, int k, int nx, int ny, int nz, const float* thr, unsigned int* xcount,
+                      unsigned int* rowcount, void* stream);
 /* g[v][ch] = 0 where y[v][ch] is not > 0, in place: the ReLU derivative
  * applied once to an output gradient, after which the weight, bias and
  * input gradients take no mask (same results as passing y as their mask) */

@@ -120,7 +120,7 @@ ABI_SYMBOLS = [
     "mcx_amd_abi_layout", "mcx_amd_atomic_bench", "mcx_run_batch",
     "mcx_dn_conv3_fwd", "mcx_dn_conv3_fwd_cat", "mcx_dn_weight_flip", "mcx_dn_conv3_wgrad", "mcx_dn_wgrad_reduce", "mcx_dn_bias_grad",
     "mcx_dn_bias_grad_noalloc",
-    "mcx_dn_maxpool_fwd", "mcx_dn_maxpool_bwd", "mcx_dn_upsample_fwd", "mcx_dn_upsample_bwd", "mcx_dn_add", "mcx_dn_relu_mask",
+    "mcx_dn_maxpool_fwd", "mcx_dn_maxpool_bwd", "mcx_dn_upsample_fwd", "mcx_dn_upsample_bwd", "mcx_dn_add", "mcx_dn_relu_mask", "mcx_dn_roi_max", "mcx_dn_roi_counts",
     "mcx_dn_head_fwd", "mcx_dn_head_bwd", "mcx_dn_adam", "mcx_dn_f32_to_bf16", "mcx_dn_features",
     "mcx_dn_inv_features",
 ]
@@ -191,6 +191,8 @@ def _setup(L):
         "mcx_dn_upsample_bwd": [V, I, V, I, I, I, I, I, I, V],
         "mcx_dn_add": [V, I, V, I, V, I, LL, I, V],
         "mcx_dn_relu_mask": [V, I, V, I, LL, I, V],
+        "mcx_dn_roi_max": [V, I, LL, V, V],
+        "mcx_dn_roi_counts": [V, I, I, I, I, V, V, V, V],
         "mcx_dn_head_fwd": [V, I, V, V, V, I, V, V, V, V, I, LL, I, V],
         "mcx_dn_head_bwd": [V, I, V, V, V, I, V, I, V, LL, I, V],
         "mcx_dn_adam": [V, V, V, V, V, LL, F, F, F, F, I, V],

@@ -1072,6 +1072,66 @@ __global__ void features(const float* __restrict__ phi, const uint8_t* __restric
     }
 }
 
+/* ---- signal box of a map (the region the network runs on) -------------
+ * roi_max: out[k] = max(0, max over the nvox values of map k); grid (n, K),
+ * out zeroed by the caller (non-negative floats order as their bits). */
+__global__ void roi_max(const float* __restrict__ maps, long long nvox, float* __restrict__ out) {
+    const float* m = maps + (long long)blockIdx.y * nvox;
+    float v = 0.f;
+    const long long n4 = nvox / 4;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) {
+        const float4 q = ((const float4*)m)[i];
+        v = fmaxf(v, fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w)));
+    }
+    for (long long i = n4 * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (long long)gridDim.x * blockDim.x)
+        v = fmaxf(v, m[i]);
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomicMax((int*)out + blockIdx.y, __float_as_int(v));
+}
+
+/* roi_rows: voxels of map k above thr[k]: rowcount[k][z*ny + y] per x row
+ * and xcount[k][x] (added; zeroed by the caller).  One wave per row at a
+ * time, each lane taking 4 consecutive x per 256-voxel stretch (nx a
+ * multiple of 4, at most 1024); the row count is the popcount of the
+ * wave's ballots, the x counts stay in registers until the end. */
+__global__ void __launch_bounds__(256) roi_rows(const float* __restrict__ maps, int nx, int ny, int nz,
+                                                const float* __restrict__ thr, unsigned int* __restrict__ xcount,
+                                                unsigned int* __restrict__ rowcount) {
+    const int k = blockIdx.y, lane = threadIdx.x & 63;
+    const long long nrow = (long long)ny * nz;
+    const float t = thr[k];
+    const float* m = maps + (long long)k * nrow * nx;
+    unsigned int xc[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) xc[j] = 0u;
+    const long long wid = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long row = wid; row < nrow; row += nw) {
+        const float* r = m + row * nx;
+        unsigned int cnt = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int x = 256 * j + 4 * lane;
+            float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+            const bool in = x < nx;
+            if (in) q = *(const float4*)(r + x);
+            const bool a0 = in && q.x > t, a1 = in && q.y > t, a2 = in && q.z > t, a3 = in && q.w > t;
+            xc[4 * j] += a0;
+            xc[4 * j + 1] += a1;
+            xc[4 * j + 2] += a2;
+            xc[4 * j + 3] += a3;
+            cnt += (unsigned int)(__popcll(__ballot(a0)) + __popcll(__ballot(a1)) + __popcll(__ballot(a2)) +
+                                  __popcll(__ballot(a3)));
+        }
+        if (lane == 0) rowcount[(long long)k * nrow + row] = cnt;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int x = 256 * (j / 4) + 4 * lane + (j & 3);
+        if (x < nx && xc[j]) atomicAdd(xcount + (long long)k * nx + x, xc[j]);
+    }
+}
+
 /* fluence from a predicted scaled log fluence, back in the engine's
  * x-fastest order (with the same optional z mirror) */
 __global__ void inv_features(const float* __restrict__ pred, float* __restrict__ phi, int nx, int ny, int nz, int flipz,
@@ -1423,6 +1483,25 @@ extern "C" int mcx_dn_add(const void* a, int acs, const void* b, int bcs, void* 
                            (const uint16_t*)b, bcs, (uint16_t*)out, ocs, nvox, c);
     return dn_ret();
 }
+extern "C" int mcx_dn_roi_max(const float* maps, int k, long long nvox, float* out, void* stream) {
+    if (!maps || !out || k <= 0 || k > 65535 || nvox <= 0 || !al16(maps) || (nvox & 3)) return -22;
+    const long long per = (nvox / 4 + 255) / 256;
+    const unsigned int nbx = (unsigned int)std::min<long long>(per, 256);
+    hipLaunchKernelGGL(roi_max, dim3(nbx, (unsigned int)k), dim3(256), 0, (hipStream_t)stream, maps, nvox, out);
+    return dn_ret();
+}
+
+extern "C" int mcx_dn_roi_counts(const float* maps, int k, int nx, int ny, int nz, const float* thr,
+                                 unsigned int* xcount, unsigned int* rowcount, void* stream) {
+    if (!maps || !thr || !xcount || !rowcount || k <= 0 || k > 65535 || nx <= 0 || ny <= 0 || nz <= 0) return -22;
+    if ((nx & 3) || nx > 1024 || !al16(maps)) return -22;
+    const long long nrow = (long long)ny * nz;
+    const unsigned int nbx = (unsigned int)std::min<long long>((nrow + 3) / 4, 64);
+    hipLaunchKernelGGL(roi_rows, dim3(nbx, (unsigned int)k), dim3(256), 0, (hipStream_t)stream, maps, nx, ny, nz, thr,
+                       xcount, rowcount);
+    return dn_ret();
+}
+
 extern "C" int mcx_dn_relu_mask(void* g, int gcs, const void* y, int ycs, long long nvox, int c, void* stream) {
     if (!g || !y || c <= 0 || nvox <= 0 || gcs < c || ycs < c) return -22;
     if (vec8(c, gcs, ycs, g, y))

@@ -254,12 +254,19 @@ class HeadDenoiser:
         K = maps.shape[0]
         if not self.roi:
             return [(0, X, 0, Y, 0, Z)] * K
-        p = maps.view(K, Z, Y, X)
-        thr = p.reshape(K, -1).max(dim=1).values * self.thr
-        m = (p > thr.view(K, 1, 1, 1)).to(torch.float32)
-        # voxels above the threshold per x, y and z slice; the box spans
-        # the central 1 - 2q of them along each axis
-        proj = [m.sum(dim=2).sum(dim=1), m.sum(dim=3).sum(dim=1), m.sum(dim=3).sum(dim=2)]  # x, y, z
+        # voxels above the threshold per x, y and z slice (two passes of the
+        # signal-box kernels over the maps); the box spans the central 1 - 2q
+        # of them along each axis
+        L, st = self.net.L, self.net._stream()
+        mx = torch.zeros(K, dtype=torch.float32, device="cuda")
+        denoise._chk(L.mcx_dn_roi_max(denoise._ptr(maps), K, X * Y * Z, denoise._ptr(mx), st), "roi_max")
+        thr = mx * self.thr
+        xc = torch.zeros((K, X), dtype=torch.int32, device="cuda")
+        rc = torch.empty((K, Z * Y), dtype=torch.int32, device="cuda")
+        denoise._chk(L.mcx_dn_roi_counts(denoise._ptr(maps), K, X, Y, Z, denoise._ptr(thr), denoise._ptr(xc),
+                                         denoise._ptr(rc), st), "roi_counts")
+        r3 = rc.view(K, Z, Y).to(torch.float32)
+        proj = [xc.to(torch.float32), r3.sum(dim=1), r3.sum(dim=2)]  # x, y, z
         lims = []
         for q in proj:
             n = q.shape[1]

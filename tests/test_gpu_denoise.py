@@ -431,3 +431,24 @@ def test_trained_denoiser_beats_its_noisy_input():
         e_den.append(dx.median_error(den[i], ref, mask))
     print("median error raw %.4f denoised %.4f" % (np.median(e_low), np.median(e_den)))
     assert np.median(e_den) < 0.9 * np.median(e_low)
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(36, 5, 7), (256, 3, 2), (1024, 2, 3)])
+def test_roi_kernels_match_torch(nx, ny, nz):
+    """per-map maximum (clamped at 0) and the counts of voxels above a
+    threshold per x and per x row, against torch; map 2 is all negative"""
+    K = 3
+    g = torch.Generator(device="cpu").manual_seed(nx)
+    maps = torch.randn((K, nz, ny, nx), generator=g).cuda()
+    maps[2] = -maps[2].abs() - 1.0
+    mx = torch.zeros(K, dtype=torch.float32, device="cuda")
+    assert L.mcx_dn_roi_max(_p(maps), K, nx * ny * nz, _p(mx), None) == 0
+    want = maps.view(K, -1).max(dim=1).values.clamp(min=0.0)
+    assert torch.equal(mx, want)
+    thr = torch.tensor([0.5, -0.25, -1.5], dtype=torch.float32, device="cuda")
+    xc = torch.zeros((K, nx), dtype=torch.int32, device="cuda")
+    rc = torch.full((K, nz * ny), -1, dtype=torch.int32, device="cuda")
+    assert L.mcx_dn_roi_counts(_p(maps), K, nx, ny, nz, _p(thr), _p(xc), _p(rc), None) == 0
+    above = maps > thr.view(K, 1, 1, 1)
+    assert torch.equal(xc, above.sum(dim=(1, 2)).to(torch.int32))
+    assert torch.equal(rc, above.sum(dim=3).view(K, -1).to(torch.int32))
