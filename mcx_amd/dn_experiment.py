@@ -214,7 +214,7 @@ class HeadDenoiser:
     pooling cells are the full-volume run's); voxels outside the box keep
     the input fluence."""
 
-    def __init__(self, net, cfg, roi=True, roi_threshold=3e-6, roi_margin=16, roi_quantile=1e-3):
+    def __init__(self, net, cfg, roi=True, roi_threshold=1e-5, roi_margin=16, roi_quantile=1e-3):
         import torch
         self.net = net
         self.shape = cfg["vol"].shape

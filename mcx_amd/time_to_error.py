@@ -133,7 +133,7 @@ def plain_arm(hc, score, counts=(1e6, 2e6, 4e6, 8e6), log=print):
             "runs_at_5pct": [{"error": e, "ms": ms} for e, ms in conf]}
 
 
-def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=print, roi_threshold=3e-6, nb=16):
+def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=print, roi_threshold=1e-5, nb=16):
     """K runs of the head volume per launch (replicas), each map denoised on
     the GPU, nb maps per U-Net call; time per volume = (tally clear +
     transport + reduction) / K + the denoising (boxes, feature kernels,
