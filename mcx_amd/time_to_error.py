@@ -40,8 +40,8 @@ MINENERGY = 1e-4
 TARGET = 0.05
 
 
-def _head(n=256):
-    hc = workloads.head256(nphoton=1_000_000, n=n, quiet=True, minenergy=MINENERGY)
+def _head(n=256, seed=99):
+    hc = workloads.head256(nphoton=1_000_000, seed=seed, n=n, quiet=True, minenergy=MINENERGY)
     hc["autopilot"] = 1
     return hc
 
@@ -194,14 +194,16 @@ def denoised_arm(hc, score, weights, K=64, scales=(1.25e5, 2.5e5, 5e5), log=prin
             "error_at": conf["error"], "denoise_ms": conf["denoise_ms"], "launch_ms": conf["launch_ms"]}
 
 
-def measure(weights=WEIGHTS, K=64, log=print):
-    """The whole comparison; returns the dict reported by bench.py."""
+def measure(weights=WEIGHTS, K=64, log=print, seed=99):
+    """The whole comparison on the head volume of this seed (99: the
+    benchmark's; the U-Net was trained on heads of seeds 6000 on); returns
+    the dict reported by bench.py."""
     if not os.path.exists(weights):
         return {"skipped": "no trained weights at %s" % os.path.relpath(weights)}
     os.environ.setdefault("MCX_AMD_VERBOSE", "1")  # phase lines of every batched call on stderr
     t0 = time.perf_counter()
     log("time-to-5%: building the head volume")
-    hc = _head()
+    hc = _head(seed=seed)
     ref, mask = reference(hc, log=log)
     score = _Scorer(ref, mask)
     # both arms run the transport kernel the fastest way for their small
@@ -215,11 +217,42 @@ def measure(weights=WEIGHTS, K=64, log=print):
         den = denoised_arm(hc, score, weights, K=K, log=log)
     finally:
         mcx_amd.set_refill(old)
-    out = {"workload": "head256: 256^3 five-label head volume (seed 99), 0.5 mm voxels, 20 gates x 0.25 ns, "
-                       "Russian roulette below %g; reference 10 x 1e8 photons" % MINENERGY,
+    out = {"workload": "head256: 256^3 five-label head volume (seed %d), 0.5 mm voxels, 20 gates x 0.25 ns, "
+                       "Russian roulette below %g; reference 10 x 1e8 photons" % (seed, MINENERGY),
            "reliable_voxels": int(mask.sum()), "plain": plain, "batched_denoised": den,
            "ratio": den["ms_per_volume_to_5pct"] / plain["ms_to_5pct"], "target_ratio": 0.1,
            "seconds": time.perf_counter() - t0}
     log("time-to-5%%: plain %.1f ms, batched+denoised %.1f ms per volume: ratio %.3f"
         % (plain["ms_to_5pct"], den["ms_per_volume_to_5pct"], out["ratio"]))
     return out
+
+
+def main(argv=None):
+    """python -m mcx_amd.time_to_error --seeds 900,901 --out r.json: the
+    comparison on other head volumes than the benchmark's (held out from
+    training as well), one line per head."""
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--seeds", default="99")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--k", type=int, default=64)
+    args = ap.parse_args(argv)
+    rows = []
+    for sd in [int(v) for v in args.seeds.split(",")]:
+        r = measure(K=args.k, seed=sd, log=lambda m: print(m, file=sys.stderr, flush=True))
+        row = {"seed": sd, "reliable_voxels": r["reliable_voxels"], "ratio": r["ratio"],
+               "plain_photons": r["plain"]["photons_for_5pct"], "plain_ms": r["plain"]["ms_to_5pct"],
+               "denoised_photons": r["batched_denoised"]["photons_for_5pct"],
+               "denoised_ms_per_volume": r["batched_denoised"]["ms_per_volume_to_5pct"],
+               "denoised_error": r["batched_denoised"]["error_at"]}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+        with open(args.out, "w") as f:
+            json.dump(rows, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
