@@ -337,6 +337,25 @@ static void finish_replicas(Config* cfg, ReplicaReq& R, DevMem& dgenergy, DevMem
     R.stats[3] = wall_ms;
 }
 
+/* Pinned host staging for the per-thread seed words, one per host thread,
+ * grown as needed and kept until the process ends (refilled launches draw
+ * millions of seed words, whose upload from pageable memory takes several
+ * times longer).  Every copy from it is waited for before the next launch
+ * writes it, and before it is replaced. */
+static uint32_t* pinned_seed_buffer(size_t nwords) {
+    thread_local uint32_t* buf = nullptr;
+    thread_local size_t cap = 0;
+    if (nwords > cap) {
+        if (buf) HIP_CHECK(hipHostFree(buf));
+        buf = nullptr;
+        cap = 0;
+        const size_t n = std::max<size_t>(nwords, (size_t)1 << 16);
+        HIP_CHECK(hipHostMalloc((void**)&buf, n * sizeof(uint32_t), hipHostMallocDefault));
+        cap = n;
+    }
+    return buf;
+}
+
 static void run_impl(Config* cfg, GPUInfo* gpu) {
     check_supported(cfg);
     int threadid = 0;
@@ -628,7 +647,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     }
 
     GlibcRand rnd((unsigned int)(cfg->seed > 0 ? cfg->seed + threadid : (int)time(0) + threadid));
-    std::vector<uint32_t> seeds((size_t)nthread * 4);
+    const size_t nseedw = (size_t)nthread * 4;
+    uint32_t* seeds = pinned_seed_buffer(nseedw);
     std::vector<float> field(cfg->issave2pt ? fieldlen : 0, 0.f);
     std::vector<float> fieldacc((cfg->issave2pt && abs(cfg->respin) > 1) ? fieldlen : 0, 0.f);
     std::vector<float> pdet;
@@ -663,8 +683,8 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
             }
             if (savedet) HIP_CHECK(hipMemsetAsync(ddet.p, 0, ddet.bytes, st.s));
             HIP_CHECK(hipMemsetAsync(dcount.p, 0, dcount.bytes, st.s));
-            rnd.seed_words(seeds.data(), seeds.size());
-            HIP_CHECK(hipMemcpyAsync(dseeds.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, st.s));
+            rnd.seed_words(seeds, nseedw);
+            HIP_CHECK(hipMemcpyAsync(dseeds.p, seeds, nseedw * 4, hipMemcpyHostToDevice, st.s));
             HIP_CHECK(hipMemcpyAsync(dkp.p, &kp, sizeof(kp), hipMemcpyHostToDevice, st.s));
             HIP_CHECK(hipStreamSynchronize(st.s));
 
