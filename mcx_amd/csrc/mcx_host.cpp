@@ -57,8 +57,8 @@ static int g_replayindex = 0;
  * voxel, deposits are few per step, the atomics do not bound the kernel
  * and the blocks only add instructions */
 static int g_tblock = 2;
-/* autopilot lanes run several virtual threads each (KParam::refill) */
-/* lane refill: 0 off, 1 (default) at least kMinStreamPhotons photons per
+/* lane refill (autopilot lanes run several virtual threads each,
+ * KParam::refill): 0 off, 1 (default) at least kMinStreamPhotons photons per
  * virtual thread and at most kDefaultRefill virtual threads per lane, 2
  * about one photon per virtual thread */
 static int g_refill = 1;
@@ -450,7 +450,7 @@ static void run_impl(Config* cfg, GPUInfo* gpu) {
     const unsigned int tgates = g_rep ? 1u : maxgate;
     const size_t planelen = nvox * tgates * blocks;
     const size_t fieldlen = planelen * (rf ? 2 : 1);
-    /* replicas: K independent runs in one launch, one set of gate planes each */
+    /* replicas: K independent runs in one launch, one time-integrated plane each */
     const unsigned int nrep = g_rep ? (unsigned int)g_rep->k : 1u;
     if (g_rep) {
         if (replay || cfg->issavedet) MCX_THROW(-1, "replicas are forward runs without detected photons");
