@@ -79,7 +79,7 @@ def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 
         return hc
 
     # the volumes are built on host threads while the GPU runs the previous ones
-    pool = ThreadPoolExecutor(4)
+    pool = ThreadPoolExecutor(max(2, min(12, (os.cpu_count() or 4) - 2)))
     for h, hc in enumerate(pool.map(make, range(nheads))):
         X, Y, Z = hc["vol"].shape
         u = float(hc.get("unitinmm", 1.0))
