@@ -55,6 +55,13 @@ def head_crop_volume(seed, n=128, crop=64):
     return cfg
 
 
+def _training_head(seed, high, minenergy):
+    """one full-size training head (a separate process builds it)"""
+    hc = workloads.head256(nphoton=int(high), seed=int(seed), n=256, quiet=True, minenergy=minenergy)
+    hc["autopilot"] = 1
+    return hc
+
+
 def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 40.0), high=1e8, minenergy=1e-4,
                   edge=64, seed=6000, log=print):
     """Training pairs from full-size head volumes (workloads.head256's
@@ -66,21 +73,21 @@ def head256_pairs(nheads, maps_per_head=4, crops_per_map=2, low=1e5, span=(3.0, 
     z so the source side comes first).  Runs are single-run replica launches
     with the maps left on the device; only the crops come back.  Returns
     (features [n, edge, edge, edge, 8] float32, targets [n, edge^3])."""
-    from concurrent.futures import ThreadPoolExecutor
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    from functools import partial
     import torch
     import mcx_amd
     g = np.random.default_rng(seed)
     feats, targs = [], []
     t0 = time.perf_counter()
-
-    def make(h):
-        hc = workloads.head256(nphoton=int(high), seed=int(seed + h), n=256, quiet=True, minenergy=minenergy)
-        hc["autopilot"] = 1
-        return hc
-
-    # the volumes are built on host threads while the GPU runs the previous ones
-    pool = ThreadPoolExecutor(max(2, min(12, (os.cpu_count() or 4) - 2)))
-    for h, hc in enumerate(pool.map(make, range(nheads))):
+    # the volumes are built in fresh host processes (numpy only; the
+    # generator holds the GIL for much of its work) while the GPU runs the
+    # previous ones
+    pool = ProcessPoolExecutor(max(2, min(12, (os.cpu_count() or 4) - 2)),
+                               mp_context=multiprocessing.get_context("spawn"))
+    make = partial(_training_head, high=float(high), minenergy=float(minenergy))
+    for h, hc in enumerate(pool.map(make, [int(seed + k) for k in range(nheads)])):
         X, Y, Z = hc["vol"].shape
         u = float(hc.get("unitinmm", 1.0))
         tgt = torch.empty((1, X * Y * Z), dtype=torch.float32, device="cuda")
