@@ -40,6 +40,13 @@ ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 
 VALU_PEAK_TFLOPS = 157.3  # FP32 vector peak of MI355X
+# VALU issue roof: a wave64 vector instruction holds a 16-lane SIMD for 4
+# cycles; 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4.0
+# SQ_INSTS_VALU per wave loop iteration of the transport kernel, keyed as
+# TRAFFIC_B_PER_PHOTON (profiles/transport_sphere100_valu_issue.md: 6.370e11
+# per 1e8-photon launch over 8.79e8 wave loop iterations)
+VALU_INSTS_PER_WAVE_ITER = {("sphere100", 2, True): 725.0}
 
 
 def _valu(steps_pp, scat_pp, photons, seconds):
@@ -49,6 +56,19 @@ def _valu(steps_pp, scat_pp, photons, seconds):
     tflops = photons * (30.0 * steps_pp + 60.0 * scat_pp) / seconds / 1e12
     return {"achieved": round(tflops, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tflops / VALU_PEAK_TFLOPS, 5), "convention": "30 flop/step + 60 flop/scatter"}
+
+
+def _issue(per_iter, wave_iterations, seconds):
+    """VALU wave-instructions issued per second against the issue roof: the
+    profiled instruction count per wave loop iteration times this run's
+    wave loop iterations (the kernel's own counters)."""
+    if not per_iter or not wave_iterations:
+        return None
+    rate = per_iter * wave_iterations / seconds
+    return {"achieved": round(rate / 1e9, 1), "peak": round(VALU_ISSUE_PEAK / 1e9, 1),
+            "unit": "G VALU wave-instructions/s", "frac": round(rate / VALU_ISSUE_PEAK, 4),
+            "source": "%.0f SQ_INSTS_VALU per wave loop iteration (profiles/transport_sphere100_valu_issue.md) "
+                      "x this run's wave loop iterations" % per_iter}
 
 
 def _atomic_roof(mode, nelem):
@@ -261,7 +281,8 @@ def main():
     bytes_pp = steps_pp * 1.0 + dep_pp * dep_bytes + reclen * 4.0 * detected_w / nph
     k_ms = sum(kernel_ms) / len(kernel_ms)
     achieved = bytes_pp * nph / (k_ms * 1e-3) / 1e9
-    tpp = TRAFFIC_B_PER_PHOTON.get((args.workload, args.tally, bool(stats_w.get("tally_blocks", 0))))
+    key = (args.workload, args.tally, bool(stats_w.get("tally_blocks", 0)))
+    tpp = TRAFFIC_B_PER_PHOTON.get(key)
     # the tally's atomic instruction: FP64 no-return (modes 2, 3), FP32
     # returning (1), FP32 no-return + rare returning shadow moves (0)
     nvox = int(cfg["vol"].size)
@@ -317,6 +338,7 @@ def main():
                        "frac": round(arate / aroof, 4),
                        "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},
             "valu": _valu(steps_pp, stats_w["nscatter"] / nph, nph, k_ms * 1e-3),
+            "issue": _issue(VALU_INSTS_PER_WAVE_ITER.get(key), stats_w.get("wave_iterations"), k_ms * 1e-3),
         },
     }
     if not args.no_cpu:
