@@ -44,9 +44,9 @@ VALU_PEAK_TFLOPS = 157.3  # FP32 vector peak of MI355X
 # cycles; 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4.0
 # SQ_INSTS_VALU per wave loop iteration of the transport kernel, keyed as
-# TRAFFIC_B_PER_PHOTON (profiles/transport_sphere100_valu_issue.md: 6.370e11
-# per 1e8-photon launch over 8.79e8 wave loop iterations)
-VALU_INSTS_PER_WAVE_ITER = {("sphere100", 2, True): 725.0}
+# TRAFFIC_B_PER_PHOTON (profiles/transport_sphere100_valu_issue.md: 6.251e11
+# per 1e8-photon launch over 8.80e8 wave loop iterations)
+VALU_INSTS_PER_WAVE_ITER = {("sphere100", 2, True): 710.0}
 
 
 def _valu(steps_pp, scat_pp, photons, seconds):

@@ -183,8 +183,6 @@ void hot_box_geometry(const Config* cfg, uint32_t cells, KParam& kp) {
     kp.hot_bx = b[0];
     kp.hot_by = b[1];
     kp.hot_bz = b[2];
-    kp.hot_lo = (o[2] * kp.ny + o[1]) * kp.nx + o[0];
-    kp.hot_span = ((o[2] + b[2] - 1) * kp.ny + (o[1] + b[1] - 1)) * kp.nx + (o[0] + b[0] - 1) - kp.hot_lo;
     div_magic(kp.dimxy, &kp.hot_mxy, &kp.hot_shxy);
     div_magic(kp.nx, &kp.hot_mx, &kp.hot_shx);
 }

@@ -85,7 +85,6 @@ struct KParam {
      * global atomics on one address serialise.  hot_cells == 0: off. */
     uint32_t hot_cells, hot_off, hot_g;
     uint32_t hot_x0, hot_y0, hot_z0, hot_bx, hot_by, hot_bz;
-    uint32_t hot_lo, hot_span;    /* linear voxel index of the box's first voxel, last minus first */
     /* division of a voxel index by dimxy and by nx (DivMagic) */
     uint32_t hot_mxy, hot_shxy, hot_mx, hot_shx;
     /* Blocked FP64 tally (tblock = 1; single-simulation FP64 tally): the

@@ -1068,15 +1068,21 @@ struct Lane {
     }
 
     /* `main`: the voxel-exit deposit of the step loop (at most one per
-     * loop iteration), which may wait in the lane's block */
-    __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight, bool main = false) {
+     * loop iteration), which may wait in the lane's block.  (cx, cy, cz):
+     * the caller's coordinates of `vox` if it has them (the step loop's
+     * voxel left); they are used only when they index `vox` inside the
+     * grid, otherwise the index is divided back into coordinates. */
+    __device__ __forceinline__ void deposit(uint32_t vox, int tshift, float weight, bool main = false,
+                                            uint32_t cx = ~0u, uint32_t cy = 0u, uint32_t cz = 0u) {
         ndep++;
-        if (TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells && (uint32_t)tshift < kp.hot_g && vox - kp.hot_lo <= kp.hot_span) {
-            const uint32_t tz = __umulhi(vox, kp.hot_mxy);
-            const uint32_t z = (tz + ((vox - tz) >> (kp.hot_shxy & 0xffu))) >> (kp.hot_shxy >> 8), r = vox - z * kp.dimxy;
-            const uint32_t ty = __umulhi(r, kp.hot_mx);
-            const uint32_t y = (ty + ((r - ty) >> (kp.hot_shx & 0xffu))) >> (kp.hot_shx >> 8);
-            const uint32_t lx = r - y * kp.nx - kp.hot_x0, ly = y - kp.hot_y0, lz = z - kp.hot_z0;
+        const bool hotbox = TALLY != MCX_TALLY_F32_SHADOW && kp.hot_cells;
+        uint32_t x = cx, y = cy, z = cz;
+        if (hotbox || (BLOCKED && kp.tblock)) {
+            if (!(cx < kp.nx && cy < kp.ny && cz < kp.nz && cz * kp.dimxy + cy * kp.nx + cx == vox))
+                vox_xyz(kp, vox, x, y, z);
+        }
+        if (hotbox && (uint32_t)tshift < kp.hot_g) {
+            const uint32_t lx = x - kp.hot_x0, ly = y - kp.hot_y0, lz = z - kp.hot_z0;
             if (lx < kp.hot_bx && ly < kp.hot_by && lz < kp.hot_bz) {
                 extern __shared__ F4 lds[];
                 const uint32_t c = (((uint32_t)tshift * kp.hot_bz + lz) * kp.hot_by + ly) * kp.hot_bx + lx;
@@ -1118,8 +1124,6 @@ struct Lane {
             }
         }
         if (BLOCKED && kp.tblock) {
-            uint32_t x, y, z;
-            vox_xyz(kp, vox, x, y, z);
             const size_t c = block_cell(kp, x, y, z, (uint32_t)tshift);
             if (main) {
                 if ((uint32_t)(c >> 3) == bopen) {
@@ -1303,6 +1307,7 @@ struct Lane {
             if (SPLIT) hitintf = (svsplit && testint) ? ray_plane(len, slen) : false;
             p.x = p.x + len * v.x; p.y = p.y + len * v.y; p.z = p.z + len * v.z;
             const bool stop = (slen == f.x) || (SPLIT && hitintf);
+            const uint32_t ox = (uint32_t)fd0, oy = (uint32_t)fd1, oz = (uint32_t)fd2;
             if (fd3 == 0) fd0 += stop ? 0 : (v.x > 0.f ? 1 : -1);
             if (fd3 == 1) fd1 += stop ? 0 : (v.y > 0.f ? 1 : -1);
             if (fd3 == 2) fd2 += stop ? 0 : (v.z > 0.f ? 1 : -1);
@@ -1356,8 +1361,9 @@ struct Lane {
                         if (kp.replaydet == -1) tshift += ((glb(kp.replaydetid)[gid] & 0xFFFF) - 1) * (int)kp.maxgate;
                     } else if (kp.outputtype == otL)
                         weight = w0 * f.z;
-                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight, true);
-                    if (kp.rfshift && kp.omega > 0.f && fabsf(imag) > 0.f) deposit(idx1dold, tshift + kp.rfshift, imag);
+                    if (fabsf(weight) > 0.f) deposit(idx1dold, tshift, weight, true, ox, oy, oz);
+                    if (kp.rfshift && kp.omega > 0.f && fabsf(imag) > 0.f)
+                        deposit(idx1dold, tshift + kp.rfshift, imag, false, ox, oy, oz);
                 }
                 w0 = p.w;
                 f.z = 0.f;
