@@ -30,12 +30,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per photon of the transport kernel on the sphere100 workload with
-# the FP64 tally and the LDS hot-cell box: (FETCH_SIZE x 2.000 + WRITE_SIZE x
-# 1.000) per launch / 1e7 photons, scale factors from the FP64->FP32 merge
-# kernel as a copy of known size; keyed by (workload, tally mode, blocked
-# tally) (profiles/transport_sphere100_1e7_counters.md,
-# profiles/transport_sphere100_blocked_tally.md)
-TRAFFIC_B_PER_PHOTON = {("sphere100", 2, False): 8254.0, ("sphere100", 2, True): 5544.0}
+# the FP64 tally and the LDS hot-cell box: (FETCH_SIZE x f + WRITE_SIZE x w)
+# per launch / photons, scale factors f, w from the FP64->FP32 merge kernel
+# of the same run as a copy of known size; keyed by (workload, tally mode,
+# blocked tally).  Linear tally: 1e7 photons, f = 2.000, w = 1.000
+# (profiles/transport_sphere100_1e7_counters.md); blocked: the default
+# 1e8-photon command, f = 0.859, w = 1.000
+# (profiles/transport_sphere100_traffic_1e8.md)
+TRAFFIC_B_PER_PHOTON = {("sphere100", 2, False): 8254.0, ("sphere100", 2, True): 5533.0}
 ATOMIC_SIZES_MB = (4, 64, 400, 2048)
 
 
@@ -331,9 +333,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": round(tpp * nph) if tpp else None,
-            "traffic_source": "FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per photon from the committed 1e7-photon "
-                              "counter profile of this tally layout (profiles/transport_sphere100_blocked_tally.md) x "
-                              "photons per launch; a constant, not measured in this run" if tpp else None,
+            "traffic_source": "FETCH_SIZE and WRITE_SIZE per photon, scaled by the merge kernel as a copy of "
+                              "known size, from the committed counter profile of this tally layout "
+                              "(profiles/transport_sphere100_traffic_1e8.md for the blocked tally) x photons per "
+                              "launch; a constant, not measured in this run" if tpp else None,
             "atomic": {"achieved": round(arate / 1e9, 3), "peak": round(aroof / 1e9, 3), "unit": "G atomics/s",
                        "frac": round(arate / aroof, 4),
                        "peak_source": "scattered-atomic microbenchmark over the tally's size, same run"},
