@@ -80,7 +80,7 @@ int  mcx_amd_run_replicas(Config* cfg, GPUInfo* gpu, int k, float* cw, double* e
  * than idling until the slowest lane of their wave is done.  The results
  * are those of a launch with one lane per virtual thread and the same
  * nthread.  mode 0: off (R = 1); 1 (default): R = photons per lane / 64,
- * at most 4, so every virtual thread keeps at least 64 photons; 2: R =
+ * at most 8, so every virtual thread keeps at least 64 photons; 2: R =
  * photons per lane, at most 32 (about one photon per virtual thread, the
  * shortest tail).  The first random number of every stream the host
  * seeding makes is biased (mean 0.4914; the reference draws the same),

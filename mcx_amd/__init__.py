@@ -45,7 +45,7 @@ def set_tally_blocks(mode):
 def set_refill(mode):
     """Autopilot lanes run several virtual threads each, taking the next
     unstarted one when theirs is done.  0/False: off; 1/True (default): at
-    least 64 photons per virtual thread, at most 4 virtual threads per lane;
+    least 64 photons per virtual thread, at most 8 virtual threads per lane;
     2 or "fast": about one photon per virtual thread, at most 32 per lane
     (shortest tail; each stream's biased first random number then weighs on
     every photon or two).  Returns the previous mode."""

@@ -64,7 +64,7 @@ static int g_tblock = 2;
 static int g_refill = 1;
 /* virtual threads per lane at most under refill (each holds 24 bytes of
  * seeds and energies) */
-static const unsigned int kMaxRefill = 32, kDefaultRefill = 4;
+static const unsigned int kMaxRefill = 32, kDefaultRefill = 8;
 /* The first xorshift128+ output of a stream seeded with the host's seed
  * words is not uniform (the words OR overlapping rand() values, so their
  * bits are biased towards 1): its mean is 0.4914 and -log of it averages

@@ -1,7 +1,7 @@
 """Lane refill under autopilot: the launched lanes run R times as many
 virtual threads, each lane taking the next unstarted one when its own has
 launched all its photons (mode 1, the default: at least 64 photons per
-virtual thread, at most 4 per lane; mode 2: about one photon per virtual
+virtual thread, at most 8 per lane; mode 2: about one photon per virtual
 thread, at most 32 per lane).  Every virtual thread keeps its RNG stream,
 photon count and energy sums, so a refilled launch must reproduce the
 oracle with the same number of virtual threads bit for bit (energies,
@@ -82,7 +82,7 @@ def test_default_refill_keeps_64_photons_per_thread():
     cfg = workloads.sphere100(100_000_000)
     r, st = _run(cfg, refill=1)
     lanes, nt = int(st["lanes"]), int(st["nthread"])
-    assert nt == max(1, min(4, 100_000_000 // (64 * lanes))) * lanes and nt > lanes
+    assert nt == max(1, min(8, 100_000_000 // (64 * lanes))) * lanes and nt > lanes
     assert 100_000_000 / nt >= 64
     small, ss = _run(workloads.sphere100(4_000_000), refill=1)
     assert ss["nthread"] == ss["lanes"]  # 15 photons per lane: no refill
