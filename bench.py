@@ -328,6 +328,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
+            # the roof the kernel runs at: the vector instruction issue rate
+            # (`issue` below); the HBM and atomic fractions show the headroom
+            "binding": "issue" if key in VALU_INSTS_PER_WAVE_ITER else None,
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
